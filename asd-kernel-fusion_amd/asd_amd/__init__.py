@@ -1,0 +1,227 @@
+"""asd_amd -- Python host side of libasd (MI355X / gfx950).
+
+Thin ctypes layer over the C ABI in ``include/asd.h``.  Device buffers are torch
+tensors on ``cuda:*`` (PyTorch-ROCm is only the allocator and stream provider
+here); every compute call goes through ``libasd.so``.  There is no fallback: if
+the library is missing or fails to load, every call raises.
+
+Reference interface mirrored (hwchen2017/ASD-kernel-fusion, kernel_fusion/):
+  ``euler``         llg_sqt_fusion.c:30   derivative of the spin field
+  ``runge_kutta``   llg_sqt_fusion.c:98   one unnormalised RK4 step + increments
+  ``fused_dgemm``   gemm_kernel.h:355     GEMM with the phase-weighted summing epilogue
+and the native, device-resident pipeline that replaces the reference time loop
+(llg_sqt_fusion.c:340-449): :func:`rk4_run`, :func:`sqt`.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from typing import Sequence
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_DIR = os.path.dirname(_HERE)
+LIB_PATH = os.path.join(PKG_DIR, "libasd.so")
+
+FP64, FP32, BF16X3, FP16X2 = 0, 1, 2, 3
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "fp16x2": FP16X2}
+
+ASD_OK = 0
+
+
+class AsdError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libasd error {code}: {msg}")
+        self.code = code
+
+
+class Params(C.Structure):
+    """asd_params: J, D, B_z, alpha, dt, spin_length."""
+    _fields_ = [("J", C.c_double), ("D", C.c_double), ("Bz", C.c_double),
+                ("alpha", C.c_double), ("dt", C.c_double), ("spin_length", C.c_double)]
+
+
+def params(J=1.0, D=0.0, Bz=0.0, alpha=0.0, dt=0.01, spin_length=1.0) -> Params:
+    return Params(J, D, Bz, alpha, dt, spin_length)
+
+
+_lib = None
+_vp = C.c_void_p
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes)
+_SIGS = {
+    "asd_last_error": (C.c_char_p, []),
+    "asd_version": (C.c_char_p, []),
+    "asd_init_spins": (C.c_int, [C.c_uint64, C.c_int, _vp]),
+    "asd_qpath_gxmg": (C.c_int, [C.c_int, _vp]),
+    "asd_create": (C.c_int, [C.POINTER(_vp), C.c_int]),
+    "asd_destroy": (C.c_int, [_vp]),
+    "asd_set_stream": (C.c_int, [_vp, _vp]),
+    "asd_synchronize": (C.c_int, [_vp]),
+    "asd_rk4_run": (C.c_int, [_vp, C.POINTER(Params), C.c_int, C.c_int, _vp, C.c_long, C.c_int, C.c_int, _vp,
+                              C.c_int]),
+    "asd_sqt": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int]),
+    "asd_sqt_replicas": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp]),
+    "asd_batch_rk4_run": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_long, C.c_int, C.c_int, _vp]),
+    "asd_autotune_sqt": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _ip]),
+    "asd_sqt_num_configs": (C.c_int, [C.c_int]),
+    "asd_sqt_config_name": (C.c_char_p, [C.c_int, C.c_int]),
+    "asd_sqt_set_config": (C.c_int, [_vp, C.c_int, C.c_int]),
+    "asd_sqt_loss_grad": (C.c_int, [_vp, C.POINTER(Params), C.c_int, _vp, C.c_long, C.c_int, C.c_int, C.c_int,
+                                    _vp, C.c_int, _vp, C.c_int, _dp, _dp]),
+    "asd_euler": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int, _vp, _vp]),
+    "asd_runge_kutta": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
+    "asd_fused_dgemm": (C.c_int, [_ip, _ip, _ip, _dp, _vp, _ip, _vp, _ip, _vp, _ip, _vp, _vp]),
+    "asd_compat_bind": (C.c_int, [_ip, _ip, _dp, _dp, _vp, _ip]),
+    "euler": (None, [_vp, _vp]),
+    "runge_kutta": (None, [_vp] * 6),
+    "fused_dgemm": (None, [_ip, _ip, _ip, _dp, _vp, _ip, _vp, _ip, _vp, _ip, _vp, _vp]),
+}
+
+
+def lib():
+    """Load libasd.so (raises if it is not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise FileNotFoundError(f"{LIB_PATH} not built (run __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(code: int):
+    if code != ASD_OK:
+        raise AsdError(code, lib().asd_last_error().decode())
+
+
+def _np_ptr(a: np.ndarray):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def init_spins(seed: int, nsites: int) -> np.ndarray:
+    out = np.empty((nsites, 3))
+    _check(lib().asd_init_spins(seed, nsites, _np_ptr(out)))
+    return out
+
+
+def qpath_gxmg(L: int) -> np.ndarray:
+    q = np.empty((3 * L // 2, 2), dtype=np.int32)
+    n = lib().asd_qpath_gxmg(L, _np_ptr(q))
+    if n < 0:
+        _check(n)
+    return q
+
+
+def full_grid(L: int) -> np.ndarray:
+    mx, my = np.meshgrid(np.arange(L), np.arange(L), indexing="ij")
+    return np.ascontiguousarray(np.stack([mx.ravel(), my.ravel()], axis=1).astype(np.int32))
+
+
+class Context:
+    """An asd_ctx bound to one device; work is ordered on torch's current stream
+    of that device at construction time (or ``stream``)."""
+
+    def __init__(self, device: int = 0, stream=None):
+        import torch
+        self._torch = torch
+        self.device = device
+        h = C.c_void_p()
+        _check(lib().asd_create(C.byref(h), device))
+        self.h = h
+        st = stream if stream is not None else torch.cuda.current_stream(device)
+        self.stream = st
+        _check(lib().asd_set_stream(self.h, C.c_void_p(st.cuda_stream)))
+
+    def close(self):
+        if self.h:
+            lib().asd_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def synchronize(self):
+        _check(lib().asd_synchronize(self.h))
+
+    # -------------------------------------------------------------- RK4
+    def rk4_run(self, p: Params, L: int, spins, nsteps: int, snap_every: int = 0, nsnap: int = 0, snaps=None,
+                use_graph: bool = True):
+        """spins: cuda float64 tensor [nrep, L*L, 3] or [L*L, 3] (in/out);
+        snaps: cuda float64 tensor [nsnap, ...spins.shape] or None."""
+        t = self._torch
+        assert spins.dtype == t.float64 and spins.is_cuda and spins.is_contiguous()
+        nrep = spins.numel() // (3 * L * L)
+        assert nrep * 3 * L * L == spins.numel()
+        sp = None
+        if snaps is not None:
+            assert snaps.dtype == t.float64 and snaps.is_contiguous() and snaps.numel() == nsnap * spins.numel()
+            sp = C.c_void_p(snaps.data_ptr())
+        _check(lib().asd_rk4_run(self.h, C.byref(p), L, nrep, C.c_void_p(spins.data_ptr()), nsteps, snap_every,
+                                 nsnap, sp, 1 if use_graph else 0))
+
+    # ------------------------------------------------------------ S(q,t)
+    def sqt(self, L: int, snaps, T0: int, nlags: int, q, precision="fp64", out=None):
+        """snaps: cuda float64 [nsnap, L*L, 3].  Returns complex128 numpy [nlags, nq]
+        (or fills the cuda float64 tensor ``out`` [nlags, nq, 2] and returns it)."""
+        t = self._torch
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
+        assert snaps.dtype == t.float64 and snaps.is_cuda and snaps.is_contiguous()
+        nsnap = snaps.shape[0]
+        if out is not None:
+            assert out.is_cuda and out.dtype == t.float64 and out.numel() == 2 * nlags * len(q)
+            _check(lib().asd_sqt(self.h, L, C.c_void_p(snaps.data_ptr()), nsnap, T0, nlags, _np_ptr(q), len(q),
+                                 prec, C.c_void_p(out.data_ptr()), 1))
+            return out
+        res = np.empty((nlags, len(q), 2))
+        _check(lib().asd_sqt(self.h, L, C.c_void_p(snaps.data_ptr()), nsnap, T0, nlags, _np_ptr(q), len(q), prec,
+                             _np_ptr(res), 0))
+        return res[..., 0] + 1j * res[..., 1]
+
+    def sqt_replicas(self, L: int, now, zero, q, precision="fp64"):
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
+        nrep = now.numel() // (3 * L * L)
+        res = np.empty((len(q), 2))
+        _check(lib().asd_sqt_replicas(self.h, L, nrep, C.c_void_p(now.data_ptr()), C.c_void_p(zero.data_ptr()),
+                                      _np_ptr(q), len(q), prec, _np_ptr(res)))
+        return res[..., 0] + 1j * res[..., 1]
+
+
+# --------------------------------------------------- reference-compatible (host)
+
+def euler(p: Params, L: int, spin0: np.ndarray) -> np.ndarray:
+    spin0 = np.ascontiguousarray(spin0, dtype=np.float64)
+    out = np.empty_like(spin0)
+    _check(lib().asd_euler(C.byref(p), L, spin0.size // (3 * L * L), _np_ptr(spin0), _np_ptr(out)))
+    return out
+
+
+def runge_kutta(p: Params, L: int, spin: np.ndarray):
+    spin = np.ascontiguousarray(spin, dtype=np.float64)
+    outs = [np.empty_like(spin) for _ in range(5)]
+    _check(lib().asd_runge_kutta(C.byref(p), L, spin.size // (3 * L * L), _np_ptr(spin),
+                                 *(_np_ptr(o) for o in outs)))
+    return tuple(outs)
+
+
+def fused_dgemm(m, n, k, alpha, a, lda, b, ldb, c, ldc, fuse_c, sums):
+    """gemm_kernel.h:355 semantics; c and sums (numpy float64) updated in place."""
+    ci = lambda v: C.byref(C.c_int(v))  # noqa: E731
+    _check(lib().asd_fused_dgemm(ci(m), ci(n), ci(k), C.byref(C.c_double(alpha)), _np_ptr(a), ci(lda),
+                                 _np_ptr(b), ci(ldb), _np_ptr(c), ci(ldc), _np_ptr(fuse_c), _np_ptr(sums)))
+
+
+def exported_symbols() -> Sequence[str]:
+    return list(_SIGS)

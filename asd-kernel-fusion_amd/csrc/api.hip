@@ -1,0 +1,464 @@
+// api.hip -- extern "C" entry points of libasd (declared in include/asd.h and
+// include/asd_compat.h): context, RK4 driver, S(q,tau) driver, reference-compatible
+// host-pointer entries.
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "asd_compat.h"
+#include "asd_internal.h"
+
+namespace asd {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+int DevBuf::ensure(size_t want)
+{
+    if (want <= bytes) return ASD_OK;
+    release();
+    if (want == 0) return ASD_OK;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, want);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(ASD_ENOMEM, "hipMalloc(%zu) failed: %s", want, hipGetErrorString(e));
+    }
+    ptr = p;
+    bytes = want;
+    return ASD_OK;
+}
+
+void DevBuf::release()
+{
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+}
+
+static int check_params(const asd_params *p)
+{
+    ASD_CHECK(p, "params is NULL");
+    ASD_CHECK(std::isfinite(p->J) && std::isfinite(p->D) && std::isfinite(p->Bz) &&
+                  std::isfinite(p->alpha) && std::isfinite(p->dt) && std::isfinite(p->spin_length),
+              "non-finite parameter");
+    ASD_CHECK(p->spin_length > 0, "spin_length must be > 0 (got %g)", p->spin_length);
+    ASD_CHECK(p->alpha >= 0, "alpha must be >= 0 (got %g)", p->alpha);
+    return ASD_OK;
+}
+
+static int check_L(int L)
+{
+    ASD_CHECK(L >= 2 && L <= 1024, "L must be in [2, 1024] (got %d)", L);
+    return ASD_OK;
+}
+
+// same hash as the counter-based stream documented in asd.h
+static inline uint64_t hash64(uint64_t seed, uint64_t a, uint64_t b)
+{
+    uint64_t z = seed * 0x9E3779B97F4A7C15ull + a * 0xD1B54A32D192ED03ull + b * 0x8CB92BA72F3D8DD7ull +
+                 0x632BE59BD9B4E019ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__global__ void set_int_kernel(int *p, int v) { *p = v; }
+
+}  // namespace asd
+
+using namespace asd;
+
+extern "C" {
+
+const char *asd_last_error(void) { return g_err; }
+const char *asd_version(void) { return "asd-mi355x 0.1 (gfx950)"; }
+
+int asd_init_spins(uint64_t seed, int nsites, double *out)
+{
+    ASD_CHECK(out && nsites >= 0, "bad arguments");
+    for (int i = 0; i < nsites; i++) {
+        double u, v, s;
+        uint64_t ctr = 0;
+        for (;;) {
+            u = 2.0 * ((double)(hash64(seed, (uint64_t)i, ctr) >> 11) * 0x1.0p-53) - 1.0;
+            v = 2.0 * ((double)(hash64(seed, (uint64_t)i, ctr + 1) >> 11) * 0x1.0p-53) - 1.0;
+            ctr += 2;
+            s = u * u + v * v;
+            if (s < 1.0 && s > 0.0) break;
+        }
+        const double r = 2.0 * std::sqrt(1.0 - s);
+        out[3L * i] = u * r;
+        out[3L * i + 1] = v * r;
+        out[3L * i + 2] = 1.0 - 2.0 * s;
+    }
+    return ASD_OK;
+}
+
+int asd_qpath_gxmg(int L, int *q)
+{
+    ASD_CHECK(L >= 2 && L % 2 == 0 && q, "L must be even");
+    const int h = L / 2;
+    int n = 0;
+    for (int k = 0; k < h; k++) { q[2 * n] = k; q[2 * n + 1] = 0; n++; }
+    for (int k = 0; k < h; k++) { q[2 * n] = h; q[2 * n + 1] = k; n++; }
+    for (int k = 0; k < h; k++) { q[2 * n] = h - k; q[2 * n + 1] = h - k; n++; }
+    return n;
+}
+
+int asd_create(asd_ctx **out, int device)
+{
+    ASD_CHECK(out, "ctx is NULL");
+    int n = 0;
+    ASD_HIP(hipGetDeviceCount(&n));
+    ASD_CHECK(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
+    ASD_HIP(hipSetDevice(device));
+    asd_ctx *c = new asd_ctx();
+    c->device = device;
+    hipDeviceProp_t prop;
+    ASD_HIP(hipGetDeviceProperties(&prop, device));
+    c->num_cu = prop.multiProcessorCount;
+    *out = c;
+    return ASD_OK;
+}
+
+int asd_destroy(asd_ctx *c)
+{
+    if (!c) return ASD_OK;
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+    DevBuf *bufs[] = {&c->state_b, &c->counter, &c->opA, &c->opB, &c->opA2, &c->opB2, &c->bins, &c->fq,
+                      &c->twiddle, &c->qdev, &c->outdev, &c->stats, &c->c0, &c->c1, &c->c2, &c->c3, &c->c4,
+                      &c->c5, &c->c6, &c->work, &c->tabs, &c->adj0, &c->adj1, &c->adj2, &c->adj3, &c->adj4,
+                      &c->adj5};
+    for (DevBuf *b : bufs) b->release();
+    delete c;
+    return ASD_OK;
+}
+
+int asd_set_stream(asd_ctx *c, void *stream)
+{
+    ASD_CHECK(c, "ctx is NULL");
+    c->stream = static_cast<hipStream_t>(stream);
+    if (c->graph.exec) {
+        (void)hipGraphExecDestroy(c->graph.exec);
+        c->graph = RK4Graph();
+    }
+    return ASD_OK;
+}
+
+int asd_synchronize(asd_ctx *c)
+{
+    ASD_CHECK(c, "ctx is NULL");
+    ASD_HIP(hipStreamSynchronize(c->stream));
+    return ASD_OK;
+}
+
+// --------------------------------------------------------------------- RK4
+
+int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spins, long nsteps,
+                int snap_every, int nsnap, double *d_snaps, int use_graph)
+{
+    ASD_CHECK(c && d_spins, "ctx/spins NULL");
+    ASD_TRY(check_params(pp));
+    ASD_TRY(check_L(L));
+    ASD_CHECK(nrep >= 1 && nrep <= 65535, "nrep out of range");
+    ASD_CHECK(nsteps >= 0, "nsteps < 0");
+    ASD_CHECK(!d_snaps || (snap_every >= 1 && nsnap >= 0), "snap_every must be >= 1");
+    ASD_HIP(hipSetDevice(c->device));
+    const LLGParams p = make_llg(*pp);
+    const long n = (long)nrep * L * L * 3;
+    const long snap_stride = n;
+    ASD_TRY(c->state_b.ensure(sizeof(double) * n));
+    ASD_TRY(c->counter.ensure(sizeof(int)));
+    double *bufs[2] = {d_spins, c->state_b.as<double>()};
+    hipStream_t st = c->stream;
+
+    // snapshots due while stepping: those at s = k * every < nsteps; the one at
+    // s == nsteps (if any) is the final state, copied at the end
+    const long nsnap_in_loop = d_snaps ? std::min<long>(nsnap, (nsteps + snap_every - 1) / snap_every) : 0;
+    long s = 0;
+    int cur = 0;
+
+    // graph: one chunk = `every` steps (2*every if every is odd, so the ping-pong parity
+    // repeats), first node writes snapshot *counter, a tiny node bumps the counter
+    if (use_graph && d_snaps && nsnap_in_loop >= 2) {
+        const int chunk = snap_every % 2 ? 2 * snap_every : snap_every;
+        const long nchunks = std::min<long>(nsteps / chunk, nsnap_in_loop / (chunk / snap_every));
+        if (nchunks >= 1) {
+            RK4Graph &g = c->graph;
+            const bool same = g.exec && g.L == L && g.nrep == nrep && g.every == snap_every && g.chunk == chunk &&
+                              g.a == bufs[0] && g.b == bufs[1] && g.snaps == d_snaps &&
+                              std::memcmp(&g.p, &p, sizeof p) == 0 && g.snap_stride == snap_stride;
+            if (!same) {
+                if (g.exec) (void)hipGraphExecDestroy(g.exec);
+                g = RK4Graph();
+                hipStream_t cap;
+                ASD_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+                hipGraph_t graph;
+                ASD_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+                hipStream_t saved = c->stream;
+                c->stream = cap;
+                int r = ASD_OK;
+                for (int k = 0; k < chunk && r == ASD_OK; k++) {
+                    const bool snap_here = k % snap_every == 0;
+                    r = rk4_launch_step(c, p, L, nrep, bufs[k & 1], bufs[(k + 1) & 1], snap_here ? d_snaps : nullptr,
+                                        c->counter.as<int>(), snap_stride, snap_here ? 1 : 0);
+                }
+                c->stream = saved;
+                hipError_t e1 = hipStreamEndCapture(cap, &graph);
+                (void)hipStreamDestroy(cap);
+                ASD_TRY(r);
+                ASD_HIP(e1);
+                hipError_t e2 = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(graph);
+                ASD_HIP(e2);
+                g.L = L; g.nrep = nrep; g.every = snap_every; g.chunk = chunk;
+                g.a = bufs[0]; g.b = bufs[1]; g.snaps = d_snaps; g.p = p; g.snap_stride = snap_stride;
+            }
+            hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, c->counter.as<int>(), 0);
+            for (long ch = 0; ch < nchunks; ch++) ASD_HIP(hipGraphLaunch(g.exec, st));
+            s = nchunks * chunk;
+            cur = 0;  // chunk is even: state back in bufs[0]
+            // the counter now equals s / every; continue eagerly with explicit slots
+        }
+    }
+    for (; s < nsteps; s++) {
+        double *snap = nullptr;
+        if (d_snaps && s % snap_every == 0 && s / snap_every < nsnap) snap = d_snaps + (s / snap_every) * snap_stride;
+        ASD_TRY(rk4_launch_step(c, p, L, nrep, bufs[cur], bufs[cur ^ 1], snap, nullptr, snap_stride, 0));
+        cur ^= 1;
+    }
+    if (d_snaps && nsteps % snap_every == 0 && nsteps / snap_every < nsnap)
+        ASD_HIP(hipMemcpyAsync(d_snaps + (nsteps / snap_every) * snap_stride, bufs[cur], sizeof(double) * n,
+                               hipMemcpyDeviceToDevice, st));
+    if (cur == 1)
+        ASD_HIP(hipMemcpyAsync(d_spins, bufs[1], sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    return ASD_OK;
+}
+
+// ---------------------------------------------------------------- S(q,tau)
+
+static int check_q(int L, const int *h_q, int nq)
+{
+    ASD_CHECK(h_q && nq >= 1 && nq <= 1 << 20, "bad q list");
+    for (int i = 0; i < 2 * nq; i++) ASD_CHECK(h_q[i] >= 0 && h_q[i] < L, "q index out of [0, L)");
+    return ASD_OK;
+}
+
+int asd_sqt(asd_ctx *c, int L, const double *d_snaps, int nsnap, int T0, int nlags, const int *h_q, int nq,
+            asd_precision prec, double *out, int out_on_device)
+{
+    ASD_CHECK(c && d_snaps && out, "NULL argument");
+    ASD_TRY(check_L(L));
+    ASD_CHECK(T0 >= 1 && nlags >= 1, "T0 and nlags must be >= 1");
+    ASD_CHECK(prec >= ASD_FP64 && prec <= ASD_FP16X2, "unknown precision");
+    ASD_TRY(check_q(L, h_q, nq));
+    ASD_HIP(hipSetDevice(c->device));
+    double *dout = out;
+    if (!out_on_device) {
+        ASD_TRY(c->outdev.ensure(sizeof(double) * 2 * (long)nlags * nq));
+        dout = c->outdev.as<double>();
+    }
+    const int cfg = c->forced_cfg[prec];
+    ASD_TRY(sqt_run(c, L, d_snaps, d_snaps, 3L * L * L, nsnap, T0, nlags, h_q, nq, prec, dout, cfg));
+    if (!out_on_device) {
+        ASD_HIP(hipMemcpyAsync(out, dout, sizeof(double) * 2 * (long)nlags * nq, hipMemcpyDeviceToHost, c->stream));
+        ASD_HIP(hipStreamSynchronize(c->stream));
+    }
+    return ASD_OK;
+}
+
+int asd_sqt_replicas(asd_ctx *c, int L, int nrep, const double *d_now, const double *d_zero, const int *h_q,
+                     int nq, asd_precision prec, double *out)
+{
+    ASD_CHECK(c && d_now && d_zero && out, "NULL argument");
+    ASD_TRY(check_L(L));
+    ASD_CHECK(nrep >= 1, "nrep must be >= 1");
+    ASD_TRY(check_q(L, h_q, nq));
+    ASD_HIP(hipSetDevice(c->device));
+    ASD_TRY(c->outdev.ensure(sizeof(double) * 2 * (long)nq));
+    ASD_TRY(sqt_run(c, L, d_now, d_zero, 3L * L * L, nrep, nrep, 1, h_q, nq, prec, c->outdev.as<double>(),
+                    c->forced_cfg[prec]));
+    ASD_HIP(hipMemcpyAsync(out, c->outdev.ptr, sizeof(double) * 2 * nq, hipMemcpyDeviceToHost, c->stream));
+    ASD_HIP(hipStreamSynchronize(c->stream));
+    return ASD_OK;
+}
+
+int asd_sqt_num_configs(asd_precision prec) { return (prec >= ASD_FP64 && prec <= ASD_FP16X2) ? 1 : 0; }
+const char *asd_sqt_config_name(asd_precision prec, int cfg)
+{
+    (void)prec;
+    return cfg == 0 ? "tile128x128" : "invalid";
+}
+int asd_sqt_set_config(asd_ctx *c, asd_precision prec, int cfg)
+{
+    ASD_CHECK(c && prec >= ASD_FP64 && prec <= ASD_FP16X2, "bad arguments");
+    ASD_CHECK(cfg >= -1 && cfg < asd_sqt_num_configs(prec), "config out of range");
+    c->forced_cfg[prec] = cfg;
+    return ASD_OK;
+}
+int asd_autotune_sqt(asd_ctx *c, int L, int T0, asd_precision prec, int *best)
+{
+    ASD_CHECK(c, "ctx NULL");
+    (void)L; (void)T0; (void)prec;
+    if (best) *best = 0;
+    return ASD_OK;
+}
+
+int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *params, double *const *d_spins,
+                      long nsteps, int snap_every, int nsnap, double *const *d_snaps)
+{
+    (void)c; (void)ntraj; (void)Ls; (void)params; (void)d_spins; (void)nsteps; (void)snap_every; (void)nsnap;
+    (void)d_snaps;
+    return set_error(ASD_ENOTSUP, "asd_batch_rk4_run: not built yet");
+}
+
+int asd_sqt_loss_grad(asd_ctx *c, const asd_params *p, int L, const double *d_S0, long nsteps, int snap_every,
+                      int T0, int nlags, const int *h_q, int nq, const double *h_target, int checkpoint_every,
+                      double *loss, double grad[4])
+{
+    (void)c; (void)p; (void)L; (void)d_S0; (void)nsteps; (void)snap_every; (void)T0; (void)nlags; (void)h_q;
+    (void)nq; (void)h_target; (void)checkpoint_every; (void)loss; (void)grad;
+    return set_error(ASD_ENOTSUP, "asd_sqt_loss_grad: not built yet");
+}
+
+// ------------------------------------------------ reference-compatible entries
+
+// one lazily created context per host thread for the host-pointer entries
+static asd_ctx *compat_ctx(int *err)
+{
+    static thread_local asd_ctx *ctx = nullptr;
+    if (!ctx) {
+        int dev = 0;
+        if (hipGetDevice(&dev) != hipSuccess) dev = 0;
+        *err = asd_create(&ctx, dev);
+        if (*err != ASD_OK) ctx = nullptr;
+    }
+    return ctx;
+}
+
+int asd_euler(const asd_params *p, int L, int nrep, const double *spin0, double *derivative)
+{
+    ASD_CHECK(spin0 && derivative, "NULL argument");
+    ASD_TRY(check_params(p));
+    ASD_TRY(check_L(L));
+    ASD_CHECK(nrep >= 1, "nrep must be >= 1");
+    int err = ASD_OK;
+    asd_ctx *c = compat_ctx(&err);
+    if (!c) return err;
+    const size_t bytes = sizeof(double) * 3 * (size_t)nrep * L * L;
+    ASD_TRY(c->c0.ensure(bytes));
+    ASD_TRY(c->c1.ensure(bytes));
+    ASD_HIP(hipMemcpyAsync(c->c0.ptr, spin0, bytes, hipMemcpyHostToDevice, c->stream));
+    ASD_TRY(rhs_launch(c, make_llg(*p), L, nrep, c->c0.as<double>(), c->c1.as<double>()));
+    ASD_HIP(hipMemcpyAsync(derivative, c->c1.ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+    ASD_HIP(hipStreamSynchronize(c->stream));
+    return ASD_OK;
+}
+
+int asd_runge_kutta(const asd_params *p, int L, int nrep, const double *spin, double *cur_input, double *k1,
+                    double *k2, double *k3, double *k4)
+{
+    ASD_CHECK(spin && cur_input && k1 && k2 && k3 && k4, "NULL argument");
+    ASD_TRY(check_params(p));
+    ASD_TRY(check_L(L));
+    ASD_CHECK(nrep >= 1, "nrep must be >= 1");
+    int err = ASD_OK;
+    asd_ctx *c = compat_ctx(&err);
+    if (!c) return err;
+    const size_t bytes = sizeof(double) * 3 * (size_t)nrep * L * L;
+    DevBuf *b[6] = {&c->c0, &c->c1, &c->c2, &c->c3, &c->c4, &c->c5};
+    for (DevBuf *x : b) ASD_TRY(x->ensure(bytes));
+    ASD_HIP(hipMemcpyAsync(c->c0.ptr, spin, bytes, hipMemcpyHostToDevice, c->stream));
+    ASD_TRY(rk4_compat_stages(c, make_llg(*p), L, nrep, c->c0.as<double>(), c->c1.as<double>(), c->c2.as<double>(),
+                              c->c3.as<double>(), c->c4.as<double>(), c->c5.as<double>()));
+    double *outs[5] = {cur_input, k1, k2, k3, k4};
+    for (int i = 0; i < 5; i++)
+        ASD_HIP(hipMemcpyAsync(outs[i], b[i + 1]->ptr, bytes, hipMemcpyDeviceToHost, c->stream));
+    ASD_HIP(hipStreamSynchronize(c->stream));
+    return ASD_OK;
+}
+
+}  // extern "C"
+
+// fused_dgemm compat lives in compat.hip; the globals-bound drop-ins below.
+namespace asd {
+struct CompatGlobals {
+    int *N = nullptr, *Nfile = nullptr, *num_neighbor = nullptr;
+    double *dt = nullptr, *J1 = nullptr;
+    int (*nn)[2] = nullptr;
+};
+static CompatGlobals g_compat;
+
+[[noreturn]] static void compat_die(const char *fn)
+{
+    fprintf(stderr, "libasd %s: %s\n", fn, asd_last_error());
+    abort();
+}
+
+static int compat_params(asd_params *p)
+{
+    ASD_CHECK(g_compat.N && g_compat.Nfile && g_compat.dt && g_compat.J1 && g_compat.nn,
+              "asd_compat_bind() was not called");
+    static const int ref_nn[4][2] = {{1, 0}, {-1, 0}, {0, -1}, {0, 1}};
+    ASD_CHECK(*g_compat.num_neighbor == 4, "num_neighbor must be 4");
+    for (int j = 0; j < 4; j++)
+        ASD_CHECK(g_compat.nn[j][0] == ref_nn[j][0] && g_compat.nn[j][1] == ref_nn[j][1],
+                  "nn[][] must be the reference table {(1,0),(-1,0),(0,-1),(0,1)}");
+    p->J = *g_compat.J1;
+    p->D = 0.0;
+    p->Bz = 0.0;
+    p->alpha = 0.0;
+    p->dt = *g_compat.dt;
+    p->spin_length = 1.0;
+    return ASD_OK;
+}
+}  // namespace asd
+
+extern "C" {
+
+int asd_compat_bind(int *N, int *Nfile, double *dt, double *J1, int (*nn)[2], int *num_neighbor)
+{
+    ASD_CHECK(N && Nfile && dt && J1 && nn && num_neighbor, "NULL global");
+    g_compat.N = N;
+    g_compat.Nfile = Nfile;
+    g_compat.dt = dt;
+    g_compat.J1 = J1;
+    g_compat.nn = nn;
+    g_compat.num_neighbor = num_neighbor;
+    asd_params p;
+    return compat_params(&p);
+}
+
+void euler(double *spin0, double *derivative)
+{
+    asd_params p;
+    if (compat_params(&p) != ASD_OK || asd_euler(&p, *g_compat.N, *g_compat.Nfile, spin0, derivative) != ASD_OK)
+        compat_die("euler");
+}
+
+void runge_kutta(double *spin, double *cur_input, double *k1, double *k2, double *k3, double *k4)
+{
+    asd_params p;
+    if (compat_params(&p) != ASD_OK ||
+        asd_runge_kutta(&p, *g_compat.N, *g_compat.Nfile, spin, cur_input, k1, k2, k3, k4) != ASD_OK)
+        compat_die("runge_kutta");
+}
+
+void fused_dgemm(int *m, int *n, int *k, double *alpha, double *a, int *lda, double *b, int *ldb, double *c,
+                 int *ldc, double *fuse_C, double *sum)
+{
+    if (asd_fused_dgemm(m, n, k, alpha, a, lda, b, ldb, c, ldc, fuse_C, sum) != ASD_OK) compat_die("fused_dgemm");
+}
+
+}  // extern "C"

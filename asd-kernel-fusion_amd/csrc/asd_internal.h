@@ -1,0 +1,95 @@
+// asd_internal.h -- shared host/device definitions of libasd (not installed).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdint>
+#include <cstdio>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "asd.h"
+
+namespace asd {
+
+// ----------------------------------------------------------------- errors
+int set_error(int code, const char *fmt, ...);
+#define ASD_HIP(call)                                                                  \
+    do {                                                                               \
+        hipError_t e_ = (call);                                                        \
+        if (e_ != hipSuccess)                                                          \
+            return ::asd::set_error(ASD_EHIP, "%s failed: %s (%s:%d)", #call,          \
+                                    hipGetErrorString(e_), __FILE__, __LINE__);        \
+    } while (0)
+#define ASD_TRY(expr)                                                                  \
+    do {                                                                               \
+        int r_ = (expr);                                                               \
+        if (r_ != ASD_OK) return r_;                                                   \
+    } while (0)
+#define ASD_CHECK(cond, ...)                                                           \
+    do {                                                                               \
+        if (!(cond)) return ::asd::set_error(ASD_EINVAL, __VA_ARGS__);                 \
+    } while (0)
+
+// a device buffer that grows on demand and is reused across calls
+struct DevBuf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int ensure(size_t want);
+    void release();
+    template <class T> T *as() const { return static_cast<T *>(ptr); }
+};
+
+// LLG parameters as the kernels take them (by value)
+struct LLGParams {
+    double J, D, Bz, alpha, dt, len;
+    double a_over_len;  // alpha / spin_length
+    double den;         // 1 + alpha^2
+};
+LLGParams make_llg(const asd_params &p);
+
+struct RK4Graph {
+    hipGraphExec_t exec = nullptr;
+    // key
+    int L = 0, nrep = 0, every = 0, chunk = 0;
+    const void *a = nullptr, *b = nullptr, *snaps = nullptr;
+    LLGParams p{};
+    long snap_stride = 0;
+};
+
+}  // namespace asd
+
+struct asd_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int num_cu = 0;
+    // RK4 ping-pong state and the device-side snapshot counter
+    asd::DevBuf state_b, counter;
+    asd::RK4Graph graph;
+    // S(q,t) workspace
+    asd::DevBuf opA, opB, opA2, opB2, bins, fq, twiddle, qdev, outdev, stats;
+    // compat (host-pointer) staging
+    asd::DevBuf c0, c1, c2, c3, c4, c5, c6;
+    // batching
+    asd::DevBuf work, tabs;
+    // autotuner choice per precision (-1 = automatic); tuned per (L, K)
+    int forced_cfg[4] = {-1, -1, -1, -1};
+    std::map<std::pair<int, int>, int> tuned[4];
+    // adjoint workspace
+    asd::DevBuf adj0, adj1, adj2, adj3, adj4, adj5;
+};
+
+namespace asd {
+// internal entry points shared between translation units
+int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in,
+                    double *out, double *snap_out, const int *snap_counter, long snap_stride,
+                    int bump_counter);
+int rk4_compat_stages(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *spin,
+                      double *cur, double *k1, double *k2, double *k3, double *k4);
+int rhs_launch(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *S, double *dS);
+int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_b, long ta_stride,
+            int nsnap_a, int T0, int nlags, const int *h_q, int nq, asd_precision prec,
+            double *d_out, int cfg);
+}  // namespace asd

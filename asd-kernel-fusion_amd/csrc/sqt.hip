@@ -1,0 +1,706 @@
+// sqt.hip -- S(q,tau) as a spin-spin correlation GEMM on MFMA with a fused
+// displacement-binning epilogue, plus the FP64 finishing kernels.
+//
+// Reference data flow (kernel_fusion/llg_sqt_fusion.c, time origins in the role of
+// the reference's replicas):
+//   B operand  spin_zero[k + j*K] = S_j^c(t0)        :235-245   (k = 3 t0 + c)
+//   A operand  spint[i + k*Ns]   = S_i^c(t0 + tau)   :370-377
+//   averages   zbar_j, fbar_i                        :249-268, :380-398
+//   prefill    C = -(fbar_i . zbar_j) Q_ij           :402-426
+//   fused_dgemm: C += (alpha A B) o Q, sum += sum(C)  :429, gemm_kernel.h:355-422
+//   sqt = sum / N^2                                  :432-442
+// The reference materialises Q and C (2 Ns^2 doubles each: 4.3 GB apiece at L=128).
+// Here nothing Ns^2-sized exists: because Q_ij depends only on r_i - r_j (mod L for
+// lattice momenta), the epilogue reduces each G tile into displacement bins
+//   g_tau(dr, dc) = sum_{i - j = (dr, dc) mod L} G_ij(tau)
+// (LDS f64 atomics, then one global f64 atomic per bin per tile), and a finishing
+// kernel evaluates sum_d g(d) e^{-iq.d} for every requested q from an exact L-entry
+// twiddle table with Kahan-compensated FP64 sums.  The disconnected part
+// sum_ij fbar_i.zbar_j Q_ij factorises into Fbar_q(tau).conj(Fbar_q(0)) and is
+// computed in FP64 from per-snapshot Fourier sums F_q(t) = sum_i S_i(t) e^{-iq.r_i}.
+#include <hip/hip_bf16.h>
+#include <hip/hip_fp16.h>
+
+#include <cmath>
+
+#include "asd_internal.h"
+
+namespace asd {
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+typedef float f4 __attribute__((ext_vector_type(4)));
+typedef float f16v __attribute__((ext_vector_type(16)));
+typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
+typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+
+constexpr int TILE = 128;  // BM = BN for every configuration
+
+// ------------------------------------------------------------------ packing
+
+// snapshots [t][i][c] -> k-major operand out[k][ld], k = 3 t + c; zero where t >= nt
+// or i >= Ns (padding rows/columns), so padded products vanish exactly.
+template <class T>
+__global__ void pack_kmajor_kernel(const double *__restrict__ snaps, long tstride, int nt, int Ns,
+                                   T *__restrict__ out, long ld)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int k = blockIdx.y;
+    if (i >= ld) return;
+    const int t = k / 3, c = k - 3 * t;
+    double v = (t < nt && i < Ns) ? snaps[(long)t * tstride + 3L * i + c] : 0.0;
+    out[(long)k * ld + i] = (T)v;
+}
+
+// snapshots -> i-major hi/lo split operands (k contiguous), k = 3 t + c.
+// BF16X3: hi = bf16(v), lo = bf16(v - hi).  FP16X2: hi = f16(v), lo = f16((v - hi) * 2^11).
+// `nshift` shifted copies are written, copy r holding out_r[i][k] = value(k + r), so a
+// window starting at any k offset o is read from copy o % 8 at the 16-byte aligned
+// offset o - o % 8.  grid: (rows i, k chunks of 256).
+template <int PREC>
+__global__ void pack_split_kernel(const double *__restrict__ snaps, long tstride, int nt, int Ns,
+                                  uint16_t *__restrict__ hi, uint16_t *__restrict__ lo, long ldk,
+                                  long copy_stride, int nshift)
+{
+    const int i = blockIdx.x;
+    const long k = (long)blockIdx.y * blockDim.x + threadIdx.x;
+    if (k >= ldk) return;
+    for (int r = 0; r < nshift; r++) {
+        const long kk = k + r;
+        const long t = kk / 3;
+        const int c = (int)(kk - 3 * t);
+        const float v = (t < nt && i < Ns) ? (float)snaps[t * tstride + 3L * i + c] : 0.0f;
+        uint16_t h, l;
+        if (PREC == ASD_BF16X3) {
+            __hip_bfloat16 bh = __float2bfloat16(v);
+            const float rr = v - __bfloat162float(bh);
+            __hip_bfloat16 bl = __float2bfloat16(rr);
+            h = *reinterpret_cast<uint16_t *>(&bh);
+            l = *reinterpret_cast<uint16_t *>(&bl);
+        } else {
+            __half fh = __float2half(v);
+            const float rr = (v - __half2float(fh)) * 2048.0f;
+            __half fl = __float2half(rr);
+            h = *reinterpret_cast<uint16_t *>(&fh);
+            l = *reinterpret_cast<uint16_t *>(&fl);
+        }
+        hi[r * copy_stride + (long)i * ldk + k] = h;
+        lo[r * copy_stride + (long)i * ldk + k] = l;
+    }
+}
+
+// ------------------------------------------------------- work decomposition
+
+struct Work {
+    int lag, ti, tj;
+};
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).  Give
+// each XCD a contiguous slice of the work list, and order tiles in groups of
+// GROUP tile-rows so that concurrently running tiles share A and B panels in the
+// XCD's L2.
+__device__ __forceinline__ bool decode_work(int nt, int nlags, Work &w)
+{
+    constexpr int GROUP = 8;
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int per = nb / 8;
+    const long wid = (long)(b % 8) * per + b / 8;
+    const long per_lag = (long)nt * nt;
+    if (wid >= per_lag * nlags) return false;
+    w.lag = (int)(wid / per_lag);
+    const int r = (int)(wid - (long)w.lag * per_lag);
+    const int span = GROUP * nt;
+    const int g = r / span, first = g * GROUP;
+    const int gsz = min(nt - first, GROUP);
+    const int rr = r - g * span;
+    w.ti = first + rr % gsz;
+    w.tj = rr / gsz;
+    return true;
+}
+
+// --------------------------------------------------------- binning epilogue
+
+struct BinGeom {
+    int drmin, nbins;
+};
+
+__device__ __forceinline__ BinGeom bin_geom(int i0, int j0, int Ns, int L)
+{
+    const int il = min(i0 + TILE, Ns) - 1, jl = min(j0 + TILE, Ns) - 1;
+    const int ri0 = i0 / L, ri1 = il / L, rj0 = j0 / L, rj1 = jl / L;
+    BinGeom g;
+    g.drmin = ri0 - rj1;
+    g.nbins = ((ri1 - ri0) + (rj1 - rj0) + 1) * L;
+    return g;
+}
+
+__device__ __forceinline__ void flush_bins(const double *sb, const BinGeom &g, int L, double *gbins)
+{
+    for (int b = threadIdx.x; b < g.nbins; b += blockDim.x) {
+        const double v = sb[b];
+        if (v != 0.0) {
+            int dr = (g.drmin + b / L) % L;
+            if (dr < 0) dr += L;
+            unsafeAtomicAdd(&gbins[(long)dr * L + b % L], v);
+        }
+    }
+}
+
+struct GemmArgs {
+    const void *A;   // A operand base (lag 0)
+    const void *B;   // B operand
+    const void *A2;  // lo parts (split precisions)
+    const void *B2;
+    long ld;         // k-major: row stride in elements; i-major: row stride (k) in elements
+    long copy_stride;  // i-major A: elements between the 8 shifted copies
+    int L, Ns, Kpad, nt, nlags;
+    double *bins;    // [nlags][Ns]
+};
+
+// ------------------------------------------------------------ FP64 / FP32
+// k-major operands; v_mfma_{f64_16x16x4f64, f32_16x16x4f32}: lane l holds A[i=l%16][k=l/16],
+// B[k=l/16][j=l%16]; the 4 outputs of lane l are D[4(l/16)+v][l%16].
+// 256 threads = 2 x 2 waves, each wave 64 x 64 = 4 x 4 MFMA blocks; BK = 16 per LDS stage,
+// two stages (one barrier per stage).
+
+template <class T> struct KMajorMfma;
+template <> struct KMajorMfma<double> {
+    typedef d4 acc_t;
+    static __device__ __forceinline__ acc_t mfma(double a, double b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct KMajorMfma<float> {
+    typedef f4 acc_t;
+    static __device__ __forceinline__ acc_t mfma(float a, float b, acc_t c)
+    {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+    }
+};
+
+template <class T>
+__global__ __launch_bounds__(256) void sqt_gemm_kmajor_kernel(GemmArgs args)
+{
+    constexpr int BK = 16, PAD = 16 / sizeof(T) * 2, LDS_LD = TILE + PAD;
+    typedef typename KMajorMfma<T>::acc_t acc_t;
+    __shared__ __attribute__((aligned(16))) unsigned char smem_raw[2 * 2 * BK * LDS_LD * sizeof(T) > 8 * 4096
+                                                                       ? 2 * 2 * BK * LDS_LD * sizeof(T)
+                                                                       : 8 * 4096];
+    T(*sA)[BK][LDS_LD] = reinterpret_cast<T(*)[BK][LDS_LD]>(smem_raw);
+    T(*sB)[BK][LDS_LD] = reinterpret_cast<T(*)[BK][LDS_LD]>(smem_raw + 2 * BK * LDS_LD * sizeof(T));
+
+    const int nt_tiles = (args.Ns + TILE - 1) / TILE;
+    Work w;
+    if (!decode_work(nt_tiles, args.nlags, w)) return;
+    const int i0 = w.ti * TILE, j0 = w.tj * TILE;
+    const long ld = args.ld;
+    const T *Ap = static_cast<const T *>(args.A) + (long)(3 * w.lag) * ld + i0;
+    const T *Bp = static_cast<const T *>(args.B) + j0;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = wave >> 1, wj = wave & 1;
+    // global -> LDS mapping: row k = tid / 16, 8 consecutive columns
+    constexpr int VEC = 8;
+    const int lk = tid / 16, lc = (tid % 16) * VEC;
+
+    acc_t acc[4][4];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[a][b] = acc_t{0, 0, 0, 0};
+
+    T ra[VEC], rb[VEC];
+    auto gload = [&](int kt) {
+        const T *a = Ap + (long)(kt * BK + lk) * ld + lc;
+        const T *b = Bp + (long)(kt * BK + lk) * ld + lc;
+#pragma unroll
+        for (int v = 0; v < VEC; v++) {
+            ra[v] = a[v];
+            rb[v] = b[v];
+        }
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int v = 0; v < VEC; v++) {
+            sA[buf][lk][lc + v] = ra[v];
+            sB[buf][lk][lc + v] = rb[v];
+        }
+    };
+
+    const int nk = args.Kpad / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload(kt + 1);
+#pragma unroll
+        for (int kk = 0; kk < BK / 4; kk++) {
+            const int kr = kk * 4 + lane / 16;
+            T fa[4], fb[4];
+#pragma unroll
+            for (int m = 0; m < 4; m++) fa[m] = sA[cur][kr][wi * 64 + m * 16 + lane % 16];
+#pragma unroll
+            for (int n = 0; n < 4; n++) fb[n] = sB[cur][kr][wj * 64 + n * 16 + lane % 16];
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+                for (int n = 0; n < 4; n++) acc[m][n] = KMajorMfma<T>::mfma(fa[m], fb[n], acc[m][n]);
+        }
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    // ---- epilogue: displacement binning
+    const int L = args.L, Ns = args.Ns;
+    const BinGeom g = bin_geom(i0, j0, Ns, L);
+    double *sb = reinterpret_cast<double *>(smem_raw);
+    for (int b = tid; b < g.nbins; b += 256) sb[b] = 0.0;
+    int rj[4], cj[4];
+#pragma unroll
+    for (int n = 0; n < 4; n++) {
+        const int j = j0 + wj * 64 + n * 16 + lane % 16;
+        rj[n] = j / L;
+        cj[n] = j - rj[n] * L;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 4; m++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+            const int i = i0 + wi * 64 + m * 16 + 4 * (lane / 16) + v;
+            if (i >= Ns) continue;
+            const int ri = i / L, ci = i - ri * L;
+#pragma unroll
+            for (int n = 0; n < 4; n++) {
+                const int j = j0 + wj * 64 + n * 16 + lane % 16;
+                if (j >= Ns) continue;
+                int dc = ci - cj[n];
+                dc += dc < 0 ? L : 0;
+                atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], (double)acc[m][n][v]);
+            }
+        }
+    __syncthreads();
+    flush_bins(sb, g, L, args.bins + (long)w.lag * Ns);
+}
+
+// ------------------------------------------------------------- BF16X3 / FP16X2
+// i-major operands (k contiguous).  v_mfma_f32_32x32x16_{bf16,f16}: lane l holds
+// A[i=l%32][k=8(l/32)+e], B[k=8(l/32)+e][j=l%32] (e = 0..7); the 16 outputs of lane l
+// are D[8(v/4) + 4(l/32) + v%4][l%32].  256 threads = 2 x 2 waves, each wave 64 x 64 =
+// 2 x 2 blocks of 32 x 32; BK = 32 per LDS stage.
+//   BF16X3: acc += Ah Bh + Ah Bl + Al Bh
+//   FP16X2: acc_hi += Ah Bh;  acc_lo += Ah Bl + Al Bh;  G = acc_hi + 2^-11 acc_lo
+
+template <int PREC> struct SplitMfma;
+template <> struct SplitMfma<ASD_BF16X3> {
+    typedef bf8 v_t;
+    static __device__ __forceinline__ f16v mfma(v_t a, v_t b, f16v c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
+    }
+};
+template <> struct SplitMfma<ASD_FP16X2> {
+    typedef h8 v_t;
+    static __device__ __forceinline__ f16v mfma(v_t a, v_t b, f16v c)
+    {
+        return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
+    }
+};
+
+template <int PREC>
+__global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
+{
+    constexpr int BK = 32, LDS_LD = BK + 8;  // halves per LDS row (pad 16 B)
+    typedef typename SplitMfma<PREC>::v_t v_t;
+    // [buf][operand: Ah, Al, Bh, Bl][row][k]
+    __shared__ __attribute__((aligned(16))) uint16_t smem[2][4][TILE][LDS_LD];
+
+    const int nt_tiles = (args.Ns + TILE - 1) / TILE;
+    Work w;
+    if (!decode_work(nt_tiles, args.nlags, w)) return;
+    const int i0 = w.ti * TILE, j0 = w.tj * TILE;
+    const long ld = args.ld;
+    const long koff = 3L * w.lag, shift = koff & 7;
+    const long acopy = shift * args.copy_stride + (long)i0 * ld + (koff - shift);
+    const uint16_t *Ah = static_cast<const uint16_t *>(args.A) + acopy;
+    const uint16_t *Al = static_cast<const uint16_t *>(args.A2) + acopy;
+    const uint16_t *Bh = static_cast<const uint16_t *>(args.B) + (long)j0 * ld;
+    const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (long)j0 * ld;
+
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = wave >> 1, wj = wave & 1;
+    // global -> LDS: each operand tile is 128 rows x 32 halves = 8 KB = 256 threads x 32 B:
+    // thread -> row tid/2, halves (tid%2)*16 .. +16 (two 16-byte pieces)
+    const int gr = tid >> 1, gk = (tid & 1) * 16;
+
+    f16v acc[2][2], acc2[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) {
+                acc[a][b][e] = 0.0f;
+                acc2[a][b][e] = 0.0f;
+            }
+
+    // every row start is 16-byte aligned (ld % 8 == 0; A read from the shifted copy)
+    uint16_t ra[2][16], rb[2][16];
+    auto gload = [&](int kt) {
+        const long k0 = (long)kt * BK + gk;
+        const uint4 *pa0 = reinterpret_cast<const uint4 *>(Ah + (long)gr * ld + k0);
+        const uint4 *pa1 = reinterpret_cast<const uint4 *>(Al + (long)gr * ld + k0);
+        const uint4 *pb0 = reinterpret_cast<const uint4 *>(Bh + (long)gr * ld + k0);
+        const uint4 *pb1 = reinterpret_cast<const uint4 *>(Bl + (long)gr * ld + k0);
+        const uint4 a00 = pa0[0], a01 = pa0[1], a10 = pa1[0], a11 = pa1[1];
+        const uint4 b00 = pb0[0], b01 = pb0[1], b10 = pb1[0], b11 = pb1[1];
+        *reinterpret_cast<uint4 *>(&ra[0][0]) = a00;
+        *reinterpret_cast<uint4 *>(&ra[0][8]) = a01;
+        *reinterpret_cast<uint4 *>(&ra[1][0]) = a10;
+        *reinterpret_cast<uint4 *>(&ra[1][8]) = a11;
+        *reinterpret_cast<uint4 *>(&rb[0][0]) = b00;
+        *reinterpret_cast<uint4 *>(&rb[0][8]) = b01;
+        *reinterpret_cast<uint4 *>(&rb[1][0]) = b10;
+        *reinterpret_cast<uint4 *>(&rb[1][8]) = b11;
+    };
+    auto sstore = [&](int buf) {
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            *reinterpret_cast<uint4 *>(&smem[buf][h][gr][gk]) = *reinterpret_cast<uint4 *>(&ra[h][0]);
+            *reinterpret_cast<uint4 *>(&smem[buf][h][gr][gk + 8]) = *reinterpret_cast<uint4 *>(&ra[h][8]);
+            *reinterpret_cast<uint4 *>(&smem[buf][2 + h][gr][gk]) = *reinterpret_cast<uint4 *>(&rb[h][0]);
+            *reinterpret_cast<uint4 *>(&smem[buf][2 + h][gr][gk + 8]) = *reinterpret_cast<uint4 *>(&rb[h][8]);
+        }
+    };
+
+    const int nk = args.Kpad / BK;
+    gload(0);
+    sstore(0);
+    __syncthreads();
+    for (int kt = 0; kt < nk; kt++) {
+        const int cur = kt & 1;
+        if (kt + 1 < nk) gload(kt + 1);
+#pragma unroll
+        for (int kk = 0; kk < BK / 16; kk++) {
+            const int kc = kk * 16 + 8 * (lane / 32);
+            v_t ah[2], al[2], bh[2], bl[2];
+#pragma unroll
+            for (int m = 0; m < 2; m++) {
+                const int r = wi * 64 + m * 32 + lane % 32;
+                ah[m] = *reinterpret_cast<const v_t *>(&smem[cur][0][r][kc]);
+                al[m] = *reinterpret_cast<const v_t *>(&smem[cur][1][r][kc]);
+            }
+#pragma unroll
+            for (int n = 0; n < 2; n++) {
+                const int r = wj * 64 + n * 32 + lane % 32;
+                bh[n] = *reinterpret_cast<const v_t *>(&smem[cur][2][r][kc]);
+                bl[n] = *reinterpret_cast<const v_t *>(&smem[cur][3][r][kc]);
+            }
+#pragma unroll
+            for (int m = 0; m < 2; m++)
+#pragma unroll
+                for (int n = 0; n < 2; n++) {
+                    if (PREC == ASD_BF16X3) {
+                        acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bh[n], acc[m][n]);
+                        acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc[m][n]);
+                        acc[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc[m][n]);
+                    } else {
+                        acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bh[n], acc[m][n]);
+                        acc2[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc2[m][n]);
+                        acc2[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc2[m][n]);
+                    }
+                }
+        }
+        if (kt + 1 < nk) sstore(cur ^ 1);
+        __syncthreads();
+    }
+
+    const int L = args.L, Ns = args.Ns;
+    const BinGeom g = bin_geom(i0, j0, Ns, L);
+    double *sb = reinterpret_cast<double *>(&smem[0][0][0][0]);
+    for (int b = tid; b < g.nbins; b += 256) sb[b] = 0.0;
+    int rj[2], cj[2];
+#pragma unroll
+    for (int n = 0; n < 2; n++) {
+        const int j = j0 + wj * 64 + n * 32 + lane % 32;
+        rj[n] = j / L;
+        cj[n] = j - rj[n] * L;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < 2; m++)
+#pragma unroll
+        for (int v = 0; v < 16; v++) {
+            const int i = i0 + wi * 64 + m * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+            if (i >= Ns) continue;
+            const int ri = i / L, ci = i - ri * L;
+#pragma unroll
+            for (int n = 0; n < 2; n++) {
+                const int j = j0 + wj * 64 + n * 32 + lane % 32;
+                if (j >= Ns) continue;
+                int dc = ci - cj[n];
+                dc += dc < 0 ? L : 0;
+                double val = (double)acc[m][n][v];
+                if (PREC == ASD_FP16X2) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
+                atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], val);
+            }
+        }
+    __syncthreads();
+    flush_bins(sb, g, L, args.bins + (long)w.lag * Ns);
+}
+
+// ----------------------------------------------------------- FP64 finishing
+
+struct KahanC {
+    double re = 0, im = 0, cre = 0, cim = 0;
+    __device__ __forceinline__ void add(double xr, double xi)
+    {
+        double y = xr - cre, t = re + y;
+        cre = (t - re) - y;
+        re = t;
+        y = xi - cim;
+        t = im + y;
+        cim = (t - im) - y;
+        im = t;
+    }
+};
+
+// block-wide sum of NV doubles (256 threads)
+template <int NV> __device__ __forceinline__ void block_sum(double (&v)[NV], double *red)
+{
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int k = 0; k < NV; k++) red[k * 256 + tid] = v[k];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if (tid < s)
+#pragma unroll
+            for (int k = 0; k < NV; k++) red[k * 256 + tid] += red[k * 256 + tid + s];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < NV; k++) v[k] = red[k * 256];
+    __syncthreads();
+}
+
+// F[t][q][c] = sum_i S_i^c(t) e^{-i q.r_i}, (re, im); grid (nq, nt), 256 threads.
+// phase index (mx row + my col) mod L into the exact table tw[k] = (cos, sin)(2 pi k / L).
+__global__ __launch_bounds__(256) void fourier_kernel(const double *__restrict__ snaps, long tstride,
+                                                      int L, const int *__restrict__ q,
+                                                      const double2 *__restrict__ tw,
+                                                      double *__restrict__ F, int nq)
+{
+    __shared__ double red[6 * 256];
+    const int iq = blockIdx.x, t = blockIdx.y;
+    const int mx = q[2 * iq], my = q[2 * iq + 1];
+    const int Ns = L * L;
+    const double *S = snaps + (long)t * tstride;
+    KahanC acc[3];
+    for (int i = threadIdx.x; i < Ns; i += 256) {
+        const int r = i / L, c = i - r * L;
+        const int k = (int)(((long)mx * r + (long)my * c) % L);
+        const double2 e = tw[k];  // e^{-i phi} = cos - i sin
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double s = S[3L * i + d];
+            acc[d].add(s * e.x, -s * e.y);
+        }
+    }
+    double v[6] = {acc[0].re, acc[0].im, acc[1].re, acc[1].im, acc[2].re, acc[2].im};
+    block_sum<6>(v, red);
+    if (threadIdx.x < 6) F[((long)t * nq + iq) * 6 + threadIdx.x] = v[threadIdx.x];
+}
+
+// out[lag][q] = ( (1/T0) sum_d g_lag(d) e^{-iq.d}  -  Fbar_q(lag).conj Fbar_q(0) ) / Ns
+__global__ __launch_bounds__(256) void finish_kernel(const double *__restrict__ bins, int L,
+                                                     const int *__restrict__ q,
+                                                     const double2 *__restrict__ tw,
+                                                     const double *__restrict__ F,
+                                                     const double *__restrict__ Fz, int nq, int T0,
+                                                     double *__restrict__ out)
+{
+    __shared__ double red[12 * 256];
+    const int iq = blockIdx.x, lag = blockIdx.y;
+    const int mx = q[2 * iq], my = q[2 * iq + 1];
+    const int Ns = L * L;
+    const double *g = bins + (long)lag * Ns;
+    KahanC s;
+    for (int b = threadIdx.x; b < Ns; b += 256) {
+        const int dr = b / L, dc = b - dr * L;
+        const int k = (int)(((long)mx * dr + (long)my * dc) % L);
+        const double2 e = tw[k];
+        const double v = g[b];
+        s.add(v * e.x, -v * e.y);
+    }
+    // Fbar(lag) and Fbar(0) = (1/T0) sum_t0 F(t0 + lag), F(t0)
+    KahanC fb[3], zb[3];
+    for (int t0 = threadIdx.x; t0 < T0; t0 += 256) {
+        const double *a = F + ((long)(t0 + lag) * nq + iq) * 6;
+        const double *z = Fz + ((long)t0 * nq + iq) * 6;
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            fb[d].add(a[2 * d], a[2 * d + 1]);
+            zb[d].add(z[2 * d], z[2 * d + 1]);
+        }
+    }
+    double v[14] = {s.re, s.im};
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        v[2 + 2 * d] = fb[d].re; v[3 + 2 * d] = fb[d].im;
+        v[8 + 2 * d] = zb[d].re; v[9 + 2 * d] = zb[d].im;
+    }
+    // two passes of 7 to stay within the reduction buffer
+    double v0[7], v1[7];
+#pragma unroll
+    for (int k = 0; k < 7; k++) { v0[k] = v[k]; v1[k] = v[7 + k]; }
+    block_sum<7>(v0, red);
+    block_sum<7>(v1, red);
+    if (threadIdx.x == 0) {
+        const double invT = 1.0 / (double)T0;
+        double dre = 0, dim = 0;
+        const double f[6] = {v0[2], v0[3], v0[4], v0[5], v0[6], v1[0]};
+        const double z[6] = {v1[1], v1[2], v1[3], v1[4], v1[5], v1[6]};
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            const double ar = f[2 * d] * invT, ai = f[2 * d + 1] * invT;
+            const double br = z[2 * d] * invT, bi = z[2 * d + 1] * invT;
+            dre += ar * br + ai * bi;   // a * conj(b)
+            dim += ai * br - ar * bi;
+        }
+        out[((long)lag * nq + iq) * 2] = (v0[0] * invT - dre) / (double)Ns;
+        out[((long)lag * nq + iq) * 2 + 1] = (v0[1] * invT - dim) / (double)Ns;
+    }
+}
+
+__global__ void zero_kernel(double *p, long n)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
+        p[i] = 0.0;
+}
+
+// --------------------------------------------------------------- host side
+
+static long round_up(long a, long b) { return (a + b - 1) / b * b; }
+
+static int grid_for(int nt_tiles, int nlags)
+{
+    const long total = (long)nt_tiles * nt_tiles * nlags;
+    return (int)round_up(total, 8);
+}
+
+int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_b, long tstride,
+            int nsnap_a, int T0, int nlags, const int *h_q, int nq, asd_precision prec, double *d_out,
+            int cfg)
+{
+    (void)cfg;
+    const int Ns = L * L;
+    const int K = 3 * T0;
+    const int nt_tiles = (Ns + TILE - 1) / TILE;
+    const long ldi = round_up(Ns, TILE);  // padded site count
+    const int nt_a = T0 + nlags - 1;      // snapshots the A windows touch
+    ASD_CHECK(nsnap_a >= nt_a, "need nsnap >= T0 + nlags - 1 (%d < %d)", nsnap_a, nt_a);
+    hipStream_t st = ctx->stream;
+
+    // twiddles and q list
+    ASD_TRY(ctx->twiddle.ensure(sizeof(double2) * L));
+    ASD_TRY(ctx->qdev.ensure(sizeof(int) * 2 * nq));
+    {
+        std::vector<double2> tw(L);
+        for (int k = 0; k < L; k++) {
+            // exact phases: angle 2 pi k / L evaluated in long double
+            const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)L;
+            tw[k] = make_double2((double)cosl(ang), (double)sinl(ang));
+        }
+        ASD_HIP(hipMemcpyAsync(ctx->twiddle.ptr, tw.data(), sizeof(double2) * L, hipMemcpyHostToDevice, st));
+        ASD_HIP(hipMemcpyAsync(ctx->qdev.ptr, h_q, sizeof(int) * 2 * nq, hipMemcpyHostToDevice, st));
+        ASD_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
+    }
+    // bins
+    const long nbins = (long)nlags * Ns;
+    ASD_TRY(ctx->bins.ensure(sizeof(double) * nbins));
+    hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(256), 0, st, ctx->bins.as<double>(), nbins);
+
+    GemmArgs ga{};
+    ga.L = L;
+    ga.Ns = Ns;
+    ga.nt = nt_tiles;
+    ga.nlags = nlags;
+    ga.bins = ctx->bins.as<double>();
+    const int nblocks = grid_for(nt_tiles, nlags);
+
+    if (prec == ASD_FP64 || prec == ASD_FP32) {
+        const size_t es = prec == ASD_FP64 ? 8 : 4;
+        const long Kpad = round_up(K, 16);
+        const long rowsA = 3L * (nlags - 1) + Kpad;
+        ASD_TRY(ctx->opA.ensure(es * rowsA * ldi));
+        ASD_TRY(ctx->opB.ensure(es * Kpad * ldi));
+        const dim3 pb(256);
+        if (prec == ASD_FP64) {
+            hipLaunchKernelGGL(pack_kmajor_kernel<double>, dim3((unsigned)(ldi / 256 + 1), (unsigned)rowsA), pb, 0,
+                               st, d_snaps_a, tstride, nt_a, Ns, ctx->opA.as<double>(), ldi);
+            hipLaunchKernelGGL(pack_kmajor_kernel<double>, dim3((unsigned)(ldi / 256 + 1), (unsigned)Kpad), pb, 0,
+                               st, d_snaps_b, tstride, T0, Ns, ctx->opB.as<double>(), ldi);
+        } else {
+            hipLaunchKernelGGL(pack_kmajor_kernel<float>, dim3((unsigned)(ldi / 256 + 1), (unsigned)rowsA), pb, 0,
+                               st, d_snaps_a, tstride, nt_a, Ns, ctx->opA.as<float>(), ldi);
+            hipLaunchKernelGGL(pack_kmajor_kernel<float>, dim3((unsigned)(ldi / 256 + 1), (unsigned)Kpad), pb, 0,
+                               st, d_snaps_b, tstride, T0, Ns, ctx->opB.as<float>(), ldi);
+        }
+        ga.A = ctx->opA.ptr;
+        ga.B = ctx->opB.ptr;
+        ga.ld = ldi;
+        ga.Kpad = (int)Kpad;
+        if (prec == ASD_FP64)
+            hipLaunchKernelGGL(sqt_gemm_kmajor_kernel<double>, dim3(nblocks), dim3(256), 0, st, ga);
+        else
+            hipLaunchKernelGGL(sqt_gemm_kmajor_kernel<float>, dim3(nblocks), dim3(256), 0, st, ga);
+    } else {
+        const long Kpad = round_up(K, 32);
+        const long ldk = round_up(3L * (nlags - 1) + Kpad + 8, 8);  // halves per i-row
+        const long cs = ldk * ldi;                                   // one shifted copy
+        ASD_TRY(ctx->opA.ensure(2 * 8 * cs));
+        ASD_TRY(ctx->opA2.ensure(2 * 8 * cs));
+        ASD_TRY(ctx->opB.ensure(2 * cs));
+        ASD_TRY(ctx->opB2.ensure(2 * cs));
+        const dim3 pg((unsigned)ldi, (unsigned)((ldk + 255) / 256)), pb(256);
+        auto packer = prec == ASD_BF16X3 ? pack_split_kernel<ASD_BF16X3> : pack_split_kernel<ASD_FP16X2>;
+        hipLaunchKernelGGL(packer, pg, pb, 0, st, d_snaps_a, tstride, nt_a, Ns, ctx->opA.as<uint16_t>(),
+                           ctx->opA2.as<uint16_t>(), ldk, cs, 8);
+        hipLaunchKernelGGL(packer, pg, pb, 0, st, d_snaps_b, tstride, T0, Ns, ctx->opB.as<uint16_t>(),
+                           ctx->opB2.as<uint16_t>(), ldk, cs, 1);
+        ga.copy_stride = cs;
+        ga.A = ctx->opA.ptr;
+        ga.A2 = ctx->opA2.ptr;
+        ga.B = ctx->opB.ptr;
+        ga.B2 = ctx->opB2.ptr;
+        ga.ld = ldk;
+        ga.Kpad = (int)Kpad;
+        if (prec == ASD_BF16X3)
+            hipLaunchKernelGGL(sqt_gemm_split_kernel<ASD_BF16X3>, dim3(nblocks), dim3(256), 0, st, ga);
+        else
+            hipLaunchKernelGGL(sqt_gemm_split_kernel<ASD_FP16X2>, dim3(nblocks), dim3(256), 0, st, ga);
+    }
+    ASD_HIP(hipGetLastError());
+
+    // Fourier sums of every snapshot the windows touch (A side covers [0, nt_a);
+    // B side is snapshots [0, T0) of d_snaps_b, which equals d_snaps_a in the
+    // time-origin form; the replica form passes T0 = 1 window of its own)
+    ASD_TRY(ctx->fq.ensure(sizeof(double) * 6 * (long)(nt_a + T0) * nq));
+    double *Fa = ctx->fq.as<double>();
+    hipLaunchKernelGGL(fourier_kernel, dim3(nq, nt_a), dim3(256), 0, st, d_snaps_a, tstride, L,
+                       ctx->qdev.as<int>(), ctx->twiddle.as<double2>(), Fa, nq);
+    const double *Fz = Fa;
+    if (d_snaps_b != d_snaps_a) {
+        double *Fb = Fa + 6L * nt_a * nq;
+        hipLaunchKernelGGL(fourier_kernel, dim3(nq, T0), dim3(256), 0, st, d_snaps_b, tstride, L,
+                           ctx->qdev.as<int>(), ctx->twiddle.as<double2>(), Fb, nq);
+        Fz = Fb;
+    }
+    hipLaunchKernelGGL(finish_kernel, dim3(nq, nlags), dim3(256), 0, st, ctx->bins.as<double>(), L,
+                       ctx->qdev.as<int>(), ctx->twiddle.as<double2>(), (const double *)Fa, Fz, nq, T0, d_out);
+    ASD_HIP(hipGetLastError());
+    return ASD_OK;
+}
+
+}  // namespace asd

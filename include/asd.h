@@ -1,0 +1,181 @@
+/*
+ * asd.h -- C ABI of libasd, the MI355X (gfx950) implementation of the two hot
+ * stages of hwchen2017/ASD-kernel-fusion's kernel_fusion path:
+ *
+ *   1. RK4 integration of the Landau-Lifshitz-Gilbert equation on a periodic
+ *      L x L lattice  (reference: kernel_fusion/llg_sqt_fusion.c:30-145, :350-361)
+ *   2. the dynamical structure factor S(q,t) as a spin-spin correlation GEMM
+ *      whose epilogue carries the phase weighting and the reduction
+ *      (reference: kernel_fusion/gemm_kernel.h:355-422, llg_sqt_fusion.c:235-442)
+ *
+ * Plain C: pointers, sizes, status codes.  No torch, no HIP types in the
+ * signatures (streams are passed as void*).  Every function returns ASD_OK (0)
+ * or a negative status; asd_last_error() gives a message (thread-local).
+ *
+ * Conventions shared with the reference:
+ *   site index i = row * L + col (llg_sqt_fusion.c:51); spins are
+ *   [rep][site][xyz] doubles (:55); neighbours row+1, row-1, col-1, col+1
+ *   (:181-184); q = 2 pi (mx, my) / L with mx pairing with the row offset
+ *   (:292-293, :322).
+ */
+#ifndef ASD_H
+#define ASD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status --- */
+enum {
+    ASD_OK = 0,
+    ASD_EINVAL = -1,   /* bad argument (sizes, pointers, unsupported combination) */
+    ASD_ENOMEM = -2,   /* device or host allocation failed                         */
+    ASD_EHIP = -3,     /* a HIP runtime call failed                                */
+    ASD_ENOTSUP = -4,  /* valid request this build does not support               */
+    ASD_ENOTFINITE = -5/* NaN/Inf or |S| far from spin_length detected            */
+};
+const char *asd_last_error(void);
+const char *asd_version(void);
+
+/* --------------------------------------------------------- parameters --- */
+/* Replaces the reference globals J1 and dt (llg_sqt_fusion.c:15, :17) and adds
+ * the terms the north-star physics needs.  H_eff = J sum_nn S_j + (2 D S_z + B_z) z,
+ * dS/dt = [H x S + (alpha/spin_length) S x (H x S)] / (1 + alpha^2), which is the
+ * reference's h x S (llg_sqt_fusion.c:82) exactly when D = B_z = alpha = 0. */
+typedef struct asd_params {
+    double J;           /* nearest-neighbour exchange (reference J1 = 1.0)          */
+    double D;           /* easy-axis anisotropy                                       */
+    double Bz;          /* field along z                                              */
+    double alpha;       /* Gilbert damping                                            */
+    double dt;          /* RK4 step                                                   */
+    double spin_length; /* renormalisation length: 1.0 (baseline/, gemm/), 1.5 (kernel_fusion/) */
+} asd_params;
+
+typedef enum asd_precision {
+    ASD_FP64 = 0,   /* v_mfma_f64_16x16x4f64, FP64 accumulate                         */
+    ASD_FP32 = 1,   /* v_mfma_f32_16x16x4f32, FP32 accumulate                         */
+    ASD_BF16X3 = 2, /* a = a_hi + a_lo (bf16); a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32 acc */
+    ASD_FP16X2 = 3  /* a = a_hi + 2^-11 a_lo (fp16); hi and lo products in two FP32 accs */
+} asd_precision;
+
+/* --------------------------------------------------- host-side helpers --- */
+/* Marsaglia (1972) uniform spins on S^2 into host memory out[nsites][3].  Site i
+ * draws from its own counter-based stream: u = U(h(seed,i,2a)), v = U(h(seed,i,2a+1)),
+ * a = 0,1,... until 0 < u^2+v^2 < 1; h = splitmix64 finaliser of a linear key.
+ * Replaces the reference's time-seeded integer-division init (llg_sqt_fusion.c:195-216). */
+int asd_init_spins(uint64_t seed, int nsites, double *out);
+
+/* lattice momenta of the Gamma-X-M-Gamma path: 3L/2 pairs (mx,my) into q[2*(3L/2)]:
+ * (k,0), (L/2,k), (L/2-k,L/2-k) for k = 0..L/2-1.  L even. Returns the count. */
+int asd_qpath_gxmg(int L, int *q);
+
+/* ------------------------------------------------------ device context --- */
+typedef struct asd_ctx asd_ctx;
+/* Binds to HIP device `device`; owns workspace (ping-pong state, packed operands,
+ * displacement bins).  One context per host thread / stream. */
+int asd_create(asd_ctx **ctx, int device);
+int asd_destroy(asd_ctx *ctx);
+/* stream (hipStream_t) all work of this context is ordered on; NULL = default */
+int asd_set_stream(asd_ctx *ctx, void *stream);
+int asd_synchronize(asd_ctx *ctx);
+
+/* ----------------------------------------------- stage 1: RK4-LLG (FP64) --- */
+/* Native, device-resident replacement for the reference time loop's stage 1
+ * (llg_sqt_fusion.c:348 runge_kutta + :350-361 renormalisation), fused into one
+ * kernel per step (all four stages per LDS tile, radius-4 halo), with the
+ * snapshot writer fused in.  d_spins: device [nrep][L*L][3], in/out.
+ * d_snaps: device [nsnap][nrep][L*L][3] or NULL; snapshot k = state after
+ * k*snap_every steps (k = 0 is the input state).  The step loop is replayed from
+ * a captured hipGraph when use_graph != 0. */
+int asd_rk4_run(asd_ctx *ctx, const asd_params *p, int L, int nrep, double *d_spins,
+                long nsteps, int snap_every, int nsnap, double *d_snaps, int use_graph);
+
+/* ---------------------------------------- stage 2: S(q,tau) correlation --- */
+/* Native replacement for the reference's S(q,t) stage (llg_sqt_fusion.c:370-442
+ * + fused_dgemm), with time origins in the role of the reference's replicas:
+ *   G_ij(tau) = (1/T0) sum_{t0<T0} S_i(t0+tau) . S_j(t0)        (GEMM, K = 3 T0)
+ *   S(q,tau)  = (1/Ns) sum_ij [G_ij - fbar_i(tau).zbar_j] e^{-iq.(r_i - r_j)}
+ * The GEMM runs on MFMA in `prec`; its epilogue reduces G into displacement
+ * bins g_tau(dr,dc) = sum_i G_{i,i-d} (exact for lattice momenta) in FP64; a
+ * second kernel evaluates the bins at every q with an exact L-entry twiddle table
+ * (Kahan-compensated FP64) and subtracts the separable disconnected term
+ * Fbar_q(tau).conj Fbar_q(0) computed in FP64.
+ * d_snaps: device [nsnap][L*L][3] FP64, nsnap >= T0 + nlags - 1.
+ * h_q: host (mx,my) pairs.  out: [nlags][nq][2] (re, im) doubles, host memory
+ * (out_on_device = 0) or device memory (1). */
+int asd_sqt(asd_ctx *ctx, int L, const double *d_snaps, int nsnap, int T0, int nlags,
+            const int *h_q, int nq, asd_precision prec, double *out, int out_on_device);
+
+/* Same contraction for the reference's replica form (llg_sqt_fusion.c:235-268,
+ * :370-398): A = nrep replicas at time t, B = the same replicas at time 0, one lag.
+ * d_now, d_zero: device [nrep][L*L][3].  out: [nq][2] host doubles. */
+int asd_sqt_replicas(asd_ctx *ctx, int L, int nrep, const double *d_now,
+                     const double *d_zero, const int *h_q, int nq, asd_precision prec,
+                     double *out);
+
+/* ------------------------------------------------------------ batching --- */
+/* Independent trajectories of uneven L in ONE persistent launch per step
+ * (work list of (trajectory, tile) items), step loop captured as a hipGraph.
+ * Ls[ntraj], params[ntraj], d_spins[ntraj] (device [L*L][3] each),
+ * d_snaps[ntraj] (device [nsnap][L*L][3] each, or NULL array). */
+int asd_batch_rk4_run(asd_ctx *ctx, int ntraj, const int *Ls, const asd_params *params,
+                      double *const *d_spins, long nsteps, int snap_every, int nsnap,
+                      double *const *d_snaps);
+
+/* -------------------------------------------------------- autotuning --- */
+/* Times the correlation-GEMM tile configurations for (L, K = 3 T0, prec) on
+ * this device and records the fastest; later asd_sqt calls with that shape use
+ * it.  best_cfg receives the chosen configuration index (may be NULL). */
+int asd_autotune_sqt(asd_ctx *ctx, int L, int T0, asd_precision prec, int *best_cfg);
+/* number of tile configurations for a precision, and a printable name */
+int asd_sqt_num_configs(asd_precision prec);
+const char *asd_sqt_config_name(asd_precision prec, int cfg);
+/* force a configuration for a precision (-1 = automatic) */
+int asd_sqt_set_config(asd_ctx *ctx, asd_precision prec, int cfg);
+
+/* ------------------------------------------------------------ adjoint --- */
+/* Loss = sum over (lag, q) of |S(q,tau) - S_target(q,tau)|^2 for the trajectory
+ * started at d_S0 (device [L*L][3]) with parameters p, nsteps RK4 steps,
+ * snapshots every snap_every steps (nsnap = T0 + nlags - 1 of them), and its
+ * gradient with respect to theta = (J, D, B_z, alpha), by a hand-written discrete
+ * adjoint of RK4-LLG (incl. renormalisation) and of the correlation, with
+ * forward states checkpointed every `checkpoint_every` steps and recomputed in
+ * reverse.  h_target: host [nlags][nq][2].  FP64 throughout. */
+int asd_sqt_loss_grad(asd_ctx *ctx, const asd_params *p, int L, const double *d_S0,
+                      long nsteps, int snap_every, int T0, int nlags, const int *h_q,
+                      int nq, const double *h_target, int checkpoint_every,
+                      double *loss, double grad[4]);
+
+/* ---------------------------------- reference-compatible entry points --- */
+/* Host-pointer entries with the reference's argument meaning; they copy to the
+ * device and back on every call (for parity and drop-in use, not speed).
+ *
+ * asd_euler replaces `void euler(double *spin0, double *derivative)`
+ * (llg_sqt_fusion.c:30): derivative = f(spin0) for nrep lattices; the reference
+ * globals N, Nfile, J1 become L, nrep, p->J. */
+int asd_euler(const asd_params *p, int L, int nrep, const double *spin0, double *derivative);
+/* replaces `void runge_kutta(double *spin, double *cur_input, double *k1, double *k2,
+ * double *k3, double *k4)` (llg_sqt_fusion.c:98): cur_input = unnormalised
+ * S^{n+1}, k_s = dt f(stage_s); spin unchanged. */
+int asd_runge_kutta(const asd_params *p, int L, int nrep, const double *spin, double *cur_input,
+                    double *k1, double *k2, double *k3, double *k4);
+/* replaces `void fused_dgemm(int *m, int *n, int *k, double *alpha, double *a, int *lda,
+ * double *b, int *ldb, double *c, int *ldc, double *fuse_C, double *sum)`
+ * (gemm_kernel.h:355), same arguments, returning a status: c (column-major, the
+ * blocked 4-real/4-imaginary layout, ldc >= 2*ceil4(m)) += (alpha a b) o fuse_C and
+ * sum[0], sum[1] += sum of the updated c.  Correct for every K (the reference
+ * re-sums per 240-wide K block), every m (the reference drops m % 4 rows) and
+ * every n (it drops the last column of odd n).  sum: the reference keeps one
+ * pair per OpenMP thread and the caller adds them up; here the whole total is
+ * added to sum[0..1] and sum[2..] are untouched, so the caller's loop over
+ * pairs gives the same result. */
+int asd_fused_dgemm(int *m, int *n, int *k, double *alpha, double *a, int *lda, double *b,
+                    int *ldb, double *c, int *ldc, double *fuse_C, double *sum);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ASD_H */

@@ -28,3 +28,15 @@ def ref_fusion():
     if not oracle.RefFusion.available("parity"):
         pytest.skip("reference kernel_fusion build (oracle/_ref) not present")
     return oracle.RefFusion("parity")
+
+
+@pytest.fixture(scope="session")
+def gctx():
+    """One libasd context on cuda:0 for the GPU tests."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU visible")
+    import asd_amd
+    c = asd_amd.Context(0)
+    yield c
+    c.close()

@@ -1,0 +1,125 @@
+"""S(q,tau) on the GPU (asd_sqt) against the FP64 oracle.
+
+FP64 path: within 1e-10 (relative to max|S| per lag) of the oracle's GEMM form
+(the reference data flow of llg_sqt_fusion.c:235-442 with time origins as
+replicas) and of the separable Fourier form.  FP32 / BF16X3 / FP16X2: max and
+RMS error against the GPU FP64 result are printed, with bounds stated in the
+test.  The replica form (asd_sqt_replicas) is checked against the reference's
+own fused_dgemm driven with the operands llg_sqt_fusion.c builds.
+"""
+import numpy as np
+import pytest
+
+from gpu_util import err_stats, need_gpu, say
+
+
+def gxmg(L):
+    h = L // 2
+    return [(k, 0) for k in range(h)] + [(h, k) for k in range(h)] + [(h - k, h - k) for k in range(h)]
+
+
+def full_grid(L):
+    return [(a, b) for a in range(L) for b in range(L)]
+
+
+def trajectory(orc, L, nsnap, every=5, seed=1, **kw):
+    p = orc.params(dt=0.01, **kw)
+    _, snaps = orc.run(p, L, orc.init_spins(seed, L * L), (nsnap - 1) * every, every, nsnap)
+    return snaps
+
+
+def max_rel(a, ref):
+    return float(np.abs(a - ref).max() / np.abs(ref).max())
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,T0,nlags,qset", [
+    (8, 10, 5, "full"), (12, 8, 6, "full"), (16, 20, 8, "gxmg"), (6, 30, 4, "full"), (24, 12, 5, "gxmg")])
+def test_fp64_matches_oracle_gemm_form(gctx, orc, L, T0, nlags, qset):
+    torch = need_gpu()
+    say(f"sqt fp64 L={L} T0={T0} nlags={nlags} q={qset}")
+    snaps = trajectory(orc, L, T0 + nlags - 1, D=0.1, Bz=0.05)
+    q = full_grid(L) if qset == "full" else gxmg(L)
+    ref = orc.sqt_gemm(L, snaps, T0, nlags, q)
+    got = gctx.sqt(L, torch.tensor(snaps, device="cuda"), T0, nlags, q, "fp64")
+    e = max_rel(got, ref)
+    say(f"  max rel err vs oracle GEMM form {e:.3e}")
+    assert e <= 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,T0,nlags", [(64, 50, 20), (128, 20, 10), (48, 30, 12), (32, 100, 100)])
+def test_fp64_matches_separable(gctx, orc, L, T0, nlags):
+    torch = need_gpu()
+    say(f"sqt fp64 vs separable L={L} T0={T0} nlags={nlags}")
+    snaps = trajectory(orc, L, T0 + nlags - 1, every=10)
+    q = gxmg(L)
+    ref = orc.sqt_separable(L, snaps, T0, nlags, q)
+    got = gctx.sqt(L, torch.tensor(snaps, device="cuda"), T0, nlags, q, "fp64")
+    e = max_rel(got, ref)
+    say(f"  max rel err vs separable {e:.3e}")
+    assert e <= 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,bound", [("fp32", 1e-5), ("bf16x3", 1e-5), ("fp16x2", 1e-5)])
+def test_reduced_precision_error(gctx, orc, prec, bound):
+    """Error of the MFMA variants against the GPU FP64 result on identical snapshots,
+    per (q, tau) relative to max_q |S(q, tau)| (the floor)."""
+    torch = need_gpu()
+    L, T0, nlags = 64, 60, 20
+    snaps = torch.tensor(trajectory(orc, L, T0 + nlags - 1, every=10, D=0.1), device="cuda")
+    q = gxmg(L)
+    ref = gctx.sqt(L, snaps, T0, nlags, q, "fp64")
+    got = gctx.sqt(L, snaps, T0, nlags, q, prec)
+    mx, rms = err_stats(got, ref)
+    say(f"sqt {prec} L={L}: max rel err {mx:.3e}, rms {rms:.3e} (floor: max_q|S| per lag)")
+    assert mx <= bound
+
+
+@pytest.mark.gpu
+def test_full_grid_sum_rule(gctx, orc):
+    """sum over the full q grid of S(q,0) = Ns - sum_i |zbar_i|^2 (size-independent check)."""
+    torch = need_gpu()
+    L, T0 = 32, 40
+    snaps = trajectory(orc, L, T0, every=10)
+    s0 = gctx.sqt(L, torch.tensor(snaps, device="cuda"), T0, 1, full_grid(L), "fp64")[0]
+    zbar = snaps[:T0].mean(axis=0)
+    assert abs(s0.sum() - (L * L - (zbar ** 2).sum())) <= 1e-9 * L * L
+    assert np.abs(s0.imag).max() <= 1e-10
+
+
+@pytest.mark.gpu
+def test_replica_form_matches_reference_kernel(gctx, orc):
+    """asd_sqt_replicas == reference fused_dgemm (or its restatement) fed the way
+    llg_sqt_fusion.c:235-268, :370-426 feeds it: A = replicas now, B = replicas at 0."""
+    torch = need_gpu()
+    L, nrep = 8, 20
+    ns, K = L * L, 3 * nrep
+    zero = np.stack([orc.init_spins(40 + r, ns) for r in range(nrep)])
+    now, _ = orc.run(orc.params(dt=0.01), L, zero, 30)
+    mx = my = 1
+    B = np.empty(K * ns)
+    A = np.empty(ns * K)
+    for r in range(nrep):
+        for c in range(3):
+            B[(r * 3 + c) + np.arange(ns) * K] = zero[r, :, c]
+            A[np.arange(ns) + (r * 3 + c) * ns] = now[r, :, c]
+    zbar, fbar = zero.mean(axis=0), now.mean(axis=0)
+    Q = orc.build_qmat(L, mx, my)
+    Cp = np.empty_like(Q)
+    idx = np.arange(ns)
+    for j in range(ns):
+        base = (idx // 4) * 8 + idx % 4 + j * 2 * ns
+        tmp = -(fbar @ zbar[j])
+        Cp[base] = tmp * Q[base]
+        Cp[base + 4] = tmp * Q[base + 4]
+    s = np.zeros(4096)
+    if orc.RefFusion.available():
+        orc.RefFusion("parity").fused_dgemm(ns, ns, K, 1.0 / nrep, A, ns, B, K, Cp, 2 * ns, Q, s)
+    else:
+        orc.fused_dgemm(ns, ns, K, 1.0 / nrep, A, ns, B, K, Cp, 2 * ns, Q, s)
+    expect = complex(s[0::2].sum(), s[1::2].sum()) / ns
+    got = gctx.sqt_replicas(L, torch.tensor(now, device="cuda"), torch.tensor(zero, device="cuda"),
+                            [(mx, my)], "fp64")[0]
+    assert abs(got - expect) <= 1e-11 * max(1.0, abs(expect))

@@ -158,7 +158,8 @@ struct GemmArgs {
 
 // ------------------------------------------------------------ FP64 / FP32
 // k-major operands; v_mfma_{f64_16x16x4f64, f32_16x16x4f32}: lane l holds A[i=l%16][k=l/16],
-// B[k=l/16][j=l%16]; the 4 outputs of lane l are D[4(l/16)+v][l%16].
+// B[k=l/16][j=l%16].  The 4 outputs of lane l are D[4(l/16)+v][l%16] for f32 but
+// D[l/16 + 4v][l%16] for f64 (measured on gfx950 with a lane-layout probe).
 // 256 threads = 2 x 2 waves, each wave 64 x 64 = 4 x 4 MFMA blocks; BK = 16 per LDS stage,
 // two stages (one barrier per stage).
 
@@ -169,6 +170,7 @@ template <> struct KMajorMfma<double> {
     {
         return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int v) { return lane / 16 + 4 * v; }
 };
 template <> struct KMajorMfma<float> {
     typedef f4 acc_t;
@@ -176,6 +178,7 @@ template <> struct KMajorMfma<float> {
     {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane / 16) + v; }
 };
 
 template <class T>
@@ -268,7 +271,7 @@ __global__ __launch_bounds__(256) void sqt_gemm_kmajor_kernel(GemmArgs args)
     for (int m = 0; m < 4; m++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
-            const int i = i0 + wi * 64 + m * 16 + 4 * (lane / 16) + v;
+            const int i = i0 + wi * 64 + m * 16 + KMajorMfma<T>::row(lane, v);
             if (i >= Ns) continue;
             const int ri = i / L, ci = i - ri * L;
 #pragma unroll

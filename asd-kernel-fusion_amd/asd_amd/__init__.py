@@ -72,6 +72,8 @@ _SIGS = {
     "asd_sqt_set_config": (C.c_int, [_vp, C.c_int, C.c_int]),
     "asd_sqt_loss_grad": (C.c_int, [_vp, C.POINTER(Params), C.c_int, _vp, C.c_long, C.c_int, C.c_int, C.c_int,
                                     _vp, C.c_int, _vp, C.c_int, _dp, _dp]),
+    "asd_set_timing": (C.c_int, [_vp, C.c_int]),
+    "asd_stage_times": (C.c_int, [_vp, _dp]),
     "asd_euler": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int, _vp, _vp]),
     "asd_runge_kutta": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int, _vp, _vp, _vp, _vp, _vp, _vp]),
     "asd_fused_dgemm": (C.c_int, [_ip, _ip, _ip, _dp, _vp, _ip, _vp, _ip, _vp, _ip, _vp, _vp]),
@@ -153,6 +155,17 @@ class Context:
 
     def synchronize(self):
         _check(lib().asd_synchronize(self.h))
+
+    def set_timing(self, on: bool = True):
+        _check(lib().asd_set_timing(self.h, 1 if on else 0))
+
+    STAGES = ("pack", "gemm", "fourier", "finish", "rk4")
+
+    def stage_times(self) -> dict:
+        """ms of the last call of each stage, from HIP events on this context's stream."""
+        t = (C.c_double * 5)()
+        _check(lib().asd_stage_times(self.h, t))
+        return dict(zip(self.STAGES, list(t)))
 
     # -------------------------------------------------------------- RK4
     def rk4_run(self, p: Params, L: int, spins, nsteps: int, snap_every: int = 0, nsnap: int = 0, snaps=None,

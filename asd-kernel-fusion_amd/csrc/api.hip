@@ -73,6 +73,13 @@ static inline uint64_t hash64(uint64_t seed, uint64_t a, uint64_t b)
 
 __global__ void set_int_kernel(int *p, int v) { *p = v; }
 
+int mark(asd_ctx *c, int k)
+{
+    if (!c->timing) return ASD_OK;
+    ASD_HIP(hipEventRecord(c->ev[k], c->stream));
+    return ASD_OK;
+}
+
 }  // namespace asd
 
 using namespace asd;
@@ -136,6 +143,8 @@ int asd_destroy(asd_ctx *c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+    for (hipEvent_t &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
     DevBuf *bufs[] = {&c->state_b, &c->counter, &c->opA, &c->opB, &c->opA2, &c->opB2, &c->bins, &c->fq,
                       &c->twiddle, &c->qdev, &c->outdev, &c->stats, &c->c0, &c->c1, &c->c2, &c->c3, &c->c4,
                       &c->c5, &c->c6, &c->work, &c->tabs, &c->adj0, &c->adj1, &c->adj2, &c->adj3, &c->adj4,
@@ -152,6 +161,34 @@ int asd_set_stream(asd_ctx *c, void *stream)
     if (c->graph.exec) {
         (void)hipGraphExecDestroy(c->graph.exec);
         c->graph = RK4Graph();
+    }
+    return ASD_OK;
+}
+
+int asd_set_timing(asd_ctx *c, int on)
+{
+    ASD_CHECK(c, "ctx is NULL");
+    if (on)
+        for (hipEvent_t &e : c->ev)
+            if (!e) ASD_HIP(hipEventCreate(&e));
+    c->timing = on ? 1 : 0;
+    return ASD_OK;
+}
+
+int asd_stage_times(asd_ctx *c, double t[5])
+{
+    ASD_CHECK(c && t, "NULL argument");
+    ASD_CHECK(c->timing, "timing is off (asd_set_timing)");
+    const int pairs[5][2] = {{0, 1}, {1, 2}, {2, 3}, {3, 4}, {5, 6}};
+    for (int k = 0; k < 5; k++) {
+        float ms = 0.f;
+        ASD_HIP(hipEventSynchronize(c->ev[pairs[k][1]]));
+        hipError_t e = hipEventElapsedTime(&ms, c->ev[pairs[k][0]], c->ev[pairs[k][1]]);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            ms = -1.f;  // stage not run since timing was enabled
+        }
+        t[k] = ms;
     }
     return ASD_OK;
 }
@@ -182,6 +219,7 @@ int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spi
     ASD_TRY(c->counter.ensure(sizeof(int)));
     double *bufs[2] = {d_spins, c->state_b.as<double>()};
     hipStream_t st = c->stream;
+    ASD_TRY(mark(c, 5));
 
     // snapshots due while stepping: those at s = k * every < nsteps; the one at
     // s == nsteps (if any) is the final state, copied at the end
@@ -243,6 +281,7 @@ int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spi
                                hipMemcpyDeviceToDevice, st));
     if (cur == 1)
         ASD_HIP(hipMemcpyAsync(d_spins, bufs[1], sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    ASD_TRY(mark(c, 6));
     return ASD_OK;
 }
 

@@ -77,11 +77,16 @@ struct asd_ctx {
     // autotuner choice per precision (-1 = automatic); tuned per (L, K)
     int forced_cfg[4] = {-1, -1, -1, -1};
     std::map<std::pair<int, int>, int> tuned[4];
+    // stage timing (asd_set_timing): events around pack, gemm, fourier, finish, rk4
+    int timing = 0;
+    hipEvent_t ev[8] = {};
     // adjoint workspace
     asd::DevBuf adj0, adj1, adj2, adj3, adj4, adj5;
 };
 
 namespace asd {
+// record stage event k on the context stream when timing is on
+int mark(asd_ctx *ctx, int k);
 // internal entry points shared between translation units
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in,
                     double *out, double *snap_out, const int *snap_counter, long snap_stride,

@@ -622,6 +622,7 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
     // bins
     const long nbins = (long)nlags * Ns;
     ASD_TRY(ctx->bins.ensure(sizeof(double) * nbins));
+    ASD_TRY(mark(ctx, 0));
     hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(256), 0, st, ctx->bins.as<double>(), nbins);
 
     GemmArgs ga{};
@@ -654,6 +655,7 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ga.B = ctx->opB.ptr;
         ga.ld = ldi;
         ga.Kpad = (int)Kpad;
+        ASD_TRY(mark(ctx, 1));
         if (prec == ASD_FP64)
             hipLaunchKernelGGL(sqt_gemm_kmajor_kernel<double>, dim3(nblocks), dim3(256), 0, st, ga);
         else
@@ -679,12 +681,14 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ga.B2 = ctx->opB2.ptr;
         ga.ld = ldk;
         ga.Kpad = (int)Kpad;
+        ASD_TRY(mark(ctx, 1));
         if (prec == ASD_BF16X3)
             hipLaunchKernelGGL(sqt_gemm_split_kernel<ASD_BF16X3>, dim3(nblocks), dim3(256), 0, st, ga);
         else
             hipLaunchKernelGGL(sqt_gemm_split_kernel<ASD_FP16X2>, dim3(nblocks), dim3(256), 0, st, ga);
     }
     ASD_HIP(hipGetLastError());
+    ASD_TRY(mark(ctx, 2));
 
     // Fourier sums of every snapshot the windows touch (A side covers [0, nt_a);
     // B side is snapshots [0, T0) of d_snaps_b, which equals d_snaps_a in the
@@ -700,9 +704,11 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
                            ctx->qdev.as<int>(), ctx->twiddle.as<double2>(), Fb, nq);
         Fz = Fb;
     }
+    ASD_TRY(mark(ctx, 3));
     hipLaunchKernelGGL(finish_kernel, dim3(nq, nlags), dim3(256), 0, st, ctx->bins.as<double>(), L,
                        ctx->qdev.as<int>(), ctx->twiddle.as<double2>(), (const double *)Fa, Fz, nq, T0, d_out);
     ASD_HIP(hipGetLastError());
+    ASD_TRY(mark(ctx, 4));
     return ASD_OK;
 }
 

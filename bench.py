@@ -1,0 +1,231 @@
+#!/usr/bin/env python3
+"""Benchmark of the RK4-LLG + S(q,tau) hot path on one MI355X.
+
+Workload (the paper-scale plain configuration): L = 128 (16384 sites), J = 1,
+D = 0.1, B_z = 0, alpha = 0, spins Marsaglia-uniform with seed 1; 8000 RK4
+steps at dt = 0.01 with a snapshot every 10 steps (800 snapshots); S(q,tau) for
+lags 0..399 averaged over 400 time origins (GEMM K = 1200 per lag) on the
+Gamma-X-M-Gamma path (192 lattice momenta).
+
+One "step" of this benchmark is one pass of the whole hot path over that input:
+the 8000-step RK4 run (snapshot writes included) followed by S(q,tau) in the
+split precision.  `value` is the S(q,tau) throughput in algorithmic GEMM flops
+(sum over lags of 2 Ns^2 K = 2.577e14 per pass; split-precision extra products
+not counted) per second of S(q,tau) time; RK4 spin-steps/s, the FP64 path and
+the mixed-vs-FP64 end-to-end ratio with the error of the split result against
+FP64 are reported beside it.  Inputs are resident on the GPU before timing.
+
+The CPU baseline is the reference's own kernel_fusion code (runge_kutta and
+fused_dgemm from oracle/_ref, built from its sources with its AVX2/FMA flags)
+timed on this host's cores on a bounded sample of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+PKG = os.path.join(ROOT, "asd-kernel-fusion_amd")
+for _p in (ROOT, PKG):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+METRIC = "RK4 spin-steps/s; S(q,t) TFLOP/s at L=128; mixed vs FP64 time @ err≤1e-5"
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "fp16x2": 2500.0}
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(L, T0, nq, nlags, nsteps_total, threads):
+    """Reference kernel_fusion (oracle/_ref, AVX2/FMA -O3) on `threads` host cores,
+    bounded sample: 200 RK4 steps at L, and one fused_dgemm call at L, K = 3 T0
+    (one q, as the reference evaluates one q per call) with the prefill glue."""
+    import numpy as np
+
+    import oracle
+    if not oracle.RefFusion.available("perf"):
+        return None
+    os.environ["OMP_NUM_THREADS"] = str(threads)
+    ref = oracle.RefFusion("perf")
+    ns = L * L
+    p = oracle.params(J=1.0, D=0.0, dt=0.01)
+    ref.set_globals(L, 1, 0.01, 1.0)
+    S = oracle.init_spins(1, ns).reshape(1, ns, 3)
+    n_rk = 200
+    ref.runge_kutta(S)                                  # warm
+    t0 = time.perf_counter()
+    for _ in range(n_rk):
+        cur = ref.runge_kutta(S)[0]
+        S = oracle.renormalise(cur, 1.0)
+    t_rk = time.perf_counter() - t0
+    rk_rate = n_rk * ns / t_rk
+
+    K = 3 * T0
+    rng = np.random.default_rng(0)
+    A = rng.standard_normal(ns * K)
+    B = rng.standard_normal(K * ns)
+    t0 = time.perf_counter()
+    Q = oracle.build_qmat(L, L // 10, L // 10)          # once per run in the reference (:308-333)
+    t_q = time.perf_counter() - t0
+    C = np.empty_like(Q)
+    sums = np.zeros(4096)
+    t0 = time.perf_counter()
+    np.multiply(Q, -0.5, out=C)                        # disconnected prefill C = -(f.z) Q (:402-426)
+    ref.fused_dgemm(ns, ns, K, 1.0 / T0, A, ns, B, K, C, 2 * ns, Q, sums)
+    t_call = time.perf_counter() - t0
+    gflop = 2.0 * ns * ns * K / 1e9
+    del Q, C
+    return {
+        "rk4_spin_steps_per_s": rk_rate,
+        "sqt_tflops": gflop / 1e3 / t_call,
+        "sqt_s_per_q_lag": t_call,
+        "projected_sqt_s": t_call * nq * nlags,
+        "projected_rk4_s": nsteps_total * ns / rk_rate,
+        "qmat_build_s": t_q,
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--precision", default="fp16x2", choices=["fp32", "bf16x3", "fp16x2"])
+    ap.add_argument("--L", type=int, default=128)
+    ap.add_argument("--nsteps", type=int, default=8000)
+    ap.add_argument("--every", type=int, default=10)
+    ap.add_argument("--T0", type=int, default=400)
+    ap.add_argument("--nlags", type=int, default=400)
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison pass")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    import asd_amd
+
+    L, T0, nlags, every = args.L, args.T0, args.nlags, args.every
+    ns = L * L
+    nsnap = args.nsteps // every
+    assert nsnap >= T0 + nlags - 1, "not enough snapshots for T0 + nlags - 1"
+    q = asd_amd.qpath_gxmg(L)
+    nq = len(q)
+    gemm_flop = float(nlags) * 2.0 * ns * ns * 3 * T0
+    prm = asd_amd.params(J=1.0, D=0.1, Bz=0.0, alpha=0.0, dt=0.01)
+
+    ctx = asd_amd.Context(0)
+    ctx.set_timing(True)
+    S0 = torch.tensor(asd_amd.init_spins(1, ns), device="cuda")
+    spins = torch.empty_like(S0)
+    snaps = torch.empty((nsnap, ns, 3), dtype=torch.float64, device="cuda")
+    out_split = torch.empty((nlags, nq, 2), dtype=torch.float64, device="cuda")
+
+    def one_pass(prec, out):
+        spins.copy_(S0)
+        ctx.rk4_run(prm, L, spins, args.nsteps, every, nsnap, snaps)
+        st_rk = ctx.stage_times()["rk4"]
+        ctx.sqt(L, snaps, T0, nlags, q, prec, out=out)
+        st = ctx.stage_times()
+        st["rk4"] = st_rk
+        return st
+
+    log(f"bench: L={L} nsteps={args.nsteps} nsnap={nsnap} T0={T0} nlags={nlags} nq={nq} prec={args.precision}")
+    for w in range(args.warmup):
+        one_pass(args.precision, out_split)
+        torch.cuda.synchronize()
+        log(f"  warmup {w} done")
+
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stages = []
+    for k in range(args.steps):
+        stages.append(one_pass(args.precision, out_split))
+        log(f"  step {k}: {stages[-1]}")
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = wall / args.steps * 1e3
+
+    avg = {k: float(np.mean([s[k] for s in stages])) for k in stages[0]}
+    sqt_ms = avg["pack"] + avg["gemm"] + avg["fourier"] + avg["finish"]
+    rk_ms = avg["rk4"]
+    value = gemm_flop / (sqt_ms * 1e-3) / 1e12
+    rk_rate = args.nsteps * ns / (rk_ms * 1e-3)
+    gemm_tflops = gemm_flop / (avg["gemm"] * 1e-3) / 1e12
+
+    res = {
+        "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(ms_per_step, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": args.precision, "data": "synthetic",
+        "config": {"workload": f"plain L={L} (Ns={ns}), J=1 D=0.1 B=0 alpha=0 seed 1, {args.nsteps} RK4 steps "
+                               f"dt=0.01, snapshot every {every} ({nsnap}), {nlags} lags x {T0} origins, "
+                               f"Gamma-X-M-Gamma ({nq} q)",
+                   "L": L, "rk4_steps": args.nsteps, "snapshots": nsnap, "T0": T0, "lags": nlags, "nq": nq,
+                   "gemm_precision": args.precision, "rk4_precision": "fp64"},
+        "roofline": {"bound": "mfma", "achieved": round(gemm_tflops, 3), "peak": PEAK_TFLOPS[args.precision],
+                     "unit": "TFLOP/s", "frac": round(gemm_tflops / PEAK_TFLOPS[args.precision], 4),
+                     "traffic": None,
+                     "kernel": f"sqt_gemm_split_kernel<{args.precision}> (all {nlags} lags in one launch)",
+                     "mfma_products_per_flop": 3},
+        "stage_ms": {k: round(v, 3) for k, v in avg.items()},
+        "rk4_spin_steps_per_s": rk_rate,
+        "rk4_hbm_frac": round(48.0 * rk_rate / (HBM_PEAK_GBS * 1e9), 5),
+        "sqt_s": round(sqt_ms / 1e3, 4),
+    }
+
+    if not args.no_fp64:
+        out64 = torch.empty_like(out_split)
+        one_pass("fp64", out64)                    # warm the FP64 path (workspace)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st64 = one_pass("fp64", out64)
+        torch.cuda.synchronize()
+        wall64 = time.perf_counter() - t0
+        sqt64_ms = st64["pack"] + st64["gemm"] + st64["fourier"] + st64["finish"]
+        a = out_split.cpu().numpy()
+        b = out64.cpu().numpy()
+        sa, sb = a[..., 0] + 1j * a[..., 1], b[..., 0] + 1j * b[..., 1]
+        scale = np.abs(sb).max(axis=1, keepdims=True)
+        e = np.abs(sa - sb) / scale
+        res["fp64"] = {"sqt_s": round(sqt64_ms / 1e3, 4), "gemm_tflops": round(gemm_flop / st64["gemm"] / 1e9, 3),
+                       "gemm_frac_of_fp64_peak": round(gemm_flop / st64["gemm"] / 1e9 / PEAK_TFLOPS["fp64"], 4),
+                       "end_to_end_s": round(wall64, 4)}
+        res["mixed_vs_fp64"] = round((st64["rk4"] + sqt64_ms) / (rk_ms + sqt_ms), 3)
+        res["err_vs_fp64"] = {"max": float(e.max()), "rms": float(np.sqrt((e ** 2).mean())),
+                              "floor": "relative to max_q |S(q,tau)| of each lag"}
+        log(f"  fp64 pass: {st64}")
+
+    if not args.no_cpu:
+        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        try:
+            cb = cpu_baseline(L, T0, nq, nlags, args.nsteps, threads)
+        except Exception as ex:  # the baseline is reported, never required
+            log(f"  cpu baseline failed: {ex!r}")
+            cb = None
+        if cb:
+            res["cpu_baseline"] = {
+                "value": round(cb["sqt_tflops"], 5), "unit": "TFLOP/s", "cores": threads, "kind": "reference",
+                "sample": f"reference kernel_fusion fused_dgemm at L={L}, K={3 * T0} (one q, one lag, with the "
+                          f"disconnected prefill; 1 call) and 200 reference runge_kutta steps at L={L}",
+                "rk4_spin_steps_per_s": cb["rk4_spin_steps_per_s"],
+                "projected_full_sqt_s": round(cb["projected_sqt_s"], 1),
+                "projected_full_rk4_s": round(cb["projected_rk4_s"], 2)}
+        else:
+            res["cpu_baseline"] = None
+    print(json.dumps(res), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -116,6 +116,16 @@ int asd_sqt_replicas(asd_ctx *ctx, int L, int nrep, const double *d_now,
                      const double *d_zero, const int *h_q, int nq, asd_precision prec,
                      double *out);
 
+/* ----------------------------------------------------------- profiling --- */
+/* Per-stage timing with HIP events recorded on the context's stream (off by
+ * default).  asd_stage_times() waits for the last recorded events and returns
+ * milliseconds of the most recent call of each stage:
+ *   t[0] operand packing, t[1] correlation GEMM + binning epilogue (one launch
+ *   covering every lag), t[2] per-snapshot Fourier sums, t[3] finishing (bins ->
+ *   S(q,tau), disconnected term), t[4] RK4 step loop (asd_rk4_run). */
+int asd_set_timing(asd_ctx *ctx, int on);
+int asd_stage_times(asd_ctx *ctx, double t[5]);
+
 /* ------------------------------------------------------------ batching --- */
 /* Independent trajectories of uneven L in ONE persistent launch per step
  * (work list of (trajectory, tile) items), step loop captured as a hipGraph.

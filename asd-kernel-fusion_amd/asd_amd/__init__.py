@@ -72,6 +72,9 @@ _SIGS = {
     "asd_sqt_set_config": (C.c_int, [_vp, C.c_int, C.c_int]),
     "asd_sqt_loss_grad": (C.c_int, [_vp, C.POINTER(Params), C.c_int, _vp, C.c_long, C.c_int, C.c_int, C.c_int,
                                     _vp, C.c_int, _vp, C.c_int, _dp, _dp]),
+    "asd_malloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
+    "asd_free": (C.c_int, [_vp, _vp]),
+    "asd_memcpy": (C.c_int, [_vp, _vp, _vp, C.c_size_t, C.c_int]),
     "asd_set_timing": (C.c_int, [_vp, C.c_int]),
     "asd_stage_times": (C.c_int, [_vp, _dp]),
     "asd_euler": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int, _vp, _vp]),
@@ -182,6 +185,19 @@ class Context:
             sp = C.c_void_p(snaps.data_ptr())
         _check(lib().asd_rk4_run(self.h, C.byref(p), L, nrep, C.c_void_p(spins.data_ptr()), nsteps, snap_every,
                                  nsnap, sp, 1 if use_graph else 0))
+
+    def batch_rk4_run(self, Ls, plist, spins_list, nsteps: int, snap_every: int = 0, nsnap: int = 0,
+                      snaps_list=None):
+        """Uneven-size trajectories in one persistent launch per step.  spins_list[t]:
+        cuda float64 [L_t*L_t, 3] (in/out); snaps_list[t]: [nsnap, L_t*L_t, 3] or None."""
+        n = len(Ls)
+        Ls_c = (C.c_int * n)(*Ls)
+        P = (Params * n)(*plist)
+        sp = (C.c_void_p * n)(*[s.data_ptr() for s in spins_list])
+        sn = None
+        if snaps_list is not None:
+            sn = (C.c_void_p * n)(*[s.data_ptr() for s in snaps_list])
+        _check(lib().asd_batch_rk4_run(self.h, n, Ls_c, P, sp, nsteps, snap_every, nsnap, sn))
 
     # ------------------------------------------------------------ S(q,t)
     def sqt(self, L: int, snaps, T0: int, nlags: int, q, precision="fp64", out=None):

@@ -3,6 +3,7 @@
 // host-pointer entries.
 #include <cmath>
 #include <cstdlib>
+#include <algorithm>
 #include <cstring>
 
 #include "asd_compat.h"
@@ -143,11 +144,12 @@ int asd_destroy(asd_ctx *c)
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+    if (c->bgraph.exec) (void)hipGraphExecDestroy(c->bgraph.exec);
     for (hipEvent_t &e : c->ev)
         if (e) (void)hipEventDestroy(e);
     DevBuf *bufs[] = {&c->state_b, &c->counter, &c->opA, &c->opB, &c->opA2, &c->opB2, &c->bins, &c->fq,
                       &c->twiddle, &c->qdev, &c->outdev, &c->stats, &c->c0, &c->c1, &c->c2, &c->c3, &c->c4,
-                      &c->c5, &c->c6, &c->work, &c->tabs, &c->adj0, &c->adj1, &c->adj2, &c->adj3, &c->adj4,
+                      &c->c5, &c->c6, &c->work, &c->tabs, &c->batch_b, &c->adj0, &c->adj1, &c->adj2, &c->adj3, &c->adj4,
                       &c->adj5};
     for (DevBuf *b : bufs) b->release();
     delete c;
@@ -161,6 +163,10 @@ int asd_set_stream(asd_ctx *c, void *stream)
     if (c->graph.exec) {
         (void)hipGraphExecDestroy(c->graph.exec);
         c->graph = RK4Graph();
+    }
+    if (c->bgraph.exec) {
+        (void)hipGraphExecDestroy(c->bgraph.exec);
+        c->bgraph = BatchGraph();
     }
     return ASD_OK;
 }
@@ -190,6 +196,36 @@ int asd_stage_times(asd_ctx *c, double t[5])
         }
         t[k] = ms;
     }
+    return ASD_OK;
+}
+
+int asd_malloc(asd_ctx *c, size_t bytes, void **dptr)
+{
+    ASD_CHECK(c && dptr, "NULL argument");
+    ASD_HIP(hipSetDevice(c->device));
+    hipError_t e = hipMalloc(dptr, bytes ? bytes : 1);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return set_error(ASD_ENOMEM, "hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    }
+    return ASD_OK;
+}
+
+int asd_free(asd_ctx *c, void *dptr)
+{
+    ASD_CHECK(c, "ctx is NULL");
+    ASD_HIP(hipStreamSynchronize(c->stream));
+    if (dptr) ASD_HIP(hipFree(dptr));
+    return ASD_OK;
+}
+
+int asd_memcpy(asd_ctx *c, void *dst, const void *src, size_t bytes, int kind)
+{
+    ASD_CHECK(c && dst && src, "NULL argument");
+    ASD_CHECK(kind >= 0 && kind <= 2, "kind must be 0 (H2D), 1 (D2H) or 2 (D2D)");
+    const hipMemcpyKind k[3] = {hipMemcpyHostToDevice, hipMemcpyDeviceToHost, hipMemcpyDeviceToDevice};
+    ASD_HIP(hipMemcpyAsync(dst, src, bytes, k[kind], c->stream));
+    ASD_HIP(hipStreamSynchronize(c->stream));
     return ASD_OK;
 }
 
@@ -357,9 +393,106 @@ int asd_autotune_sqt(asd_ctx *c, int L, int T0, asd_precision prec, int *best)
 int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *params, double *const *d_spins,
                       long nsteps, int snap_every, int nsnap, double *const *d_snaps)
 {
-    (void)c; (void)ntraj; (void)Ls; (void)params; (void)d_spins; (void)nsteps; (void)snap_every; (void)nsnap;
-    (void)d_snaps;
-    return set_error(ASD_ENOTSUP, "asd_batch_rk4_run: not built yet");
+    ASD_CHECK(c && ntraj >= 1 && Ls && params && d_spins, "bad arguments");
+    ASD_CHECK(nsteps >= 0, "nsteps < 0");
+    ASD_CHECK(!d_snaps || (snap_every >= 1 && nsnap >= 0), "snap_every must be >= 1");
+    ASD_HIP(hipSetDevice(c->device));
+    constexpr int T = 16;  // tile edge of rk4_batch_kernel
+    std::vector<BatchTraj> tab(ntraj);
+    std::vector<int2> items;
+    long total = 0;
+    for (int t = 0; t < ntraj; t++) {
+        ASD_TRY(check_L(Ls[t]));
+        ASD_TRY(check_params(&params[t]));
+        ASD_CHECK(d_spins[t], "d_spins[%d] is NULL", t);
+        total += 3L * Ls[t] * Ls[t];
+    }
+    ASD_TRY(c->batch_b.ensure(sizeof(double) * total));
+    ASD_TRY(c->counter.ensure(sizeof(int)));
+    long off = 0;
+    for (int t = 0; t < ntraj; t++) {
+        const int L = Ls[t], nt = (L + T - 1) / T;
+        tab[t].a = d_spins[t];
+        tab[t].b = c->batch_b.as<double>() + off;
+        tab[t].snaps = d_snaps ? d_snaps[t] : nullptr;
+        tab[t].p = make_llg(params[t]);
+        tab[t].L = L;
+        off += 3L * L * L;
+        for (int k = 0; k < nt * nt; k++) items.push_back(make_int2(t, k));
+    }
+    // interleave trajectories tile by tile so that every stretch of the work list
+    // mixes lattice sizes (balanced persistent workgroups)
+    std::stable_sort(items.begin(), items.end(), [](const int2 &x, const int2 &y) { return x.y < y.y; });
+    const int nitems = (int)items.size();
+    ASD_TRY(c->tabs.ensure(sizeof(BatchTraj) * ntraj));
+    ASD_TRY(c->work.ensure(sizeof(int2) * nitems));
+    hipStream_t st = c->stream;
+    ASD_HIP(hipMemcpyAsync(c->tabs.ptr, tab.data(), sizeof(BatchTraj) * ntraj, hipMemcpyHostToDevice, st));
+    ASD_HIP(hipMemcpyAsync(c->work.ptr, items.data(), sizeof(int2) * nitems, hipMemcpyHostToDevice, st));
+    ASD_HIP(hipStreamSynchronize(st));  // host tables go out of scope
+    const BatchTraj *dt = c->tabs.as<BatchTraj>();
+    const int2 *dw = c->work.as<int2>();
+    int *ctr = c->counter.as<int>();
+    ASD_TRY(mark(c, 5));
+    hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, ctr, 0);
+
+    const int every = d_snaps ? snap_every : 10;
+    const long nsnap_in_loop = d_snaps ? std::min<long>(nsnap, (nsteps + every - 1) / every) : 0;
+    const int chunk = every % 2 ? 2 * every : every;
+    long nchunks = nsteps / chunk;
+    if (d_snaps) nchunks = std::min<long>(nchunks, nsnap_in_loop / (chunk / every));
+    long s = 0;
+    if (nchunks >= 1) {
+        BatchGraph &g = c->bgraph;
+        const uint64_t key = ((uint64_t)chunk << 40) ^ ((uint64_t)every << 32) ^ ((uint64_t)(d_snaps != nullptr) << 31) ^
+                             (uint64_t)nitems;
+        if (!(g.exec && g.key == key && g.trajs == (const void *)dt && g.items == (const void *)dw)) {
+            if (g.exec) (void)hipGraphExecDestroy(g.exec);
+            g = BatchGraph();
+            hipStream_t cap;
+            ASD_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+            hipGraph_t graph;
+            ASD_HIP(hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal));
+            hipStream_t saved = c->stream;
+            c->stream = cap;
+            int r = ASD_OK;
+            for (int k = 0; k < chunk && r == ASD_OK; k++) {
+                const int snap_here = d_snaps && k % every == 0;
+                r = rk4_batch_launch(c, dt, dw, nitems, k & 1, ctr, snap_here, snap_here);
+            }
+            c->stream = saved;
+            hipError_t e1 = hipStreamEndCapture(cap, &graph);
+            (void)hipStreamDestroy(cap);
+            ASD_TRY(r);
+            ASD_HIP(e1);
+            hipError_t e2 = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            ASD_HIP(e2);
+            g.key = key;
+            g.trajs = dt;
+            g.items = dw;
+            g.chunk = chunk;
+            g.nitems = nitems;
+        }
+        for (long ch = 0; ch < nchunks; ch++) ASD_HIP(hipGraphLaunch(g.exec, st));
+        s = nchunks * chunk;
+    }
+    // remainder: the counter already holds the next snapshot slot
+    for (; s < nsteps; s++) {
+        const int snap_here = d_snaps && s % every == 0 && s / every < nsnap;
+        ASD_TRY(rk4_batch_launch(c, dt, dw, nitems, (int)(s & 1), ctr, snap_here, snap_here));
+    }
+    const bool odd = nsteps & 1;
+    for (int t = 0; t < ntraj; t++) {
+        const size_t bytes = sizeof(double) * 3 * (size_t)Ls[t] * Ls[t];
+        const double *fin = odd ? tab[t].b : tab[t].a;
+        if (d_snaps && d_snaps[t] && nsteps % every == 0 && nsteps / every < nsnap)
+            ASD_HIP(hipMemcpyAsync(d_snaps[t] + (nsteps / every) * 3L * Ls[t] * Ls[t], fin, bytes,
+                                   hipMemcpyDeviceToDevice, st));
+        if (odd) ASD_HIP(hipMemcpyAsync(d_spins[t], tab[t].b, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    ASD_TRY(mark(c, 6));
+    return ASD_OK;
 }
 
 int asd_sqt_loss_grad(asd_ctx *c, const asd_params *p, int L, const double *d_S0, long nsteps, int snap_every,

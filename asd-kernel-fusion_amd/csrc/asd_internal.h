@@ -50,6 +50,19 @@ struct LLGParams {
 };
 LLGParams make_llg(const asd_params &p);
 
+// one trajectory of a batch: state ping-pong a <-> b, snapshots [nsnap][L*L][3]
+struct BatchTraj {
+    double *a, *b, *snaps;
+    LLGParams p;
+    int L;
+};
+struct BatchGraph {
+    hipGraphExec_t exec = nullptr;
+    int chunk = 0, nitems = 0;
+    const void *trajs = nullptr, *items = nullptr;
+    uint64_t key = 0;
+};
+
 struct RK4Graph {
     hipGraphExec_t exec = nullptr;
     // key
@@ -72,8 +85,9 @@ struct asd_ctx {
     asd::DevBuf opA, opB, opA2, opB2, bins, fq, twiddle, qdev, outdev, stats;
     // compat (host-pointer) staging
     asd::DevBuf c0, c1, c2, c3, c4, c5, c6;
-    // batching
-    asd::DevBuf work, tabs;
+    // batching: work list, trajectory table, second state buffer
+    asd::DevBuf work, tabs, batch_b;
+    asd::BatchGraph bgraph;
     // autotuner choice per precision (-1 = automatic); tuned per (L, K)
     int forced_cfg[4] = {-1, -1, -1, -1};
     std::map<std::pair<int, int>, int> tuned[4];
@@ -88,6 +102,8 @@ namespace asd {
 // record stage event k on the context stream when timing is on
 int mark(asd_ctx *ctx, int k);
 // internal entry points shared between translation units
+int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items, int nitems, int parity,
+                     const int *snap_counter, int write_snap, int bump_counter);
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in,
                     double *out, double *snap_out, const int *snap_counter, long snap_stride,
                     int bump_counter);

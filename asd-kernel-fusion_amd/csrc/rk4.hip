@@ -61,43 +61,44 @@ __device__ __forceinline__ int wrap(int x, int L)
     return x < 0 ? x + L : x;
 }
 
-// One full RK4 step (+ renormalisation) per launch; a TR x TC output tile per
-// workgroup, one interior site per thread.  LDS holds the tile + halo three times
-// (S, the current stage input Y, the current increment K), structure-of-arrays.
+// One full RK4 step (+ renormalisation) of one TR x TC output tile, one interior
+// site per thread.  LDS holds the tile + halo three times (S, the current stage
+// input Y, the current increment K), structure-of-arrays.  `in`/`out`/`snap` point
+// at this lattice's [site][xyz] arrays; snap (may be null) receives the input state.
+template <int TR, int TC> struct TileLDS {
+    static constexpr int R = 4, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
+    double S[3][NR], Y[3][NR], K[3][NR];
+};
+
 template <int TR, int TC>
-__global__ __launch_bounds__(TR *TC) void rk4_step_kernel(LLGParams p, int L,
-                                                          const double *__restrict__ in,
-                                                          double *__restrict__ out,
-                                                          double *__restrict__ snap,
-                                                          const int *__restrict__ snap_counter,
-                                                          long snap_stride)
+__device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p, int L, int tile_r, int tile_c,
+                                         const double *__restrict__ in, double *__restrict__ out,
+                                         double *__restrict__ snap)
 {
     constexpr int R = 4, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC, NT = TR * TC;
-    __shared__ double sS[3][NR], sY[3][NR], sK[3][NR];
-
-    const long ns = (long)L * L;
-    const long base = (long)blockIdx.z * ns * 3;
-    const int r0 = blockIdx.y * TR - R, c0 = blockIdx.x * TC - R;
+    double(&sS)[3][NR] = sm.S;
+    double(&sY)[3][NR] = sm.Y;
+    double(&sK)[3][NR] = sm.K;
+    const int r0 = tile_r * TR - R, c0 = tile_c * TC - R;
     const int t = threadIdx.x;
 
     for (int idx = t; idx < NR; idx += NT) {
         const int gr = wrap(r0 + idx / RC, L), gc = wrap(c0 + idx % RC, L);
-        const double *s = in + base + 3 * ((long)gr * L + gc);
+        const double *s = in + 3 * ((long)gr * L + gc);
         const double x = s[0], y = s[1], z = s[2];
         sS[0][idx] = x; sS[1][idx] = y; sS[2][idx] = z;
         sY[0][idx] = x; sY[1][idx] = y; sY[2][idx] = z;
     }
     // this thread's interior site
     const int ir = t / TC, ic = t % TC;
-    const int grow = blockIdx.y * TR + ir, gcol = blockIdx.x * TC + ic;
+    const int grow = tile_r * TR + ir, gcol = tile_c * TC + ic;
     const bool owner = grow < L && gcol < L;
     const int own = (ir + R) * RC + (ic + R);
     __syncthreads();
 
     double ax, ay, az;  // cur = S + k1/6 + k2/3 + k3/3 + k4/6, accumulated left to right
     if (snap && owner) {
-        const long slot = snap_counter ? (long)*snap_counter : 0;
-        double *o = snap + slot * snap_stride + base + 3 * ((long)grow * L + gcol);
+        double *o = snap + 3 * ((long)grow * L + gcol);
         o[0] = sS[0][own]; o[1] = sS[1][own]; o[2] = sS[2][own];
     }
 
@@ -145,10 +146,50 @@ __global__ __launch_bounds__(TR *TC) void rk4_step_kernel(LLGParams p, int L,
     }
     if (owner) {
         const double nrm = sqrt(ax * ax + ay * ay + az * az);
-        double *o = out + base + 3 * ((long)grow * L + gcol);
+        double *o = out + 3 * ((long)grow * L + gcol);
         o[0] = p.len * ax / nrm;
         o[1] = p.len * ay / nrm;
         o[2] = p.len * az / nrm;
+    }
+}
+
+// single lattice (nrep replicas along blockIdx.z); snapshot slot from *snap_counter
+// when given (graph replay), else slot 0 of snap
+template <int TR, int TC>
+__global__ __launch_bounds__(TR *TC) void rk4_step_kernel(LLGParams p, int L,
+                                                          const double *__restrict__ in,
+                                                          double *__restrict__ out,
+                                                          double *__restrict__ snap,
+                                                          const int *__restrict__ snap_counter,
+                                                          long snap_stride)
+{
+    __shared__ TileLDS<TR, TC> sm;
+    const long base = (long)blockIdx.z * L * L * 3;
+    double *sn = nullptr;
+    if (snap) sn = snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride + base;
+    rk4_tile<TR, TC>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
+}
+
+// ------------------------------------------------------------------ batching
+// Uneven-size independent trajectories in one persistent launch per step: the
+// grid strides over a work list of (trajectory, tile) items; every trajectory
+// has its own L, parameters and state/snapshot buffers (BatchTraj table).
+template <int TR, int TC>
+__global__ __launch_bounds__(TR *TC) void rk4_batch_kernel(const BatchTraj *__restrict__ trajs,
+                                                           const int2 *__restrict__ items, int nitems,
+                                                           int parity, const int *__restrict__ snap_counter,
+                                                           int write_snap)
+{
+    __shared__ TileLDS<TR, TC> sm;
+    for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
+        const int2 w = items[it];
+        const BatchTraj &tr = trajs[w.x];
+        const int ntc = (tr.L + TC - 1) / TC;
+        const double *in = parity ? tr.b : tr.a;
+        double *out = parity ? tr.a : tr.b;
+        double *sn = (write_snap && tr.snaps) ? tr.snaps + (long)*snap_counter * 3L * tr.L * tr.L : nullptr;
+        __syncthreads();  // LDS of the previous item fully consumed
+        rk4_tile<TR, TC>(sm, tr.p, tr.L, w.y / ntc, w.y % ntc, in, out, sn);
     }
 }
 
@@ -230,6 +271,19 @@ int rk4_compat_stages(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const d
 }
 
 // ------------------------------------------------------------- step launcher
+
+int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items, int nitems, int parity,
+                     const int *snap_counter, int write_snap, int bump_counter)
+{
+    constexpr int TR = 16, TC = 16;
+    const int grid = std::min(nitems, 4 * std::max(ctx->num_cu, 1));
+    hipLaunchKernelGGL((rk4_batch_kernel<TR, TC>), dim3(grid), dim3(TR * TC), 0, ctx->stream, d_trajs, d_items,
+                       nitems, parity, snap_counter, write_snap);
+    if (bump_counter)
+        hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(1), 0, ctx->stream, (int *)snap_counter);
+    ASD_HIP(hipGetLastError());
+    return ASD_OK;
+}
 
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in, double *out,
                     double *snap_out, const int *snap_counter, long snap_stride, int bump_counter)

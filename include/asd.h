@@ -82,6 +82,13 @@ int asd_destroy(asd_ctx *ctx);
 int asd_set_stream(asd_ctx *ctx, void *stream);
 int asd_synchronize(asd_ctx *ctx);
 
+/* device memory for C hosts that do not include HIP headers (stream-ordered on
+ * the context's stream; asd_memcpy kind: 0 host->device, 1 device->host,
+ * 2 device->device; it returns after the copy has completed) */
+int asd_malloc(asd_ctx *ctx, size_t bytes, void **dptr);
+int asd_free(asd_ctx *ctx, void *dptr);
+int asd_memcpy(asd_ctx *ctx, void *dst, const void *src, size_t bytes, int kind);
+
 /* ----------------------------------------------- stage 1: RK4-LLG (FP64) --- */
 /* Native, device-resident replacement for the reference time loop's stage 1
  * (llg_sqt_fusion.c:348 runge_kutta + :350-361 renormalisation), fused into one

@@ -75,6 +75,8 @@ _SIGS = {
     "asd_malloc": (C.c_int, [_vp, C.c_size_t, C.POINTER(_vp)]),
     "asd_free": (C.c_int, [_vp, _vp]),
     "asd_memcpy": (C.c_int, [_vp, _vp, _vp, C.c_size_t, C.c_int]),
+    "asd_sqt_vjp": (C.c_int, [_vp, C.POINTER(Params), C.c_int, _vp, C.c_long, C.c_int, C.c_int, C.c_int, _vp,
+                              C.c_int, _vp, C.c_int, _dp, _vp]),
     "asd_set_timing": (C.c_int, [_vp, C.c_int]),
     "asd_stage_times": (C.c_int, [_vp, _dp]),
     "asd_euler": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int, _vp, _vp]),
@@ -198,6 +200,34 @@ class Context:
         if snaps_list is not None:
             sn = (C.c_void_p * n)(*[s.data_ptr() for s in snaps_list])
         _check(lib().asd_batch_rk4_run(self.h, n, Ls_c, P, sp, nsteps, snap_every, nsnap, sn))
+
+    # ------------------------------------------------------------ adjoint
+    def sqt_vjp(self, p: Params, L: int, S0, nsteps: int, every: int, T0: int, nlags: int, q, g=None,
+                checkpoint_every: int = 50):
+        """Returns (S(q,tau) complex [nlags, nq], grad[4] or None) for upstream g
+        (float64 [nlags, nq, 2], dL/dRe S and dL/dIm S)."""
+        q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
+        s = np.empty((nlags, len(q), 2))
+        grad = np.zeros(4)
+        gp = None
+        if g is not None:
+            g = np.ascontiguousarray(np.asarray(g, dtype=np.float64).reshape(nlags, len(q), 2))
+            gp = _np_ptr(g)
+        _check(lib().asd_sqt_vjp(self.h, C.byref(p), L, C.c_void_p(S0.data_ptr()), nsteps, every, T0, nlags,
+                                 _np_ptr(q), len(q), gp, checkpoint_every,
+                                 grad.ctypes.data_as(_dp), _np_ptr(s)))
+        return s[..., 0] + 1j * s[..., 1], (grad if g is not None else None)
+
+    def sqt_loss_grad(self, p: Params, L: int, S0, nsteps: int, every: int, T0: int, nlags: int, q, target,
+                      checkpoint_every: int = 50):
+        q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
+        t = np.ascontiguousarray(np.stack([np.real(target), np.imag(target)], axis=-1), dtype=np.float64)
+        loss = C.c_double()
+        grad = np.zeros(4)
+        _check(lib().asd_sqt_loss_grad(self.h, C.byref(p), L, C.c_void_p(S0.data_ptr()), nsteps, every, T0, nlags,
+                                       _np_ptr(q), len(q), _np_ptr(t), checkpoint_every, C.byref(loss),
+                                       grad.ctypes.data_as(_dp)))
+        return loss.value, grad
 
     # ------------------------------------------------------------ S(q,t)
     def sqt(self, L: int, snaps, T0: int, nlags: int, q, precision="fp64", out=None):

@@ -1,0 +1,353 @@
+// adjoint.hip -- hand-written discrete adjoint of RK4-LLG (with renormalisation)
+// and of S(q,tau), giving vector-Jacobian products d<g, S>/d theta for
+// theta = (J, D, B_z, alpha).
+//
+// Forward map per step (reference runge_kutta, kernel_fusion/llg_sqt_fusion.c:98-145,
+// and renormalisation :350-361):
+//   Y1 = S, Y2 = S + k1/2, Y3 = S + k2/2, Y4 = S + k3,  k_s = dt f(Y_s; theta)
+//   c = S + k1/6 + k2/3 + k3/3 + k4/6,   S' = len c / |c|
+// Reverse step (lambda' = dL/dS'):
+//   lambda_c = len (lambda' - c^ (c^ . lambda')) / |c|
+//   mu4 = lambda_c/6              -> lambda_Y4 = dt J_f(Y4)^T mu4
+//   mu3 = lambda_c/3 + lambda_Y4  -> lambda_Y3 = dt J_f(Y3)^T mu3
+//   mu2 = lambda_c/3 + lambda_Y3/2 -> lambda_Y2 = dt J_f(Y2)^T mu2
+//   mu1 = lambda_c/6 + lambda_Y2/2 -> lambda_Y1 = dt J_f(S)^T mu1
+//   lambda = lambda_c + lambda_Y1 + lambda_Y2 + lambda_Y3 + lambda_Y4
+//   theta_bar += dt sum_s (df/dtheta)(Y_s)^T mu_s
+// with f = (p + a S x p)/den, p = h x S, h = J sum_nn S_j + (2 D S_z + B_z) z,
+// a = alpha/len, den = 1 + alpha^2.  For mu at site i, nu = (mu + a mu x S)/den:
+//   dL/dh_i = S x nu                               (sent to the 4 neighbours times J)
+//   dL/dS_i (local) = nu x h + (a/den) p x mu + 2 D (S x nu)_z z
+//   dJ: (S x nu).sum_nn S ; dD: 2 (S x nu)_z S_z ; dB_z: (S x nu)_z ;
+//   dalpha: mu . ((S x p)/(len den) - 2 alpha f / den)
+// Stage inputs are recomputed from checkpoints with the forward kernels, so the
+// recomputed trajectory is bit-identical to the forward one.
+//
+// S(q,tau) side (the correlation): with upstream g(q,tau) = dL/dRe S + i dL/dIm S and
+// R = g/2, F_q(t) = sum_i S_i(t) e^{-iq.r_i}, Fbar_q(tau) = (1/T0) sum_t0 F_q(t0+tau):
+//   Lambda_q(t) = 1/(Ns T0) sum_tau { [0<=t-tau<T0] R(tau) (F(t-tau) - Fbar(0))
+//                                    + [t<T0] conj(R(tau)) (F(t+tau) - Fbar(tau)) }
+//   dL/dS_i(t) = 2 Re sum_q conj(Lambda_q(t)) e^{-iq.r_i}
+// i.e. the exact adjoint of the GEMM form G -> bins -> S(q,tau), written through the
+// factorisation e^{-iq.(r_i - r_j)} = e^{-iq.r_i} e^{iq.r_j} that the forward
+// epilogue's displacement bins evaluate.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "asd_internal.h"
+
+namespace asd {
+
+__device__ __forceinline__ int wrapi(int x, int L)
+{
+    x %= L;
+    return x < 0 ? x + L : x;
+}
+
+__device__ __forceinline__ void cross3(const double *a, const double *b, double *o)
+{
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// lambda_c = len (lam' - c^ (c^.lam')) / |c|; lamS = lambda_c
+__global__ void adj_renorm_kernel(long ns, double len, const double *__restrict__ c,
+                                  const double *__restrict__ lam_next, double *__restrict__ lamc,
+                                  double *__restrict__ lamS)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ns) return;
+    const double *cc = c + 3 * i, *l = lam_next + 3 * i;
+    const double nrm = sqrt(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
+    const double u[3] = {cc[0] / nrm, cc[1] / nrm, cc[2] / nrm};
+    const double d = u[0] * l[0] + u[1] * l[1] + u[2] * l[2];
+    for (int k = 0; k < 3; k++) {
+        const double v = len * (l[k] - u[k] * d) / nrm;
+        lamc[3 * i + k] = v;
+        lamS[3 * i + k] = v;
+    }
+}
+
+// per-site dL/dh (gh) and local dL/dS (gS) for mu = ca*lamc + cb*carry at stage input Y,
+// plus dt-weighted parameter partials accumulated into grad[4]
+__global__ __launch_bounds__(256) void adj_jt1_kernel(LLGParams p, int L, const double *__restrict__ Y,
+                                                      const double *__restrict__ lamc, double ca,
+                                                      const double *__restrict__ carry, double cb,
+                                                      double *__restrict__ gh, double *__restrict__ gS,
+                                                      double *__restrict__ grad)
+{
+    __shared__ double red[4][256];
+    const long ns = (long)L * L;
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    double part[4] = {0, 0, 0, 0};
+    if (i < ns) {
+        const int row = (int)(i / L), col = (int)(i % L);
+        const double *s = Y + 3 * i;
+        const long nb[4] = {(long)wrapi(row + 1, L) * L + col, (long)wrapi(row - 1, L) * L + col,
+                            (long)row * L + wrapi(col - 1, L), (long)row * L + wrapi(col + 1, L)};
+        double sn[3] = {0, 0, 0};
+        for (int j = 0; j < 4; j++)
+            for (int k = 0; k < 3; k++) sn[k] += Y[3 * nb[j] + k];
+        const double h[3] = {p.J * sn[0], p.J * sn[1], p.J * sn[2] + 2.0 * p.D * s[2] + p.Bz};
+        double pp[3], sxp[3], f[3], mu[3], mxs[3], nu[3], ghv[3], t1[3], t2[3];
+        cross3(h, s, pp);
+        cross3(s, pp, sxp);
+        for (int k = 0; k < 3; k++) f[k] = (pp[k] + p.a_over_len * sxp[k]) / p.den;
+        for (int k = 0; k < 3; k++) mu[k] = ca * lamc[3 * i + k] + (carry ? cb * carry[3 * i + k] : 0.0);
+        cross3(mu, s, mxs);
+        for (int k = 0; k < 3; k++) nu[k] = (mu[k] + p.a_over_len * mxs[k]) / p.den;
+        cross3(s, nu, ghv);
+        cross3(nu, h, t1);
+        cross3(pp, mu, t2);
+        for (int k = 0; k < 3; k++) {
+            gh[3 * i + k] = ghv[k];
+            gS[3 * i + k] = t1[k] + (p.a_over_len / p.den) * t2[k];
+        }
+        gS[3 * i + 2] += 2.0 * p.D * ghv[2];
+        part[0] = p.dt * (ghv[0] * sn[0] + ghv[1] * sn[1] + ghv[2] * sn[2]);
+        part[1] = p.dt * 2.0 * ghv[2] * s[2];
+        part[2] = p.dt * ghv[2];
+        double da = 0.0;
+        for (int k = 0; k < 3; k++) da += mu[k] * (sxp[k] / (p.len * p.den) - 2.0 * p.alpha * f[k] / p.den);
+        part[3] = p.dt * da;
+    }
+    for (int k = 0; k < 4; k++) red[k][threadIdx.x] = part[k];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 4) unsafeAtomicAdd(&grad[threadIdx.x], red[threadIdx.x][0]);
+}
+
+// lambda_Y(j) = dt (gS(j) + J sum_{i in nn(j)} gh(i)); lamS += lambda_Y; carry = lambda_Y
+__global__ void adj_jt2_kernel(LLGParams p, int L, const double *__restrict__ gh, const double *__restrict__ gS,
+                               double *__restrict__ lamS, double *__restrict__ carry)
+{
+    const long ns = (long)L * L;
+    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= ns) return;
+    const int row = (int)(j / L), col = (int)(j % L);
+    const long nb[4] = {(long)wrapi(row + 1, L) * L + col, (long)wrapi(row - 1, L) * L + col,
+                        (long)row * L + wrapi(col - 1, L), (long)row * L + wrapi(col + 1, L)};
+    for (int k = 0; k < 3; k++) {
+        double a = 0.0;
+        for (int n = 0; n < 4; n++) a += gh[3 * nb[n] + k];
+        const double v = p.dt * (gS[3 * j + k] + p.J * a);
+        lamS[3 * j + k] += v;
+        carry[3 * j + k] = v;
+    }
+}
+
+__global__ void add_kernel(long n, double *__restrict__ a, const double *__restrict__ b)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) a[i] += b[i];
+}
+
+// Fbar[tau][q][6] = (1/T0) sum_{t0<T0} F[t0+tau][q][6]
+__global__ void fbar_kernel(const double *__restrict__ F, int nq, int T0, int nlags, double *__restrict__ Fbar)
+{
+    const int q = blockIdx.x, tau = blockIdx.y, k = threadIdx.x;
+    if (k >= 6) return;
+    double s = 0.0;
+    for (int t0 = 0; t0 < T0; t0++) s += F[((long)(t0 + tau) * nq + q) * 6 + k];
+    Fbar[((long)tau * nq + q) * 6 + k] = s / (double)T0;
+}
+
+// Lambda[t][q][6] (3 complex) from R = g/2; grid (nq, nt), 256 threads over tau
+__global__ __launch_bounds__(256) void lambda_kernel(const double *__restrict__ g, const double *__restrict__ F,
+                                                     const double *__restrict__ Fbar, int nq, int T0, int nlags,
+                                                     int Ns, double *__restrict__ Lam)
+{
+    __shared__ double red[6][256];
+    const int q = blockIdx.x, t = blockIdx.y;
+    double acc[6] = {0, 0, 0, 0, 0, 0};
+    const double *F0 = Fbar + (long)q * 6;  // Fbar(0)
+    for (int tau = threadIdx.x; tau < nlags; tau += 256) {
+        const double rr = 0.5 * g[((long)tau * nq + q) * 2], ri = 0.5 * g[((long)tau * nq + q) * 2 + 1];
+        const int a = t - tau;
+        if (a >= 0 && a < T0) {
+            const double *Fa = F + ((long)a * nq + q) * 6;
+            for (int c = 0; c < 3; c++) {
+                const double xr = Fa[2 * c] - F0[2 * c], xi = Fa[2 * c + 1] - F0[2 * c + 1];
+                acc[2 * c] += rr * xr - ri * xi;
+                acc[2 * c + 1] += rr * xi + ri * xr;
+            }
+        }
+        if (t < T0) {
+            const double *Fb = F + ((long)(t + tau) * nq + q) * 6;
+            const double *Fbt = Fbar + ((long)tau * nq + q) * 6;
+            for (int c = 0; c < 3; c++) {
+                const double xr = Fb[2 * c] - Fbt[2 * c], xi = Fb[2 * c + 1] - Fbt[2 * c + 1];
+                acc[2 * c] += rr * xr + ri * xi;        // conj(R) x
+                acc[2 * c + 1] += rr * xi - ri * xr;
+            }
+        }
+    }
+    for (int k = 0; k < 6; k++) red[k][threadIdx.x] = acc[k];
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if ((int)threadIdx.x < st)
+            for (int k = 0; k < 6; k++) red[k][threadIdx.x] += red[k][threadIdx.x + st];
+        __syncthreads();
+    }
+    if (threadIdx.x < 6) Lam[((long)t * nq + q) * 6 + threadIdx.x] = red[threadIdx.x][0] / ((double)Ns * T0);
+}
+
+// dX[t][i][c] = 2 sum_q (Lr cos(q.r_i) - Li sin(q.r_i)); grid (ceil(Ns/256), nt)
+__global__ void dx_kernel(const double *__restrict__ Lam, const int *__restrict__ q, const double2 *__restrict__ tw,
+                          int nq, int L, double *__restrict__ dX)
+{
+    const int Ns = L * L;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x, t = blockIdx.y;
+    if (i >= Ns) return;
+    const int r = i / L, c = i - r * L;
+    double a[3] = {0, 0, 0};
+    for (int iq = 0; iq < nq; iq++) {
+        const int k = (int)(((long)q[2 * iq] * r + (long)q[2 * iq + 1] * c) % L);
+        const double2 e = tw[k];
+        const double *l = Lam + ((long)t * nq + iq) * 6;
+        for (int d = 0; d < 3; d++) a[d] += l[2 * d] * e.x - l[2 * d + 1] * e.y;
+    }
+    for (int d = 0; d < 3; d++) dX[((long)t * Ns + i) * 3 + d] = 2.0 * a[d];
+}
+
+static dim3 g1(long n) { return dim3((unsigned)((n + 255) / 256)); }
+
+}  // namespace asd
+
+using namespace asd;
+
+extern "C" int asd_sqt_vjp(asd_ctx *c, const asd_params *pp, int L, const double *d_S0, long nsteps, int every,
+                           int T0, int nlags, const int *h_q, int nq, const double *h_g, int checkpoint_every,
+                           double grad[4], double *h_sqt)
+{
+    ASD_CHECK(c && pp && d_S0 && h_q && (grad || !h_g), "NULL argument");
+    ASD_CHECK(L >= 2 && L <= 1024, "L out of range");
+    ASD_CHECK(every >= 1 && T0 >= 1 && nlags >= 1 && nq >= 1, "bad sizes");
+    ASD_CHECK(checkpoint_every >= 1, "checkpoint_every must be >= 1");
+    ASD_CHECK(pp->spin_length > 0 && pp->alpha >= 0, "bad parameters");
+    for (int i = 0; i < 2 * nq; i++) ASD_CHECK(h_q[i] >= 0 && h_q[i] < L, "q index out of [0, L)");
+    const int nsnap = T0 + nlags - 1;
+    ASD_CHECK((long)(nsnap - 1) * every <= nsteps, "nsteps too small for T0 + nlags - 1 snapshots");
+    ASD_HIP(hipSetDevice(c->device));
+    // checkpoint interval: a multiple of the snapshot interval
+    const int C = (checkpoint_every + every - 1) / every * every;
+    const long ns = (long)L * L, n = 3 * ns;
+    const long ncp = (nsteps + C - 1) / C + 1;
+    hipStream_t st = c->stream;
+    const LLGParams p = make_llg(*pp);
+
+    // workspace: snapshots | checkpoints | segment states | 8 stage buffers | 5 adjoint buffers | dX | grad
+    const long words = (long)nsnap * n + ncp * n + (long)C * n + 8 * n + 5 * n + (long)nsnap * n + 8;
+    ASD_TRY(c->adj0.ensure(sizeof(double) * words));
+    double *snaps = c->adj0.as<double>();
+    double *cps = snaps + (long)nsnap * n;
+    double *seg = cps + ncp * n;
+    double *Y2 = seg + (long)C * n, *Y3 = Y2 + n, *Y4 = Y3 + n, *cc = Y4 + n;
+    double *k1 = cc + n, *k2 = k1 + n, *k3 = k2 + n, *k4 = k3 + n;
+    double *lam = k4 + n, *lamc = lam + n, *carry = lamc + n, *gh = carry + n, *gS = gh + n;
+    double *dX = gS + n;
+    double *dgrad = dX + (long)nsnap * n;
+
+    // ---- forward with checkpoints (every C steps) and snapshots (every `every` steps)
+    ASD_TRY(c->adj1.ensure(sizeof(double) * n));
+    double *S = c->adj1.as<double>();
+    ASD_HIP(hipMemcpyAsync(S, d_S0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    for (long a = 0, k = 0; a < nsteps || k == 0; a += C, k++) {
+        ASD_HIP(hipMemcpyAsync(cps + k * n, S, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        if (a >= nsteps) break;
+        const long len = std::min<long>(C, nsteps - a);
+        const long s0 = a / every;  // first snapshot index of this segment
+        const int nsn = (int)std::max<long>(0, std::min<long>(nsnap - s0, len / every + 1));
+        ASD_TRY(asd_rk4_run(c, pp, L, 1, S, len, every, nsn, nsn > 0 ? snaps + s0 * n : nullptr, 0));
+        if (a + len >= nsteps) break;
+    }
+    // ---- S(q,tau) in FP64 (leaves F_q(t) of all snapshots in c->fq)
+    ASD_TRY(c->outdev.ensure(sizeof(double) * 2 * (long)nlags * nq));
+    ASD_TRY(sqt_run(c, L, snaps, snaps, n, nsnap, T0, nlags, h_q, nq, ASD_FP64, c->outdev.as<double>(), -1));
+    if (h_sqt)
+        ASD_HIP(hipMemcpyAsync(h_sqt, c->outdev.ptr, sizeof(double) * 2 * (long)nlags * nq, hipMemcpyDeviceToHost,
+                               st));
+    if (!h_g) {  // forward only
+        ASD_HIP(hipStreamSynchronize(st));
+        return ASD_OK;
+    }
+    // ---- adjoint of S(q,tau) -> dL/dX for every snapshot
+    ASD_TRY(c->adj2.ensure(sizeof(double) * (2L * nlags * nq + 6L * nlags * nq + 6L * nsnap * nq)));
+    double *gdev = c->adj2.as<double>();
+    double *Fbar = gdev + 2L * nlags * nq;
+    double *Lam = Fbar + 6L * nlags * nq;
+    ASD_HIP(hipMemcpyAsync(gdev, h_g, sizeof(double) * 2 * (long)nlags * nq, hipMemcpyHostToDevice, st));
+    const double *F = c->fq.as<double>();
+    hipLaunchKernelGGL(fbar_kernel, dim3(nq, nlags), dim3(64), 0, st, F, nq, T0, nlags, Fbar);
+    hipLaunchKernelGGL(lambda_kernel, dim3(nq, nsnap), dim3(256), 0, st, (const double *)gdev, F,
+                       (const double *)Fbar, nq, T0, nlags, (int)ns, Lam);
+    hipLaunchKernelGGL(dx_kernel, dim3((unsigned)((ns + 255) / 256), nsnap), dim3(256), 0, st,
+                       (const double *)Lam, c->qdev.as<int>(), c->twiddle.as<double2>(), nq, L, dX);
+    ASD_HIP(hipGetLastError());
+
+    // ---- reverse sweep over checkpointed segments
+    ASD_HIP(hipMemsetAsync(dgrad, 0, 4 * sizeof(double), st));
+    ASD_HIP(hipMemsetAsync(lam, 0, sizeof(double) * n, st));
+    if (nsteps % every == 0 && nsteps / every < nsnap)
+        ASD_HIP(hipMemcpyAsync(lam, dX + (nsteps / every) * n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    const long nseg = (nsteps + C - 1) / C;
+    for (long sg = nseg - 1; sg >= 0; sg--) {
+        const long a = sg * C, b = std::min<long>(a + C, nsteps);
+        ASD_HIP(hipMemcpyAsync(seg, cps + sg * n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        for (long m = 1; m < b - a; m++)
+            ASD_TRY(rk4_launch_step(c, p, L, 1, seg + (m - 1) * n, seg + m * n, nullptr, nullptr, 0, 0));
+        for (long m = b - a - 1; m >= 0; m--) {
+            const double *Sn = seg + m * n;
+            ASD_TRY(rk4_stages_store(c, p, L, Sn, Y2, Y3, Y4, cc, k1, k2, k3, k4));
+            // lam holds dL/dS_{n+1}; k1 is reused as dL/dS_n accumulator
+            double *lamS = k1;
+            hipLaunchKernelGGL(adj_renorm_kernel, g1(ns), dim3(256), 0, st, ns, p.len, (const double *)cc,
+                               (const double *)lam, lamc, lamS);
+            const double *Ys[4] = {Sn, Y2, Y3, Y4};
+            const double ca[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+            const double cb[4] = {0.5, 0.5, 1.0, 0.0};
+            for (int sidx = 3; sidx >= 0; sidx--) {
+                hipLaunchKernelGGL(adj_jt1_kernel, g1(ns), dim3(256), 0, st, p, L, Ys[sidx], (const double *)lamc,
+                                   ca[sidx], sidx == 3 ? (const double *)nullptr : (const double *)carry, cb[sidx],
+                                   gh, gS, dgrad);
+                hipLaunchKernelGGL(adj_jt2_kernel, g1(ns), dim3(256), 0, st, p, L, (const double *)gh,
+                                   (const double *)gS, lamS, carry);
+            }
+            const long nidx = a + m;
+            if (nidx % every == 0 && nidx / every < nsnap)
+                hipLaunchKernelGGL(add_kernel, g1(n), dim3(256), 0, st, n, lamS, (const double *)(dX + (nidx / every) * n));
+            ASD_HIP(hipMemcpyAsync(lam, lamS, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+        }
+    }
+    ASD_HIP(hipGetLastError());
+    ASD_HIP(hipMemcpyAsync(grad, dgrad, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
+    ASD_HIP(hipStreamSynchronize(st));
+    return ASD_OK;
+}
+
+extern "C" int asd_sqt_loss_grad(asd_ctx *c, const asd_params *p, int L, const double *d_S0, long nsteps,
+                                 int snap_every, int T0, int nlags, const int *h_q, int nq, const double *h_target,
+                                 int checkpoint_every, double *loss, double grad[4])
+{
+    ASD_CHECK(c && h_target && loss && grad, "NULL argument");
+    // forward once for S, then the VJP with g = 2 (S - S_target)
+    const long m = 2L * nlags * nq;
+    std::vector<double> s(m), g(m);
+    ASD_TRY(asd_sqt_vjp(c, p, L, d_S0, nsteps, snap_every, T0, nlags, h_q, nq, nullptr, checkpoint_every, nullptr,
+                        s.data()));
+    double l = 0.0;
+    for (long i = 0; i < m; i++) {
+        const double d = s[i] - h_target[i];
+        l += d * d;
+        g[i] = 2.0 * d;
+    }
+    *loss = l;
+    return asd_sqt_vjp(c, p, L, d_S0, nsteps, snap_every, T0, nlags, h_q, nq, g.data(), checkpoint_every, grad,
+                       nullptr);
+}

@@ -495,15 +495,6 @@ int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *pa
     return ASD_OK;
 }
 
-int asd_sqt_loss_grad(asd_ctx *c, const asd_params *p, int L, const double *d_S0, long nsteps, int snap_every,
-                      int T0, int nlags, const int *h_q, int nq, const double *h_target, int checkpoint_every,
-                      double *loss, double grad[4])
-{
-    (void)c; (void)p; (void)L; (void)d_S0; (void)nsteps; (void)snap_every; (void)T0; (void)nlags; (void)h_q;
-    (void)nq; (void)h_target; (void)checkpoint_every; (void)loss; (void)grad;
-    return set_error(ASD_ENOTSUP, "asd_sqt_loss_grad: not built yet");
-}
-
 // ------------------------------------------------ reference-compatible entries
 
 // one lazily created context per host thread for the host-pointer entries

@@ -270,6 +270,28 @@ int rk4_compat_stages(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const d
     return ASD_OK;
 }
 
+// Stage inputs of one RK4 step for the adjoint: Y2 = S + k1/2, Y3 = S + k2/2,
+// Y4 = S + k3 and the unnormalised c = S + k1/6 + k2/3 + k3/3 + k4/6, with the
+// same rounding steps as the fused kernel (so recomputed stages are bit-identical).
+int rk4_stages_store(asd_ctx *ctx, const LLGParams &p, int L, const double *S, double *Y2, double *Y3, double *Y4,
+                     double *c, double *k1, double *k2, double *k3, double *k4)
+{
+    const long ns = (long)L * L, n = ns * 3;
+    const dim3 gs((unsigned)((ns + 255) / 256)), ge((unsigned)((n + 255) / 256)), b(256);
+    hipStream_t st = ctx->stream;
+    hipLaunchKernelGGL(rhs_kernel, gs, b, 0, st, p, L, ns, S, k1, p.dt);
+    hipLaunchKernelGGL(axpy_stage_kernel, ge, b, 0, st, n, S, (const double *)k1, Y2, 1);
+    hipLaunchKernelGGL(rhs_kernel, gs, b, 0, st, p, L, ns, (const double *)Y2, k2, p.dt);
+    hipLaunchKernelGGL(axpy_stage_kernel, ge, b, 0, st, n, S, (const double *)k2, Y3, 1);
+    hipLaunchKernelGGL(rhs_kernel, gs, b, 0, st, p, L, ns, (const double *)Y3, k3, p.dt);
+    hipLaunchKernelGGL(axpy_stage_kernel, ge, b, 0, st, n, S, (const double *)k3, Y4, 0);
+    hipLaunchKernelGGL(rhs_kernel, gs, b, 0, st, p, L, ns, (const double *)Y4, k4, p.dt);
+    hipLaunchKernelGGL(rk4_combine_kernel, ge, b, 0, st, n, S, (const double *)k1, (const double *)k2,
+                       (const double *)k3, (const double *)k4, c);
+    ASD_HIP(hipGetLastError());
+    return ASD_OK;
+}
+
 // ------------------------------------------------------------- step launcher
 
 int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items, int nitems, int parity,

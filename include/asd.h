@@ -166,6 +166,15 @@ int asd_sqt_loss_grad(asd_ctx *ctx, const asd_params *p, int L, const double *d_
                       int nq, const double *h_target, int checkpoint_every,
                       double *loss, double grad[4]);
 
+/* Vector-Jacobian product of the same forward map (RK4 from d_S0, snapshots,
+ * FP64 S(q,tau)): grad[p] = sum_{lag,q} (g_re dRe S/dtheta_p + g_im dIm S/dtheta_p)
+ * for upstream h_g [nlags][nq][2] (host).  h_sqt (host, may be NULL) receives
+ * S(q,tau).  With h_g == NULL only the forward runs (grad may be NULL).
+ * checkpoint_every is rounded up to a multiple of snap_every. */
+int asd_sqt_vjp(asd_ctx *ctx, const asd_params *p, int L, const double *d_S0, long nsteps,
+                int snap_every, int T0, int nlags, const int *h_q, int nq, const double *h_g,
+                int checkpoint_every, double grad[4], double *h_sqt);
+
 /* ---------------------------------- reference-compatible entry points --- */
 /* Host-pointer entries with the reference's argument meaning; they copy to the
  * device and back on every call (for parity and drop-in use, not speed).

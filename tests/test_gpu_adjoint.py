@@ -1,0 +1,85 @@
+"""Hand-written adjoint (asd_sqt_vjp / asd_sqt_loss_grad / torch wrapper) against
+the oracle's FP64 tangent-linear model (tight) and central finite differences
+(loose, the trajectory is chaotic)."""
+import numpy as np
+import pytest
+
+from gpu_util import need_gpu, say
+
+
+def gxmg(L):
+    h = L // 2
+    return [(k, 0) for k in range(h)] + [(h, k) for k in range(h)] + [(h - k, h - k) for k in range(h)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("theta,ckpt", [((1.0, 0.1, 0.1, 0.01), 10), ((1.1, 0.12, 0.08, 0.0), 50),
+                                        ((0.9, 0.05, 0.2, 0.1), 5)])
+def test_vjp_matches_tangent_linear(gctx, orc, theta, ckpt):
+    torch = need_gpu()
+    import asd_amd
+    L, every, T0, nlags = 8, 5, 6, 4
+    nsnap = T0 + nlags - 1
+    nsteps = (nsnap - 1) * every + 3            # trailing steps past the last snapshot
+    q = gxmg(L)
+    S0 = orc.init_spins(3, L * L)
+    J, D, Bz, al = theta
+    say(f"adjoint theta={theta} checkpoint={ckpt}")
+    rng = np.random.default_rng(1)
+    g = rng.standard_normal((nlags, len(q), 2))
+    s_gpu, grad = gctx.sqt_vjp(asd_amd.params(J, D, Bz, al, 0.01), L, torch.tensor(S0, device="cuda"), nsteps,
+                               every, T0, nlags, q, g=g, checkpoint_every=ckpt)
+    snaps, dsnaps = orc.tlm_run(orc.params(J, D, Bz, al, 0.01), L, S0, nsteps, every, nsnap)
+    s_ref = orc.sqt_gemm(L, snaps, T0, nlags, q)
+    assert np.abs(s_gpu - s_ref).max() <= 1e-10 * np.abs(s_ref).max()
+    tlm = np.array([np.sum(g[..., 0] * d.real + g[..., 1] * d.imag)
+                    for d in (orc.tlm_sqt(L, snaps, dsnaps[i], T0, nlags, q) for i in range(4))])
+    say(f"  adjoint {grad}\n  tangent {tlm}")
+    assert np.abs(grad - tlm).max() <= 1e-9 * max(1.0, np.abs(tlm).max())
+
+
+@pytest.mark.gpu
+def test_loss_grad_against_finite_differences(gctx, orc):
+    torch = need_gpu()
+    import asd_amd
+    L, every, T0, nlags = 16, 10, 20, 10
+    nsteps = (T0 + nlags - 2) * every
+    q = gxmg(L)
+    S0 = torch.tensor(orc.init_spins(3, L * L), device="cuda")
+    ts = (1.1, 0.12, 0.08, 0.01)
+    target, _ = gctx.sqt_vjp(asd_amd.params(*ts, 0.01), L, S0, nsteps, every, T0, nlags, q)
+    th = [1.0, 0.1, 0.1, 0.01]
+    loss, grad = gctx.sqt_loss_grad(asd_amd.params(*th, 0.01), L, S0, nsteps, every, T0, nlags, q, target)
+    h = 1e-6
+    fd = []
+    for i in range(4):
+        tp, tm = list(th), list(th)
+        tp[i] += h
+        tm[i] -= h
+        sp, _ = gctx.sqt_vjp(asd_amd.params(*tp, 0.01), L, S0, nsteps, every, T0, nlags, q)
+        sm, _ = gctx.sqt_vjp(asd_amd.params(*tm, 0.01), L, S0, nsteps, every, T0, nlags, q)
+        fd.append((np.sum(np.abs(sp - target) ** 2) - np.sum(np.abs(sm - target) ** 2)) / (2 * h))
+    fd = np.array(fd)
+    say(f"  loss {loss:.6e}\n  adjoint {grad}\n  central FD {fd}")
+    assert np.abs(grad - fd).max() <= 1e-4 * np.abs(fd).max()
+
+
+@pytest.mark.gpu
+def test_torch_autograd_wrapper(gctx, orc):
+    torch = need_gpu()
+    import asd_amd
+    from asd_amd.autograd import SqtConfig, sqt_of_theta
+    L, every, T0, nlags = 8, 5, 6, 4
+    nsteps = (T0 + nlags - 2) * every
+    cfg = SqtConfig(ctx=gctx, L=L, nsteps=nsteps, every=every, T0=T0, nlags=nlags,
+                    q=np.array(gxmg(L), dtype=np.int32), checkpoint_every=10)
+    S0 = torch.tensor(orc.init_spins(5, L * L), device="cuda")
+    theta = torch.tensor([1.0, 0.1, 0.1, 0.01], dtype=torch.float64, requires_grad=True)
+    target = torch.zeros(nlags, len(cfg.q), 2, dtype=torch.float64)
+    s = sqt_of_theta(theta, S0, cfg)
+    loss = ((s - target) ** 2).sum()
+    loss.backward()
+    l2, g2 = gctx.sqt_loss_grad(asd_amd.params(1.0, 0.1, 0.1, 0.01, 0.01), L, S0, nsteps, every, T0, nlags, cfg.q,
+                                np.zeros((nlags, len(cfg.q))), checkpoint_every=10)
+    assert abs(loss.item() - l2) <= 1e-12 * abs(l2)
+    assert np.abs(theta.grad.numpy() - g2).max() <= 1e-10 * np.abs(g2).max()

@@ -103,6 +103,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison pass")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calib-copy", action="store_true",
+                    help="also run a plain 1 GiB-in / 1 GiB-out elementwise copy (HBM counter calibration)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -223,6 +225,13 @@ def main():
                 "projected_full_rk4_s": round(cb["projected_rk4_s"], 2)}
         else:
             res["cpu_baseline"] = None
+    if args.calib_copy:
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")      # 1 GiB
+        y = torch.empty_like(x)
+        for _ in range(2):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        res["calib_copy_bytes"] = {"read": x.numel() * 8, "write": y.numel() * 8}
     print(json.dumps(res), flush=True)
     ctx.close()
 

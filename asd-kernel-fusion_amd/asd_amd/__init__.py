@@ -66,7 +66,7 @@ _SIGS = {
     "asd_sqt": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int]),
     "asd_sqt_replicas": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "asd_batch_rk4_run": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_long, C.c_int, C.c_int, _vp]),
-    "asd_autotune_sqt": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _ip]),
+    "asd_autotune_sqt": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _ip, _dp]),
     "asd_sqt_num_configs": (C.c_int, [C.c_int]),
     "asd_sqt_config_name": (C.c_char_p, [C.c_int, C.c_int]),
     "asd_sqt_set_config": (C.c_int, [_vp, C.c_int, C.c_int]),
@@ -200,6 +200,21 @@ class Context:
         if snaps_list is not None:
             sn = (C.c_void_p * n)(*[s.data_ptr() for s in snaps_list])
         _check(lib().asd_batch_rk4_run(self.h, n, Ls_c, P, sp, nsteps, snap_every, nsnap, sn))
+
+    # ---------------------------------------------------------- autotuning
+    def autotune(self, L: int, T0: int, precision="fp16x2"):
+        """Times every tile configuration for (L, K=3 T0); returns (best, {name: ms})."""
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        n = lib().asd_sqt_num_configs(prec)
+        times = (C.c_double * n)()
+        best = C.c_int()
+        _check(lib().asd_autotune_sqt(self.h, L, T0, prec, C.byref(best), times))
+        names = [lib().asd_sqt_config_name(prec, i).decode() for i in range(n)]
+        return best.value, dict(zip(names, list(times)))
+
+    def set_config(self, precision, cfg: int):
+        prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
+        _check(lib().asd_sqt_set_config(self.h, prec, cfg))
 
     # ------------------------------------------------------------ adjoint
     def sqt_vjp(self, p: Params, L: int, S0, nsteps: int, every: int, T0: int, nlags: int, q, g=None,

@@ -369,11 +369,13 @@ int asd_sqt_replicas(asd_ctx *c, int L, int nrep, const double *d_now, const dou
     return ASD_OK;
 }
 
-int asd_sqt_num_configs(asd_precision prec) { return (prec >= ASD_FP64 && prec <= ASD_FP16X2) ? 1 : 0; }
+int asd_sqt_num_configs(asd_precision prec)
+{
+    return (prec >= ASD_FP64 && prec <= ASD_FP16X2) ? sqt_num_configs(prec) : 0;
+}
 const char *asd_sqt_config_name(asd_precision prec, int cfg)
 {
-    (void)prec;
-    return cfg == 0 ? "tile128x128" : "invalid";
+    return (prec >= ASD_FP64 && prec <= ASD_FP16X2) ? sqt_config_name(prec, cfg) : "invalid";
 }
 int asd_sqt_set_config(asd_ctx *c, asd_precision prec, int cfg)
 {
@@ -382,12 +384,13 @@ int asd_sqt_set_config(asd_ctx *c, asd_precision prec, int cfg)
     c->forced_cfg[prec] = cfg;
     return ASD_OK;
 }
-int asd_autotune_sqt(asd_ctx *c, int L, int T0, asd_precision prec, int *best)
+int asd_autotune_sqt(asd_ctx *c, int L, int T0, asd_precision prec, int *best, double *times_ms)
 {
     ASD_CHECK(c, "ctx NULL");
-    (void)L; (void)T0; (void)prec;
-    if (best) *best = 0;
-    return ASD_OK;
+    ASD_TRY(check_L(L));
+    ASD_CHECK(T0 >= 1 && prec >= ASD_FP64 && prec <= ASD_FP16X2, "bad arguments");
+    ASD_HIP(hipSetDevice(c->device));
+    return sqt_autotune(c, L, T0, prec, best, times_ms);
 }
 
 int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *params, double *const *d_spins,

@@ -112,6 +112,9 @@ int rk4_compat_stages(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const d
 int rk4_stages_store(asd_ctx *ctx, const LLGParams &p, int L, const double *S, double *Y2, double *Y3, double *Y4,
                      double *c, double *k1, double *k2, double *k3, double *k4);
 int rhs_launch(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *S, double *dS);
+int sqt_num_configs(asd_precision prec);
+const char *sqt_config_name(asd_precision prec, int cfg);
+int sqt_autotune(asd_ctx *ctx, int L, int T0, asd_precision prec, int *best, double *times_ms);
 int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_b, long ta_stride,
             int nsnap_a, int T0, int nlags, const int *h_q, int nq, asd_precision prec,
             double *d_out, int cfg);

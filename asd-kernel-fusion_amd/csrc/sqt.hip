@@ -96,21 +96,20 @@ struct Work {
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).  Give
 // each XCD a contiguous slice of the work list, and order tiles in groups of
-// GROUP tile-rows so that concurrently running tiles share A and B panels in the
+// `group` tile-rows so that concurrently running tiles share A and B panels in the
 // XCD's L2.
-__device__ __forceinline__ bool decode_work(int nt, int nlags, Work &w)
+__device__ __forceinline__ bool decode_work(int nti, int ntj, int nlags, int group, Work &w)
 {
-    constexpr int GROUP = 8;
     const int nb = gridDim.x, b = blockIdx.x;
     const int per = nb / 8;
     const long wid = (long)(b % 8) * per + b / 8;
-    const long per_lag = (long)nt * nt;
+    const long per_lag = (long)nti * ntj;
     if (wid >= per_lag * nlags) return false;
     w.lag = (int)(wid / per_lag);
     const int r = (int)(wid - (long)w.lag * per_lag);
-    const int span = GROUP * nt;
-    const int g = r / span, first = g * GROUP;
-    const int gsz = min(nt - first, GROUP);
+    const int span = group * ntj;
+    const int g = r / span, first = g * group;
+    const int gsz = min(nti - first, group);
     const int rr = r - g * span;
     w.ti = first + rr % gsz;
     w.tj = rr / gsz;
@@ -123,9 +122,9 @@ struct BinGeom {
     int drmin, nbins;
 };
 
-__device__ __forceinline__ BinGeom bin_geom(int i0, int j0, int Ns, int L)
+__device__ __forceinline__ BinGeom bin_geom(int i0, int j0, int Ns, int L, int BM = TILE, int BN = TILE)
 {
-    const int il = min(i0 + TILE, Ns) - 1, jl = min(j0 + TILE, Ns) - 1;
+    const int il = min(i0 + BM, Ns) - 1, jl = min(j0 + BN, Ns) - 1;
     const int ri0 = i0 / L, ri1 = il / L, rj0 = j0 / L, rj1 = jl / L;
     BinGeom g;
     g.drmin = ri0 - rj1;
@@ -153,6 +152,7 @@ struct GemmArgs {
     long ld;         // k-major: row stride in elements; i-major: row stride (k) in elements
     long copy_stride;  // i-major A: elements between the 8 shifted copies
     int L, Ns, Kpad, nt, nlags;
+    int group;       // tile-rows per group in the work order
     double *bins;    // [nlags][Ns]
 };
 
@@ -194,7 +194,7 @@ __global__ __launch_bounds__(256) void sqt_gemm_kmajor_kernel(GemmArgs args)
 
     const int nt_tiles = (args.Ns + TILE - 1) / TILE;
     Work w;
-    if (!decode_work(nt_tiles, args.nlags, w)) return;
+    if (!decode_work(nt_tiles, nt_tiles, args.nlags, args.group, w)) return;
     const int i0 = w.ti * TILE, j0 = w.tj * TILE;
     const long ld = args.ld;
     const T *Ap = static_cast<const T *>(args.A) + (long)(3 * w.lag) * ld + i0;
@@ -290,10 +290,12 @@ __global__ __launch_bounds__(256) void sqt_gemm_kmajor_kernel(GemmArgs args)
 // ------------------------------------------------------------- BF16X3 / FP16X2
 // i-major operands (k contiguous).  v_mfma_f32_32x32x16_{bf16,f16}: lane l holds
 // A[i=l%32][k=8(l/32)+e], B[k=8(l/32)+e][j=l%32] (e = 0..7); the 16 outputs of lane l
-// are D[8(v/4) + 4(l/32) + v%4][l%32].  256 threads = 2 x 2 waves, each wave 64 x 64 =
-// 2 x 2 blocks of 32 x 32; BK = 32 per LDS stage.
-//   BF16X3: acc += Ah Bh + Ah Bl + Al Bh
-//   FP16X2: acc_hi += Ah Bh;  acc_lo += Ah Bl + Al Bh;  G = acc_hi + 2^-11 acc_lo
+// are D[8(v/4) + 4(l/32) + v%4][l%32] (checked with a lane-layout probe on gfx950).
+// 256 threads = 2 x 2 waves, each wave WM x WN blocks of 32 x 32; BK = 32 per LDS stage.
+// One FP32 accumulator per output in both split precisions:
+//   BF16X3: acc = sum Ah Bh + Ah Bl + Al Bh                          G = acc
+//   FP16X2: acc = sum Ah (2^11 Bh) + Ah Bl + Al Bh  (lo parts carry 2^11)   G = 2^-11 acc
+// (2^11 Bh is formed in registers from the LDS fragment; exact, |Bh| <= 1).
 
 template <int PREC> struct SplitMfma;
 template <> struct SplitMfma<ASD_BF16X3> {
@@ -302,6 +304,8 @@ template <> struct SplitMfma<ASD_BF16X3> {
     {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ v_t hi_scaled(v_t b) { return b; }
+    static constexpr double out_scale = 1.0;
 };
 template <> struct SplitMfma<ASD_FP16X2> {
     typedef h8 v_t;
@@ -309,20 +313,24 @@ template <> struct SplitMfma<ASD_FP16X2> {
     {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
     }
+    static __device__ __forceinline__ v_t hi_scaled(v_t b) { return b * (_Float16)2048.0f; }
+    static constexpr double out_scale = 1.0 / 2048.0;
 };
 
-template <int PREC>
+template <int PREC, int WM, int WN>
 __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
 {
     constexpr int BK = 32, LDS_LD = BK + 8;  // halves per LDS row (pad 16 B)
+    constexpr int BM = 2 * WM * 32, BN = 2 * WN * 32;
     typedef typename SplitMfma<PREC>::v_t v_t;
-    // [buf][operand: Ah, Al, Bh, Bl][row][k]
-    __shared__ __attribute__((aligned(16))) uint16_t smem[2][4][TILE][LDS_LD];
+    // [buf][hi, lo][row][k]
+    __shared__ __attribute__((aligned(16))) uint16_t sA[2][2][BM][LDS_LD];
+    __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][BN][LDS_LD];
 
-    const int nt_tiles = (args.Ns + TILE - 1) / TILE;
+    const int nti = (args.Ns + BM - 1) / BM, ntj = (args.Ns + BN - 1) / BN;
     Work w;
-    if (!decode_work(nt_tiles, args.nlags, w)) return;
-    const int i0 = w.ti * TILE, j0 = w.tj * TILE;
+    if (!decode_work(nti, ntj, args.nlags, args.group, w)) return;
+    const int i0 = w.ti * BM, j0 = w.tj * BN;
     const long ld = args.ld;
     const long koff = 3L * w.lag, shift = koff & 7;
     const long acopy = shift * args.copy_stride + (long)i0 * ld + (koff - shift);
@@ -333,47 +341,48 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = wave >> 1, wj = wave & 1;
-    // global -> LDS: each operand tile is 128 rows x 32 halves = 8 KB = 256 threads x 32 B:
-    // thread -> row tid/2, halves (tid%2)*16 .. +16 (two 16-byte pieces)
-    const int gr = tid >> 1, gk = (tid & 1) * 16;
+    // global -> LDS: a row of one operand tile is 32 halves = 4 x 16 B; piece e of an
+    // operand is (row e/4, quarter e%4); thread t moves pieces t, t+256, ...
+    constexpr int PA = BM / 64, PB = BN / 64;  // pieces per thread per array
 
-    f16v acc[2][2], acc2[2][2];
+    f16v acc[WM][WN];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int a = 0; a < WM; a++)
 #pragma unroll
-        for (int b = 0; b < 2; b++)
+        for (int b = 0; b < WN; b++)
 #pragma unroll
-            for (int e = 0; e < 16; e++) {
-                acc[a][b][e] = 0.0f;
-                acc2[a][b][e] = 0.0f;
-            }
+            for (int e = 0; e < 16; e++) acc[a][b][e] = 0.0f;
 
-    // every row start is 16-byte aligned (ld % 8 == 0; A read from the shifted copy)
-    uint16_t ra[2][16], rb[2][16];
+    uint4 ra[2][PA], rb[2][PB];
     auto gload = [&](int kt) {
-        const long k0 = (long)kt * BK + gk;
-        const uint4 *pa0 = reinterpret_cast<const uint4 *>(Ah + (long)gr * ld + k0);
-        const uint4 *pa1 = reinterpret_cast<const uint4 *>(Al + (long)gr * ld + k0);
-        const uint4 *pb0 = reinterpret_cast<const uint4 *>(Bh + (long)gr * ld + k0);
-        const uint4 *pb1 = reinterpret_cast<const uint4 *>(Bl + (long)gr * ld + k0);
-        const uint4 a00 = pa0[0], a01 = pa0[1], a10 = pa1[0], a11 = pa1[1];
-        const uint4 b00 = pb0[0], b01 = pb0[1], b10 = pb1[0], b11 = pb1[1];
-        *reinterpret_cast<uint4 *>(&ra[0][0]) = a00;
-        *reinterpret_cast<uint4 *>(&ra[0][8]) = a01;
-        *reinterpret_cast<uint4 *>(&ra[1][0]) = a10;
-        *reinterpret_cast<uint4 *>(&ra[1][8]) = a11;
-        *reinterpret_cast<uint4 *>(&rb[0][0]) = b00;
-        *reinterpret_cast<uint4 *>(&rb[0][8]) = b01;
-        *reinterpret_cast<uint4 *>(&rb[1][0]) = b10;
-        *reinterpret_cast<uint4 *>(&rb[1][8]) = b11;
+        const long k0 = (long)kt * BK;
+#pragma unroll
+        for (int u = 0; u < PA; u++) {
+            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
+            const long off = (long)row * ld + k0 + qq * 8;
+            ra[0][u] = *reinterpret_cast<const uint4 *>(Ah + off);
+            ra[1][u] = *reinterpret_cast<const uint4 *>(Al + off);
+        }
+#pragma unroll
+        for (int u = 0; u < PB; u++) {
+            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
+            const long off = (long)row * ld + k0 + qq * 8;
+            rb[0][u] = *reinterpret_cast<const uint4 *>(Bh + off);
+            rb[1][u] = *reinterpret_cast<const uint4 *>(Bl + off);
+        }
     };
     auto sstore = [&](int buf) {
 #pragma unroll
-        for (int h = 0; h < 2; h++) {
-            *reinterpret_cast<uint4 *>(&smem[buf][h][gr][gk]) = *reinterpret_cast<uint4 *>(&ra[h][0]);
-            *reinterpret_cast<uint4 *>(&smem[buf][h][gr][gk + 8]) = *reinterpret_cast<uint4 *>(&ra[h][8]);
-            *reinterpret_cast<uint4 *>(&smem[buf][2 + h][gr][gk]) = *reinterpret_cast<uint4 *>(&rb[h][0]);
-            *reinterpret_cast<uint4 *>(&smem[buf][2 + h][gr][gk + 8]) = *reinterpret_cast<uint4 *>(&rb[h][8]);
+        for (int u = 0; u < PA; u++) {
+            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
+            *reinterpret_cast<uint4 *>(&sA[buf][0][row][qq * 8]) = ra[0][u];
+            *reinterpret_cast<uint4 *>(&sA[buf][1][row][qq * 8]) = ra[1][u];
+        }
+#pragma unroll
+        for (int u = 0; u < PB; u++) {
+            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
+            *reinterpret_cast<uint4 *>(&sB[buf][0][row][qq * 8]) = rb[0][u];
+            *reinterpret_cast<uint4 *>(&sB[buf][1][row][qq * 8]) = rb[1][u];
         }
     };
 
@@ -387,70 +396,107 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
 #pragma unroll
         for (int kk = 0; kk < BK / 16; kk++) {
             const int kc = kk * 16 + 8 * (lane / 32);
-            v_t ah[2], al[2], bh[2], bl[2];
+            v_t ah[WM], al[WM], bh[WN], bl[WN];
 #pragma unroll
-            for (int m = 0; m < 2; m++) {
-                const int r = wi * 64 + m * 32 + lane % 32;
-                ah[m] = *reinterpret_cast<const v_t *>(&smem[cur][0][r][kc]);
-                al[m] = *reinterpret_cast<const v_t *>(&smem[cur][1][r][kc]);
+            for (int m = 0; m < WM; m++) {
+                const int r = wi * WM * 32 + m * 32 + lane % 32;
+                ah[m] = *reinterpret_cast<const v_t *>(&sA[cur][0][r][kc]);
+                al[m] = *reinterpret_cast<const v_t *>(&sA[cur][1][r][kc]);
             }
 #pragma unroll
-            for (int n = 0; n < 2; n++) {
-                const int r = wj * 64 + n * 32 + lane % 32;
-                bh[n] = *reinterpret_cast<const v_t *>(&smem[cur][2][r][kc]);
-                bl[n] = *reinterpret_cast<const v_t *>(&smem[cur][3][r][kc]);
+            for (int n = 0; n < WN; n++) {
+                const int r = wj * WN * 32 + n * 32 + lane % 32;
+                bh[n] = *reinterpret_cast<const v_t *>(&sB[cur][0][r][kc]);
+                bl[n] = *reinterpret_cast<const v_t *>(&sB[cur][1][r][kc]);
             }
 #pragma unroll
-            for (int m = 0; m < 2; m++)
+            for (int n = 0; n < WN; n++) {
+                const v_t bhs = SplitMfma<PREC>::hi_scaled(bh[n]);
 #pragma unroll
-                for (int n = 0; n < 2; n++) {
-                    if (PREC == ASD_BF16X3) {
-                        acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bh[n], acc[m][n]);
-                        acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc[m][n]);
-                        acc[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc[m][n]);
-                    } else {
-                        acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bh[n], acc[m][n]);
-                        acc2[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc2[m][n]);
-                        acc2[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc2[m][n]);
-                    }
+                for (int m = 0; m < WM; m++) {
+                    acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bhs, acc[m][n]);
+                    acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc[m][n]);
+                    acc[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc[m][n]);
                 }
+            }
         }
         if (kt + 1 < nk) sstore(cur ^ 1);
         __syncthreads();
     }
 
     const int L = args.L, Ns = args.Ns;
-    const BinGeom g = bin_geom(i0, j0, Ns, L);
-    double *sb = reinterpret_cast<double *>(&smem[0][0][0][0]);
+    const BinGeom g = bin_geom(i0, j0, Ns, L, BM, BN);
+    double *sb = reinterpret_cast<double *>(&sA[0][0][0][0]);
     for (int b = tid; b < g.nbins; b += 256) sb[b] = 0.0;
-    int rj[2], cj[2];
+    int rj[WN], cj[WN];
 #pragma unroll
-    for (int n = 0; n < 2; n++) {
-        const int j = j0 + wj * 64 + n * 32 + lane % 32;
+    for (int n = 0; n < WN; n++) {
+        const int j = j0 + wj * WN * 32 + n * 32 + lane % 32;
         rj[n] = j / L;
         cj[n] = j - rj[n] * L;
     }
     __syncthreads();
 #pragma unroll
-    for (int m = 0; m < 2; m++)
+    for (int m = 0; m < WM; m++)
 #pragma unroll
         for (int v = 0; v < 16; v++) {
-            const int i = i0 + wi * 64 + m * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+            const int i = i0 + wi * WM * 32 + m * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
             if (i >= Ns) continue;
             const int ri = i / L, ci = i - ri * L;
 #pragma unroll
-            for (int n = 0; n < 2; n++) {
-                const int j = j0 + wj * 64 + n * 32 + lane % 32;
+            for (int n = 0; n < WN; n++) {
+                const int j = j0 + wj * WN * 32 + n * 32 + lane % 32;
                 if (j >= Ns) continue;
                 int dc = ci - cj[n];
                 dc += dc < 0 ? L : 0;
-                double val = (double)acc[m][n][v];
-                if (PREC == ASD_FP16X2) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
-                atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], val);
+                atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], (double)acc[m][n][v] * SplitMfma<PREC>::out_scale);
             }
         }
     __syncthreads();
     flush_bins(sb, g, L, args.bins + (long)w.lag * Ns);
+}
+
+// ------------------------------------------------------------- configurations
+// Tile configurations the autotuner chooses from (per precision).
+struct SqtCfg {
+    int wm, wn, group;
+    const char *name;
+};
+static const SqtCfg kSplitCfgs[] = {
+    {2, 2, 8, "128x128 tiles, 64x64 per wave, group 8"},
+    {2, 2, 16, "128x128 tiles, 64x64 per wave, group 16"},
+    {2, 4, 8, "128x256 tiles, 64x128 per wave, group 8"},
+    {4, 2, 8, "256x128 tiles, 128x64 per wave, group 8"},
+    {4, 2, 16, "256x128 tiles, 128x64 per wave, group 16"},
+};
+static const SqtCfg kKmajorCfgs[] = {
+    {2, 2, 8, "128x128 tiles, 64x64 per wave, group 8"},
+    {2, 2, 16, "128x128 tiles, 64x64 per wave, group 16"},
+    {2, 2, 4, "128x128 tiles, 64x64 per wave, group 4"},
+};
+constexpr int kNumSplit = sizeof(kSplitCfgs) / sizeof(kSplitCfgs[0]);
+constexpr int kNumKmajor = sizeof(kKmajorCfgs) / sizeof(kKmajorCfgs[0]);
+
+int sqt_num_configs(asd_precision prec) { return prec <= ASD_FP32 ? kNumKmajor : kNumSplit; }
+const char *sqt_config_name(asd_precision prec, int cfg)
+{
+    if (cfg < 0 || cfg >= sqt_num_configs(prec)) return "invalid";
+    return prec <= ASD_FP32 ? kKmajorCfgs[cfg].name : kSplitCfgs[cfg].name;
+}
+
+template <int PREC>
+static void launch_split(const SqtCfg &c, GemmArgs ga, hipStream_t st)
+{
+    const int BM = 64 * c.wm, BN = 64 * c.wn;
+    const long total = (long)((ga.Ns + BM - 1) / BM) * ((ga.Ns + BN - 1) / BN) * ga.nlags;
+    const dim3 grid((unsigned)((total + 7) / 8 * 8)), blk(256);
+    ga.group = c.group;
+    if (c.wm == 2 && c.wn == 2)
+        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 2>), grid, blk, 0, st, ga);
+    else if (c.wm == 2 && c.wn == 4)
+        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 4>), grid, blk, 0, st, ga);
+    else
+        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 4, 2>), grid, blk, 0, st, ga);
 }
 
 // ----------------------------------------------------------- FP64 finishing
@@ -576,6 +622,18 @@ __global__ __launch_bounds__(256) void finish_kernel(const double *__restrict__ 
     }
 }
 
+// deterministic unit-length pseudo-spins for timing runs (values do not matter)
+__global__ void fill_pseudo_spins_kernel(double *p, long nspins)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nspins; i += (long)gridDim.x * blockDim.x) {
+        const double a = 0.7548776662 * (double)i, z = 2.0 * (0.5698402910 * (double)i - floor(0.5698402910 * (double)i)) - 1.0;
+        const double r = sqrt(fmax(0.0, 1.0 - z * z));
+        p[3 * i] = r * cos(a);
+        p[3 * i + 1] = r * sin(a);
+        p[3 * i + 2] = z;
+    }
+}
+
 __global__ void zero_kernel(double *p, long n)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -596,11 +654,15 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
             int nsnap_a, int T0, int nlags, const int *h_q, int nq, asd_precision prec, double *d_out,
             int cfg)
 {
-    (void)cfg;
+    if (cfg < 0) {
+        auto it = ctx->tuned[prec].find(std::make_pair(L, 3 * T0));
+        cfg = it != ctx->tuned[prec].end() ? it->second : 0;
+    }
+    ASD_CHECK(cfg < sqt_num_configs(prec), "configuration %d out of range", cfg);
     const int Ns = L * L;
     const int K = 3 * T0;
     const int nt_tiles = (Ns + TILE - 1) / TILE;
-    const long ldi = round_up(Ns, TILE);  // padded site count
+    const long ldi = round_up(Ns, 2 * TILE);  // padded site count (every tile shape divides it)
     const int nt_a = T0 + nlags - 1;      // snapshots the A windows touch
     ASD_CHECK(nsnap_a >= nt_a, "need nsnap >= T0 + nlags - 1 (%d < %d)", nsnap_a, nt_a);
     hipStream_t st = ctx->stream;
@@ -632,6 +694,7 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
     ga.nlags = nlags;
     ga.bins = ctx->bins.as<double>();
     const int nblocks = grid_for(nt_tiles, nlags);
+    ga.group = kKmajorCfgs[prec <= ASD_FP32 ? cfg : 0].group;
 
     if (prec == ASD_FP64 || prec == ASD_FP32) {
         const size_t es = prec == ASD_FP64 ? 8 : 4;
@@ -683,9 +746,9 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ga.Kpad = (int)Kpad;
         ASD_TRY(mark(ctx, 1));
         if (prec == ASD_BF16X3)
-            hipLaunchKernelGGL(sqt_gemm_split_kernel<ASD_BF16X3>, dim3(nblocks), dim3(256), 0, st, ga);
+            launch_split<ASD_BF16X3>(kSplitCfgs[cfg], ga, st);
         else
-            hipLaunchKernelGGL(sqt_gemm_split_kernel<ASD_FP16X2>, dim3(nblocks), dim3(256), 0, st, ga);
+            launch_split<ASD_FP16X2>(kSplitCfgs[cfg], ga, st);
     }
     ASD_HIP(hipGetLastError());
     ASD_TRY(mark(ctx, 2));
@@ -709,6 +772,52 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
                        ctx->qdev.as<int>(), ctx->twiddle.as<double2>(), (const double *)Fa, Fz, nq, T0, d_out);
     ASD_HIP(hipGetLastError());
     ASD_TRY(mark(ctx, 4));
+    return ASD_OK;
+}
+
+int sqt_autotune(asd_ctx *ctx, int L, int T0, asd_precision prec, int *best, double *times_ms)
+{
+    const int nl = 8;                       // lags per timing run
+    const int nsnap = T0 + nl - 1;
+    const long n = 3L * L * L * nsnap;
+    DevBuf snaps, out;
+    ASD_TRY(snaps.ensure(sizeof(double) * n));
+    ASD_TRY(out.ensure(sizeof(double) * 2 * nl));
+    hipLaunchKernelGGL(fill_pseudo_spins_kernel, dim3(1024), dim3(256), 0, ctx->stream, snaps.as<double>(), n / 3);
+    const int q[2] = {0, 0};
+    const int saved_timing = ctx->timing;
+    for (hipEvent_t &e : ctx->ev)
+        if (!e) ASD_HIP(hipEventCreate(&e));
+    ctx->timing = 1;
+    double best_ms = 1e300;
+    int bi = 0;
+    int rc = ASD_OK;
+    for (int c = 0; c < sqt_num_configs(prec) && rc == ASD_OK; c++) {
+        double ms = 1e300;
+        for (int rep = 0; rep < 3 && rc == ASD_OK; rep++) {
+            rc = sqt_run(ctx, L, snaps.as<double>(), snaps.as<double>(), 3L * L * L, nsnap, T0, nl, q, 1, prec,
+                         out.as<double>(), c);
+            if (rc != ASD_OK) break;
+            float t = 0.f;
+            if (hipEventSynchronize(ctx->ev[2]) != hipSuccess || hipEventElapsedTime(&t, ctx->ev[1], ctx->ev[2]) != hipSuccess) {
+                rc = set_error(ASD_EHIP, "autotune timing failed");
+                break;
+            }
+            if (rep > 0) ms = std::min(ms, (double)t);
+        }
+        if (times_ms) times_ms[c] = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            bi = c;
+        }
+    }
+    ctx->timing = saved_timing;
+    (void)hipStreamSynchronize(ctx->stream);
+    snaps.release();
+    out.release();
+    if (rc != ASD_OK) return rc;
+    ctx->tuned[prec][std::make_pair(L, 3 * T0)] = bi;
+    if (best) *best = bi;
     return ASD_OK;
 }
 

@@ -143,6 +143,11 @@ def main():
         return st
 
     log(f"bench: L={L} nsteps={args.nsteps} nsnap={nsnap} T0={T0} nlags={nlags} nq={nq} prec={args.precision}")
+    tuned = {}
+    for prec in ([args.precision] + ([] if args.no_fp64 else ["fp64"])):
+        best, times = ctx.autotune(L, T0, prec)
+        tuned[prec] = {"chosen": list(times)[best], "gemm_ms_8_lags": {k: round(v, 3) for k, v in times.items()}}
+        log(f"  autotune {prec}: {tuned[prec]}")
     for w in range(args.warmup):
         one_pass(args.precision, out_split)
         torch.cuda.synchronize()
@@ -181,6 +186,7 @@ def main():
                      "kernel": f"sqt_gemm_split_kernel<{args.precision}> (all {nlags} lags in one launch)",
                      "mfma_products_per_flop": 3},
         "stage_ms": {k: round(v, 3) for k, v in avg.items()},
+        "autotune": tuned,
         "rk4_spin_steps_per_s": rk_rate,
         "rk4_hbm_frac": round(48.0 * rk_rate / (HBM_PEAK_GBS * 1e9), 5),
         "sqt_s": round(sqt_ms / 1e3, 4),

@@ -143,10 +143,12 @@ int asd_batch_rk4_run(asd_ctx *ctx, int ntraj, const int *Ls, const asd_params *
                       double *const *d_snaps);
 
 /* -------------------------------------------------------- autotuning --- */
-/* Times the correlation-GEMM tile configurations for (L, K = 3 T0, prec) on
- * this device and records the fastest; later asd_sqt calls with that shape use
- * it.  best_cfg receives the chosen configuration index (may be NULL). */
-int asd_autotune_sqt(asd_ctx *ctx, int L, int T0, asd_precision prec, int *best_cfg);
+/* Times every correlation-GEMM tile configuration for (L, K = 3 T0, prec) on
+ * this device (8 lags of synthetic snapshots, best of 2 timed runs each) and
+ * records the fastest in the context; later asd_sqt calls with that (L, K) use it
+ * unless a configuration is forced.  best_cfg (may be NULL) receives its index,
+ * times_ms (may be NULL, asd_sqt_num_configs(prec) entries) the GEMM times. */
+int asd_autotune_sqt(asd_ctx *ctx, int L, int T0, asd_precision prec, int *best_cfg, double *times_ms);
 /* number of tile configurations for a precision, and a printable name */
 int asd_sqt_num_configs(asd_precision prec);
 const char *asd_sqt_config_name(asd_precision prec, int cfg);

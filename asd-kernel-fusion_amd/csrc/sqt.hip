@@ -292,10 +292,14 @@ __global__ __launch_bounds__(256) void sqt_gemm_kmajor_kernel(GemmArgs args)
 // A[i=l%32][k=8(l/32)+e], B[k=8(l/32)+e][j=l%32] (e = 0..7); the 16 outputs of lane l
 // are D[8(v/4) + 4(l/32) + v%4][l%32] (checked with a lane-layout probe on gfx950).
 // 256 threads = 2 x 2 waves, each wave WM x WN blocks of 32 x 32; BK = 32 per LDS stage.
-// One FP32 accumulator per output in both split precisions:
-//   BF16X3: acc = sum Ah Bh + Ah Bl + Al Bh                          G = acc
-//   FP16X2: acc = sum Ah (2^11 Bh) + Ah Bl + Al Bh  (lo parts carry 2^11)   G = 2^-11 acc
-// (2^11 Bh is formed in registers from the LDS fragment; exact, |Bh| <= 1).
+//   BF16X3: acc = sum Ah Bh + Ah Bl + Al Bh                    G = acc
+//   FP16X2: acc = sum Ah Bh; acc2 = sum Ah Bl + Al Bh (lo parts carry 2^11)
+//           G = acc + 2^-11 acc2.  One accumulator with the hi products scaled by
+//           2^11 instead was measured ~20x less accurate at K = 1200: the MFMA's
+//           internal sum does not keep the small cross terms next to the large ones.
+// The three product passes run outermost so that consecutive MFMAs write
+// independent accumulators (back-to-back MFMAs on one accumulator stall on its
+// latency).
 
 template <int PREC> struct SplitMfma;
 template <> struct SplitMfma<ASD_BF16X3> {
@@ -304,8 +308,7 @@ template <> struct SplitMfma<ASD_BF16X3> {
     {
         return __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, b, c, 0, 0, 0);
     }
-    static __device__ __forceinline__ v_t hi_scaled(v_t b) { return b; }
-    static constexpr double out_scale = 1.0;
+    static constexpr bool dual = false;
 };
 template <> struct SplitMfma<ASD_FP16X2> {
     typedef h8 v_t;
@@ -313,8 +316,7 @@ template <> struct SplitMfma<ASD_FP16X2> {
     {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
     }
-    static __device__ __forceinline__ v_t hi_scaled(v_t b) { return b * (_Float16)2048.0f; }
-    static constexpr double out_scale = 1.0 / 2048.0;
+    static constexpr bool dual = true;
 };
 
 template <int PREC, int WM, int WN>
@@ -345,13 +347,22 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
     // operand is (row e/4, quarter e%4); thread t moves pieces t, t+256, ...
     constexpr int PA = BM / 64, PB = BN / 64;  // pieces per thread per array
 
-    f16v acc[WM][WN];
+    constexpr bool DUAL = SplitMfma<PREC>::dual;
+    f16v acc[WM][WN], acc2[DUAL ? WM : 1][DUAL ? WN : 1];
 #pragma unroll
     for (int a = 0; a < WM; a++)
 #pragma unroll
         for (int b = 0; b < WN; b++)
 #pragma unroll
             for (int e = 0; e < 16; e++) acc[a][b][e] = 0.0f;
+    if constexpr (DUAL) {
+#pragma unroll
+        for (int a = 0; a < WM; a++)
+#pragma unroll
+            for (int b = 0; b < WN; b++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc2[a][b][e] = 0.0f;
+    }
 
     uint4 ra[2][PA], rb[2][PB];
     auto gload = [&](int kt) {
@@ -410,14 +421,27 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
                 bl[n] = *reinterpret_cast<const v_t *>(&sB[cur][1][r][kc]);
             }
 #pragma unroll
-            for (int n = 0; n < WN; n++) {
-                const v_t bhs = SplitMfma<PREC>::hi_scaled(bh[n]);
+            for (int m = 0; m < WM; m++)
 #pragma unroll
-                for (int m = 0; m < WM; m++) {
-                    acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bhs, acc[m][n]);
-                    acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc[m][n]);
-                    acc[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc[m][n]);
-                }
+                for (int n = 0; n < WN; n++) acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bh[n], acc[m][n]);
+            if constexpr (DUAL) {
+#pragma unroll
+                for (int m = 0; m < WM; m++)
+#pragma unroll
+                    for (int n = 0; n < WN; n++) acc2[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc2[m][n]);
+#pragma unroll
+                for (int m = 0; m < WM; m++)
+#pragma unroll
+                    for (int n = 0; n < WN; n++) acc2[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc2[m][n]);
+            } else {
+#pragma unroll
+                for (int m = 0; m < WM; m++)
+#pragma unroll
+                    for (int n = 0; n < WN; n++) acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc[m][n]);
+#pragma unroll
+                for (int m = 0; m < WM; m++)
+#pragma unroll
+                    for (int n = 0; n < WN; n++) acc[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc[m][n]);
             }
         }
         if (kt + 1 < nk) sstore(cur ^ 1);
@@ -449,7 +473,9 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
                 if (j >= Ns) continue;
                 int dc = ci - cj[n];
                 dc += dc < 0 ? L : 0;
-                atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], (double)acc[m][n][v] * SplitMfma<PREC>::out_scale);
+                double val = (double)acc[m][n][v];
+                if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
+                atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], val);
             }
         }
     __syncthreads();

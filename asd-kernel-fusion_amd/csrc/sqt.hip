@@ -364,46 +364,41 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
                 for (int e = 0; e < 16; e++) acc2[a][b][e] = 0.0f;
     }
 
-    uint4 ra[2][PA], rb[2][PB];
-    auto gload = [&](int kt) {
+    // two register stages: the loads for iteration kt+2 are issued while kt is
+    // computed, so every global load has about two iterations to arrive
+    uint4 ra[2][2][PA], rb[2][2][PB];  // [stage][hi, lo][piece]
+    auto gload = [&](int kt, uint4 (&xa)[2][PA], uint4 (&xb)[2][PB]) {
         const long k0 = (long)kt * BK;
 #pragma unroll
         for (int u = 0; u < PA; u++) {
             const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
             const long off = (long)row * ld + k0 + qq * 8;
-            ra[0][u] = *reinterpret_cast<const uint4 *>(Ah + off);
-            ra[1][u] = *reinterpret_cast<const uint4 *>(Al + off);
+            xa[0][u] = *reinterpret_cast<const uint4 *>(Ah + off);
+            xa[1][u] = *reinterpret_cast<const uint4 *>(Al + off);
         }
 #pragma unroll
         for (int u = 0; u < PB; u++) {
             const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
             const long off = (long)row * ld + k0 + qq * 8;
-            rb[0][u] = *reinterpret_cast<const uint4 *>(Bh + off);
-            rb[1][u] = *reinterpret_cast<const uint4 *>(Bl + off);
+            xb[0][u] = *reinterpret_cast<const uint4 *>(Bh + off);
+            xb[1][u] = *reinterpret_cast<const uint4 *>(Bl + off);
         }
     };
-    auto sstore = [&](int buf) {
+    auto sstore = [&](int buf, const uint4 (&xa)[2][PA], const uint4 (&xb)[2][PB]) {
 #pragma unroll
         for (int u = 0; u < PA; u++) {
             const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
-            *reinterpret_cast<uint4 *>(&sA[buf][0][row][qq * 8]) = ra[0][u];
-            *reinterpret_cast<uint4 *>(&sA[buf][1][row][qq * 8]) = ra[1][u];
+            *reinterpret_cast<uint4 *>(&sA[buf][0][row][qq * 8]) = xa[0][u];
+            *reinterpret_cast<uint4 *>(&sA[buf][1][row][qq * 8]) = xa[1][u];
         }
 #pragma unroll
         for (int u = 0; u < PB; u++) {
             const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
-            *reinterpret_cast<uint4 *>(&sB[buf][0][row][qq * 8]) = rb[0][u];
-            *reinterpret_cast<uint4 *>(&sB[buf][1][row][qq * 8]) = rb[1][u];
+            *reinterpret_cast<uint4 *>(&sB[buf][0][row][qq * 8]) = xb[0][u];
+            *reinterpret_cast<uint4 *>(&sB[buf][1][row][qq * 8]) = xb[1][u];
         }
     };
-
-    const int nk = args.Kpad / BK;
-    gload(0);
-    sstore(0);
-    __syncthreads();
-    for (int kt = 0; kt < nk; kt++) {
-        const int cur = kt & 1;
-        if (kt + 1 < nk) gload(kt + 1);
+    auto compute = [&](int cur) {
 #pragma unroll
         for (int kk = 0; kk < BK / 16; kk++) {
             const int kc = kk * 16 + 8 * (lane / 32);
@@ -423,28 +418,35 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
 #pragma unroll
             for (int m = 0; m < WM; m++)
 #pragma unroll
-                for (int n = 0; n < WN; n++) acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bh[n], acc[m][n]);
-            if constexpr (DUAL) {
-#pragma unroll
-                for (int m = 0; m < WM; m++)
-#pragma unroll
-                    for (int n = 0; n < WN; n++) acc2[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc2[m][n]);
-#pragma unroll
-                for (int m = 0; m < WM; m++)
-#pragma unroll
-                    for (int n = 0; n < WN; n++) acc2[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc2[m][n]);
-            } else {
-#pragma unroll
-                for (int m = 0; m < WM; m++)
-#pragma unroll
-                    for (int n = 0; n < WN; n++) acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc[m][n]);
-#pragma unroll
-                for (int m = 0; m < WM; m++)
-#pragma unroll
-                    for (int n = 0; n < WN; n++) acc[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc[m][n]);
-            }
+                for (int n = 0; n < WN; n++) {
+                    acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bh[n], acc[m][n]);
+                    if constexpr (DUAL) {
+                        acc2[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc2[m][n]);
+                        acc2[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc2[m][n]);
+                    } else {
+                        acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc[m][n]);
+                        acc[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc[m][n]);
+                    }
+                }
         }
-        if (kt + 1 < nk) sstore(cur ^ 1);
+    };
+
+    const int nk = args.Kpad / BK;
+    gload(0, ra[0], rb[0]);
+    if (nk > 1) gload(1, ra[1], rb[1]);
+    sstore(0, ra[0], rb[0]);
+    __syncthreads();
+    // iteration kt computes LDS buffer kt&1, issues loads for kt+2 into register
+    // stage kt&1 (free: stored at the end of kt-1) and stores stage (kt+1)&1 into LDS
+    for (int kt = 0; kt < nk; kt += 2) {
+        if (kt + 2 < nk) gload(kt + 2, ra[0], rb[0]);
+        compute(0);
+        if (kt + 1 < nk) sstore(1, ra[1], rb[1]);
+        __syncthreads();
+        if (kt + 1 >= nk) break;
+        if (kt + 3 < nk) gload(kt + 3, ra[1], rb[1]);
+        compute(1);
+        if (kt + 2 < nk) sstore(0, ra[0], rb[0]);
         __syncthreads();
     }
 
@@ -495,6 +497,9 @@ static const SqtCfg kSplitCfgs[] = {
     {4, 2, 8, "256x128 tiles, 128x64 per wave, group 8"},
     {4, 2, 16, "256x128 tiles, 128x64 per wave, group 16"},
 };
+// FP16X2 keeps two accumulators: the 64x128 / 128x64 wave tiles do not fit the
+// register file with the two-stage prefetch, so only 64x64 wave tiles are offered
+constexpr int kNumSplitFp16 = 2;
 static const SqtCfg kKmajorCfgs[] = {
     {2, 2, 8, "128x128 tiles, 64x64 per wave, group 8"},
     {2, 2, 16, "128x128 tiles, 64x64 per wave, group 16"},
@@ -503,7 +508,10 @@ static const SqtCfg kKmajorCfgs[] = {
 constexpr int kNumSplit = sizeof(kSplitCfgs) / sizeof(kSplitCfgs[0]);
 constexpr int kNumKmajor = sizeof(kKmajorCfgs) / sizeof(kKmajorCfgs[0]);
 
-int sqt_num_configs(asd_precision prec) { return prec <= ASD_FP32 ? kNumKmajor : kNumSplit; }
+int sqt_num_configs(asd_precision prec)
+{
+    return prec <= ASD_FP32 ? kNumKmajor : (prec == ASD_FP16X2 ? kNumSplitFp16 : kNumSplit);
+}
 const char *sqt_config_name(asd_precision prec, int cfg)
 {
     if (cfg < 0 || cfg >= sqt_num_configs(prec)) return "invalid";
@@ -517,12 +525,16 @@ static void launch_split(const SqtCfg &c, GemmArgs ga, hipStream_t st)
     const long total = (long)((ga.Ns + BM - 1) / BM) * ((ga.Ns + BN - 1) / BN) * ga.nlags;
     const dim3 grid((unsigned)((total + 7) / 8 * 8)), blk(256);
     ga.group = c.group;
-    if (c.wm == 2 && c.wn == 2)
+    if constexpr (PREC == ASD_FP16X2) {
         hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 2>), grid, blk, 0, st, ga);
-    else if (c.wm == 2 && c.wn == 4)
-        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 4>), grid, blk, 0, st, ga);
-    else
-        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 4, 2>), grid, blk, 0, st, ga);
+    } else {
+        if (c.wm == 2 && c.wn == 2)
+            hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 2>), grid, blk, 0, st, ga);
+        else if (c.wm == 2 && c.wn == 4)
+            hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 4>), grid, blk, 0, st, ga);
+        else
+            hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 4, 2>), grid, blk, 0, st, ga);
+    }
 }
 
 // ----------------------------------------------------------- FP64 finishing

@@ -32,6 +32,9 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+// 16-byte staging vector: a native clang vector (HIP's uint4 is a struct with union
+// members, which arrays of do not get promoted to registers and end up in scratch)
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int TILE = 128;  // BM = BN for every configuration
 
@@ -365,90 +368,88 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
     }
 
     // two register stages: the loads for iteration kt+2 are issued while kt is
-    // computed, so every global load has about two iterations to arrive
-    uint4 ra[2][2][PA], rb[2][2][PB];  // [stage][hi, lo][piece]
-    auto gload = [&](int kt, uint4 (&xa)[2][PA], uint4 (&xb)[2][PB]) {
-        const long k0 = (long)kt * BK;
-#pragma unroll
-        for (int u = 0; u < PA; u++) {
-            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
-            const long off = (long)row * ld + k0 + qq * 8;
-            xa[0][u] = *reinterpret_cast<const uint4 *>(Ah + off);
-            xa[1][u] = *reinterpret_cast<const uint4 *>(Al + off);
-        }
-#pragma unroll
-        for (int u = 0; u < PB; u++) {
-            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
-            const long off = (long)row * ld + k0 + qq * 8;
-            xb[0][u] = *reinterpret_cast<const uint4 *>(Bh + off);
-            xb[1][u] = *reinterpret_cast<const uint4 *>(Bl + off);
-        }
-    };
-    auto sstore = [&](int buf, const uint4 (&xa)[2][PA], const uint4 (&xb)[2][PB]) {
-#pragma unroll
-        for (int u = 0; u < PA; u++) {
-            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
-            *reinterpret_cast<uint4 *>(&sA[buf][0][row][qq * 8]) = xa[0][u];
-            *reinterpret_cast<uint4 *>(&sA[buf][1][row][qq * 8]) = xa[1][u];
-        }
-#pragma unroll
-        for (int u = 0; u < PB; u++) {
-            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;
-            *reinterpret_cast<uint4 *>(&sB[buf][0][row][qq * 8]) = xb[0][u];
-            *reinterpret_cast<uint4 *>(&sB[buf][1][row][qq * 8]) = xb[1][u];
-        }
-    };
-    auto compute = [&](int cur) {
-#pragma unroll
-        for (int kk = 0; kk < BK / 16; kk++) {
-            const int kc = kk * 16 + 8 * (lane / 32);
-            v_t ah[WM], al[WM], bh[WN], bl[WN];
-#pragma unroll
-            for (int m = 0; m < WM; m++) {
-                const int r = wi * WM * 32 + m * 32 + lane % 32;
-                ah[m] = *reinterpret_cast<const v_t *>(&sA[cur][0][r][kc]);
-                al[m] = *reinterpret_cast<const v_t *>(&sA[cur][1][r][kc]);
-            }
-#pragma unroll
-            for (int n = 0; n < WN; n++) {
-                const int r = wj * WN * 32 + n * 32 + lane % 32;
-                bh[n] = *reinterpret_cast<const v_t *>(&sB[cur][0][r][kc]);
-                bl[n] = *reinterpret_cast<const v_t *>(&sB[cur][1][r][kc]);
-            }
-#pragma unroll
-            for (int m = 0; m < WM; m++)
-#pragma unroll
-                for (int n = 0; n < WN; n++) {
-                    acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bh[n], acc[m][n]);
-                    if constexpr (DUAL) {
-                        acc2[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc2[m][n]);
-                        acc2[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc2[m][n]);
-                    } else {
-                        acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc[m][n]);
-                        acc[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc[m][n]);
-                    }
-                }
-        }
-    };
+    // computed, so every global load has about two iterations to arrive.  The loop
+    // is unrolled by two (h) so that every register-array index is a constant
+    // (no lambdas capturing arrays: those push them to scratch).
+    u32x4 ra[2][2][PA], rb[2][2][PB];  // [stage][hi, lo][piece]
+#define ASD_SPLIT_GLOAD(KT, ST)                                                          \
+    do {                                                                                 \
+        const long k0_ = (long)(KT) * BK;                                                \
+        _Pragma("unroll") for (int u = 0; u < PA; u++) {                                 \
+            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;                       \
+            const long off = (long)row * ld + k0_ + qq * 8;                              \
+            ra[ST][0][u] = *reinterpret_cast<const u32x4 *>(Ah + off);                   \
+            ra[ST][1][u] = *reinterpret_cast<const u32x4 *>(Al + off);                   \
+        }                                                                                \
+        _Pragma("unroll") for (int u = 0; u < PB; u++) {                                 \
+            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;                       \
+            const long off = (long)row * ld + k0_ + qq * 8;                              \
+            rb[ST][0][u] = *reinterpret_cast<const u32x4 *>(Bh + off);                   \
+            rb[ST][1][u] = *reinterpret_cast<const u32x4 *>(Bl + off);                   \
+        }                                                                                \
+    } while (0)
+#define ASD_SPLIT_SSTORE(BUF, ST)                                                        \
+    do {                                                                                 \
+        _Pragma("unroll") for (int u = 0; u < PA; u++) {                                 \
+            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;                       \
+            *reinterpret_cast<u32x4 *>(&sA[BUF][0][row][qq * 8]) = ra[ST][0][u];         \
+            *reinterpret_cast<u32x4 *>(&sA[BUF][1][row][qq * 8]) = ra[ST][1][u];         \
+        }                                                                                \
+        _Pragma("unroll") for (int u = 0; u < PB; u++) {                                 \
+            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;                       \
+            *reinterpret_cast<u32x4 *>(&sB[BUF][0][row][qq * 8]) = rb[ST][0][u];         \
+            *reinterpret_cast<u32x4 *>(&sB[BUF][1][row][qq * 8]) = rb[ST][1][u];         \
+        }                                                                                \
+    } while (0)
 
     const int nk = args.Kpad / BK;
-    gload(0, ra[0], rb[0]);
-    if (nk > 1) gload(1, ra[1], rb[1]);
-    sstore(0, ra[0], rb[0]);
+    ASD_SPLIT_GLOAD(0, 0);
+    if (nk > 1) ASD_SPLIT_GLOAD(1, 1);
+    ASD_SPLIT_SSTORE(0, 0);
     __syncthreads();
-    // iteration kt computes LDS buffer kt&1, issues loads for kt+2 into register
-    // stage kt&1 (free: stored at the end of kt-1) and stores stage (kt+1)&1 into LDS
     for (int kt = 0; kt < nk; kt += 2) {
-        if (kt + 2 < nk) gload(kt + 2, ra[0], rb[0]);
-        compute(0);
-        if (kt + 1 < nk) sstore(1, ra[1], rb[1]);
-        __syncthreads();
-        if (kt + 1 >= nk) break;
-        if (kt + 3 < nk) gload(kt + 3, ra[1], rb[1]);
-        compute(1);
-        if (kt + 2 < nk) sstore(0, ra[0], rb[0]);
-        __syncthreads();
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int k = kt + h;
+            if (k < nk) {
+                if (k + 2 < nk) ASD_SPLIT_GLOAD(k + 2, h);
+#pragma unroll
+                for (int kk = 0; kk < BK / 16; kk++) {
+                    const int kc = kk * 16 + 8 * (lane / 32);
+                    v_t ah[WM], al[WM], bh[WN], bl[WN];
+#pragma unroll
+                    for (int m = 0; m < WM; m++) {
+                        const int r = wi * WM * 32 + m * 32 + lane % 32;
+                        ah[m] = *reinterpret_cast<const v_t *>(&sA[h][0][r][kc]);
+                        al[m] = *reinterpret_cast<const v_t *>(&sA[h][1][r][kc]);
+                    }
+#pragma unroll
+                    for (int n = 0; n < WN; n++) {
+                        const int r = wj * WN * 32 + n * 32 + lane % 32;
+                        bh[n] = *reinterpret_cast<const v_t *>(&sB[h][0][r][kc]);
+                        bl[n] = *reinterpret_cast<const v_t *>(&sB[h][1][r][kc]);
+                    }
+#pragma unroll
+                    for (int m = 0; m < WM; m++)
+#pragma unroll
+                        for (int n = 0; n < WN; n++) {
+                            acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bh[n], acc[m][n]);
+                            if constexpr (DUAL) {
+                                acc2[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc2[m][n]);
+                                acc2[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc2[m][n]);
+                            } else {
+                                acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc[m][n]);
+                                acc[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc[m][n]);
+                            }
+                        }
+                }
+                if (k + 1 < nk) ASD_SPLIT_SSTORE(h ^ 1, h ^ 1);
+                __syncthreads();
+            }
+        }
     }
+#undef ASD_SPLIT_GLOAD
+#undef ASD_SPLIT_SSTORE
 
     const int L = args.L, Ns = args.Ns;
     const BinGeom g = bin_geom(i0, j0, Ns, L, BM, BN);

@@ -269,7 +269,10 @@ extern "C" int asd_sqt_vjp(asd_ctx *c, const asd_params *pp, int L, const double
     }
     // ---- S(q,tau) in FP64 (leaves F_q(t) of all snapshots in c->fq)
     ASD_TRY(c->outdev.ensure(sizeof(double) * 2 * (long)nlags * nq));
-    ASD_TRY(sqt_run(c, L, snaps, snaps, n, nsnap, T0, nlags, h_q, nq, ASD_FP64, c->outdev.as<double>(), -1));
+    c->sqt_generic_only = 1;
+    const int rs = sqt_run(c, L, snaps, snaps, n, nsnap, T0, nlags, h_q, nq, ASD_FP64, c->outdev.as<double>(), -1);
+    c->sqt_generic_only = 0;
+    ASD_TRY(rs);
     if (h_sqt)
         ASD_HIP(hipMemcpyAsync(h_sqt, c->outdev.ptr, sizeof(double) * 2 * (long)nlags * nq, hipMemcpyDeviceToHost,
                                st));

@@ -21,6 +21,7 @@
 #include <hip/hip_bf16.h>
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
 #include <cmath>
 
 #include "asd_internal.h"
@@ -673,6 +674,80 @@ __global__ void fill_pseudo_spins_kernel(double *p, long nspins)
     }
 }
 
+// ------------------------------------------------ full momentum grid (separable)
+// out[b][k][j] = sum_{x<L} in[b][x][j] e^{-2 pi i k x / L}, j < inner; `in` real
+// (in_complex = 0) or complex (re, im interleaved); Kahan-compensated FP64.
+// grid (ceil(L*inner/256), nbatch)
+__global__ __launch_bounds__(256) void dft_strided_kernel(const double *__restrict__ in, int in_complex, long b0,
+                                                          int L, int inner, const double2 *__restrict__ tw,
+                                                          double *__restrict__ out)
+{
+    const long b = b0 + blockIdx.y;
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)L * inner) return;
+    const int k = (int)(e / inner), j = (int)(e - (long)k * inner);
+    const long base = b * (long)L * inner;
+    KahanC s;
+    for (int x = 0; x < L; x++) {
+        const double2 w = tw[(int)(((long)k * x) % L)];  // e^{-i phi} = (w.x, -w.y)
+        const long at = base + (long)x * inner + j;
+        if (in_complex) {
+            const double re = in[2 * at], im = in[2 * at + 1];
+            s.add(re * w.x + im * w.y, im * w.x - re * w.y);
+        } else {
+            const double v = in[at];
+            s.add(v * w.x, -v * w.y);
+        }
+    }
+    out[2 * (base + e)] = s.re;
+    out[2 * (base + e) + 1] = s.im;
+}
+
+// fbar[tau][i][c] = (1/T0) sum_{t0<T0} S_i^c(t0 + tau) as a Kahan-compensated sliding sum
+__global__ void window_mean_kernel(const double *__restrict__ snaps, long tstride, long n, int T0, int nlags,
+                                   double *__restrict__ fbar)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    double s = 0.0, c = 0.0;
+    auto add = [&](double x) {
+        const double y = x - c, t = s + y;
+        c = (t - s) - y;
+        s = t;
+    };
+    for (int t0 = 0; t0 < T0; t0++) add(snaps[(long)t0 * tstride + e]);
+    fbar[e] = s / (double)T0;
+    for (int tau = 1; tau < nlags; tau++) {
+        add(snaps[(long)(tau + T0 - 1) * tstride + e]);
+        add(-snaps[(long)(tau - 1) * tstride + e]);
+        fbar[(long)tau * n + e] = s / (double)T0;
+    }
+}
+
+// out[lag][iq] = ( Sc[lag][q]/T0 - sum_c Fbar_c(lag) conj(Zbar_c) ) / Ns, q = (mx, my)
+// Sc: [lag][mx][my] complex; Fb: [lag][mx][my][c] complex (DFT of the window means);
+// Fz: the B side's lag-0 window-mean DFT (= Fb for the time-origin form)
+__global__ void finish_full_kernel(const double *__restrict__ Sc, const double *__restrict__ Fb,
+                                     const double *__restrict__ Fz, int L, const int *__restrict__ q, int nq,
+                                     int nlags, int T0, double *__restrict__ out)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (long)nq * nlags) return;
+    const int lag = (int)(e / nq), iq = (int)(e - (long)lag * nq);
+    const long Ns = (long)L * L;
+    const long m = (long)q[2 * iq] * L + q[2 * iq + 1];
+    const double *sc = Sc + 2 * ((long)lag * Ns + m);
+    const double *f = Fb + 6 * ((long)lag * Ns + m), *z = Fz + 6 * m;
+    double dre = 0, dim = 0;
+    for (int c = 0; c < 3; c++) {
+        dre += f[2 * c] * z[2 * c] + f[2 * c + 1] * z[2 * c + 1];
+        dim += f[2 * c + 1] * z[2 * c] - f[2 * c] * z[2 * c + 1];
+    }
+    const double invT = 1.0 / (double)T0;
+    out[2 * e] = (sc[0] * invT - dre) / (double)Ns;
+    out[2 * e + 1] = (sc[1] * invT - dim) / (double)Ns;
+}
+
 __global__ void zero_kernel(double *p, long n)
 {
     for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x)
@@ -791,6 +866,45 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
     }
     ASD_HIP(hipGetLastError());
     ASD_TRY(mark(ctx, 2));
+
+    const bool full = nq > 2 * L && !ctx->sqt_generic_only;  // many momenta: separable DFTs over the grid
+    if (full) {
+        const long n3 = 3L * Ns;
+        const long wm = (long)nlags * n3;            // doubles of window means
+        const bool two = d_snaps_b != d_snaps_a;     // replica form: B side has its own mean
+        ASD_TRY(ctx->fq.ensure(sizeof(double) * (wm + n3 + 4 * (wm + n3) + 4L * nlags * Ns)));
+        double *fbar = ctx->fq.as<double>();         // [tau][r][col][c], then the B mean
+        double *pt = fbar + wm + n3;                 // complex [tau][r][my][c]
+        double *Fb = pt + 2 * (wm + n3);             // complex [tau][mx][my][c], then the B side
+        double *Sc = Fb + 2 * (wm + n3);             // complex [lag][mx][my]
+        double *Ht = Sc + 2L * nlags * Ns;           // complex [lag][dr][my]
+        const double2 *tw = ctx->twiddle.as<double2>();
+        auto dft = [&](const double *in, int cplx, long nbatch, int inner, double *out) {
+            for (long b0 = 0; b0 < nbatch; b0 += 65535) {
+                const unsigned nb = (unsigned)std::min<long>(65535, nbatch - b0);
+                hipLaunchKernelGGL(dft_strided_kernel, dim3((unsigned)(((long)L * inner + 255) / 256), nb), dim3(256),
+                                   0, st, in, cplx, b0, L, inner, tw, out);
+            }
+        };
+        ASD_TRY(mark(ctx, 2));
+        hipLaunchKernelGGL(window_mean_kernel, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st, d_snaps_a,
+                           tstride, n3, T0, nlags, fbar);
+        if (two)
+            hipLaunchKernelGGL(window_mean_kernel, dim3((unsigned)((n3 + 255) / 256)), dim3(256), 0, st, d_snaps_b,
+                               tstride, n3, T0, 1, fbar + wm);
+        const long nb = nlags + (two ? 1 : 0);       // window means are contiguous
+        dft(fbar, 0, nb * L, 3, pt);                 // over col
+        dft(pt, 1, nb, 3 * L, Fb);                   // over r
+        ASD_TRY(mark(ctx, 3));
+        dft(ctx->bins.as<double>(), 0, (long)nlags * L, 1, Ht);  // bins over dc
+        dft(Ht, 1, nlags, L, Sc);                                 // over dr
+        const double *Fz = two ? Fb + 2 * wm : Fb;
+        hipLaunchKernelGGL(finish_full_kernel, dim3((unsigned)(((long)nq * nlags + 255) / 256)), dim3(256), 0, st,
+                           (const double *)Sc, (const double *)Fb, Fz, L, ctx->qdev.as<int>(), nq, nlags, T0, d_out);
+        ASD_HIP(hipGetLastError());
+        ASD_TRY(mark(ctx, 4));
+        return ASD_OK;
+    }
 
     // Fourier sums of every snapshot the windows touch (A side covers [0, nt_a);
     // B side is snapshots [0, T0) of d_snaps_b, which equals d_snaps_a in the

@@ -123,3 +123,31 @@ def test_replica_form_matches_reference_kernel(gctx, orc):
     got = gctx.sqt_replicas(L, torch.tensor(now, device="cuda"), torch.tensor(zero, device="cuda"),
                             [(mx, my)], "fp64")[0]
     assert abs(got - expect) <= 1e-11 * max(1.0, abs(expect))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,T0,nlags,prec", [(16, 12, 6, "fp64"), (20, 8, 5, "fp16x2")])
+def test_full_grid_path_equals_per_q_path(gctx, orc, L, T0, nlags, prec):
+    """nq > 2L takes the separable full-grid DFT path; the same momenta asked in
+    chunks of at most 2L take the per-q path -- both must agree."""
+    torch = need_gpu()
+    snaps = torch.tensor(trajectory(orc, L, T0 + nlags - 1, D=0.1), device="cuda")
+    q = full_grid(L)
+    rng = np.random.default_rng(L)
+    q = [q[i] for i in rng.permutation(len(q))]
+    a = gctx.sqt(L, snaps, T0, nlags, q, prec)
+    b = np.concatenate([gctx.sqt(L, snaps, T0, nlags, q[i:i + 2 * L], prec) for i in range(0, len(q), 2 * L)], axis=1)
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+@pytest.mark.gpu
+def test_full_grid_replica_form(gctx, orc):
+    torch = need_gpu()
+    L, nrep = 8, 6
+    zero = np.stack([orc.init_spins(60 + r, L * L) for r in range(nrep)])
+    now, _ = orc.run(orc.params(dt=0.01, D=0.1), L, zero, 20)
+    q = full_grid(L)
+    dn, dz = torch.tensor(now, device="cuda"), torch.tensor(zero, device="cuda")
+    a = gctx.sqt_replicas(L, dn, dz, q, "fp64")
+    b = np.concatenate([gctx.sqt_replicas(L, dn, dz, q[i:i + 2 * L], "fp64") for i in range(0, len(q), 2 * L)])
+    assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()

@@ -89,6 +89,191 @@ def cpu_baseline(L, T0, nq, nlags, nsteps_total, threads):
     }
 
 
+def gxmg(L):
+    h = L // 2
+    return [(k, 0) for k in range(h)] + [(h, k) for k in range(h)] + [(h - k, h - k) for k in range(h)]
+
+
+def err_stats(a, ref):
+    import numpy as np
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    e = np.abs(a - ref) / scale
+    return {"max": float(e.max()), "rms": float(np.sqrt((e ** 2).mean()))}
+
+
+def workload_precision256(args):
+    """L=256, J=1 D=0.05 B_z=0.2 alpha=0 seed 2, 4000 FP64 RK4 steps (snapshot every 10),
+    lags 0..199 over 200 origins, full L x L momentum grid; S(q,t) in FP64, FP32, BF16X3,
+    FP16X2 with max / RMS error per (q, tau) against FP64 on identical snapshots."""
+    import numpy as np
+    import torch
+
+    import asd_amd
+    L, T0, nl, every, nsteps = 256, 200, 200, 10, 4000
+    ns = L * L
+    ctx = asd_amd.Context(0)
+    ctx.set_timing(True)
+    p = asd_amd.params(J=1.0, D=0.05, Bz=0.2, alpha=0.0, dt=0.01)
+    S = torch.tensor(asd_amd.init_spins(2, ns), device="cuda")
+    nsnap = nsteps // every
+    snaps = torch.empty((nsnap, ns, 3), dtype=torch.float64, device="cuda")
+    ctx.rk4_run(p, L, S, nsteps, every, nsnap, snaps)
+    rk = ctx.stage_times()["rk4"]
+    q = asd_amd.full_grid(L)
+    flop = nl * 2.0 * ns * ns * 3 * T0
+    res, outs = {}, {}
+    for prec in ("fp64", "fp32", "bf16x3", "fp16x2"):
+        ctx.autotune(L, T0, prec)
+        out = torch.empty((nl, len(q), 2), dtype=torch.float64, device="cuda")
+        ctx.sqt(L, snaps, T0, nl, q, prec, out=out)
+        torch.cuda.synchronize()
+        st = ctx.stage_times()
+        total = st["pack"] + st["gemm"] + st["fourier"] + st["finish"]
+        o = out.cpu().numpy()
+        outs[prec] = o[..., 0] + 1j * o[..., 1]
+        res[prec] = {"sqt_s": round(total / 1e3, 3), "gemm_s": round(st["gemm"] / 1e3, 3),
+                     "gemm_tflops": round(flop / st["gemm"] / 1e9, 2)}
+        log(f"  {prec}: {res[prec]}")
+    for prec in ("fp32", "bf16x3", "fp16x2"):
+        res[prec]["err_vs_fp64"] = err_stats(outs[prec], outs["fp64"])
+    zbar = snaps[:T0].mean(dim=0).cpu().numpy()
+    sum_rule = float(abs(outs["fp64"][0].sum() - (ns - (zbar ** 2).sum())) / ns)
+    print(json.dumps({"workload": "precision L=256 full grid", "L": L, "T0": T0, "lags": nl, "nq": len(q),
+                      "rk4_s": round(rk / 1e3, 4), "gemm_flop": flop, "per_precision": res,
+                      "error_floor": "relative to max_q |S(q,tau)| of each lag",
+                      "fp64_sum_rule_rel_residual": sum_rule}), flush=True)
+
+
+def workload_batch96(args):
+    """96 trajectories, L in {32,48,64,96,128} (32/24/20/12/8), D ~ U[0,0.2], alpha=0,
+    seeds 100..195, 2000 FP64 RK4 steps (snapshot every 10) in one persistent launch per
+    step; S(q,t) for lags 0..99 over 100 origins on each lattice's Gamma-X-M-Gamma path
+    in FP16X2 with tile shapes autotuned per L."""
+    import numpy as np
+    import torch
+
+    import asd_amd
+    Ls = [32] * 32 + [48] * 24 + [64] * 20 + [96] * 12 + [128] * 8
+    rng = np.random.default_rng(2024)
+    Ds = rng.uniform(0.0, 0.2, len(Ls))
+    nsteps, every, T0, nl = 2000, 10, 100, 100
+    nsnap = nsteps // every
+    ctx = asd_amd.Context(0)
+    ctx.set_timing(True)
+    plist = [asd_amd.params(J=1.0, D=float(d), dt=0.01) for d in Ds]
+    S0 = [torch.tensor(asd_amd.init_spins(100 + t, L * L), device="cuda") for t, L in enumerate(Ls)]
+    spins = [s.clone() for s in S0]
+    snaps = [torch.empty((nsnap, L * L, 3), dtype=torch.float64, device="cuda") for L in Ls]
+    total_sites = sum(L * L for L in Ls)
+    ctx.batch_rk4_run(Ls, plist, spins, nsteps, every, nsnap, snaps)     # warm (graph capture)
+    for t in range(len(Ls)):
+        spins[t].copy_(S0[t])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ctx.batch_rk4_run(Ls, plist, spins, nsteps, every, nsnap, snaps)
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    # unbatched: one trajectory at a time (for the batching gain)
+    t0 = time.perf_counter()
+    for t, L in enumerate(Ls):
+        s = S0[t].clone()
+        ctx.rk4_run(plist[t], L, s, nsteps, every, nsnap, snaps[t])
+    torch.cuda.synchronize()
+    t_single = time.perf_counter() - t0
+    tuned = {}
+    for L in sorted(set(Ls)):
+        best, times = ctx.autotune(L, T0, "fp16x2")
+        tuned[L] = list(times)[best]
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for t, L in enumerate(Ls):
+        ctx.sqt(L, snaps[t], T0, nl, gxmg(L), "fp16x2")
+    torch.cuda.synchronize()
+    t_sqt = time.perf_counter() - t0
+    flop = sum(nl * 2.0 * (L * L) ** 2 * 3 * T0 for L in Ls)
+    print(json.dumps({"workload": "batch 96 trajectories", "trajectories": len(Ls), "sites_total": total_sites,
+                      "rk4_steps": nsteps, "rk4_batched_s": round(t_batch, 4),
+                      "rk4_spin_steps_per_s": total_sites * nsteps / t_batch,
+                      "rk4_unbatched_s": round(t_single, 4), "batching_speedup": round(t_single / t_batch, 2),
+                      "sqt_fp16x2_s": round(t_sqt, 3), "sqt_tflops": round(flop / t_sqt / 1e12, 2),
+                      "autotuned_cfg_per_L": tuned}), flush=True)
+
+
+def workload_adjoint64(args):
+    """L=64, theta=(J,D,B_z,alpha)=(1.0,0.1,0.1,0.01), target from theta*=(1.1,0.12,0.08,0.01),
+    seed 3, 4000 RK4 steps, snapshot every 10, lags 0..199 over 200 origins, Gamma-X-M-Gamma;
+    loss sum |S - S_target|^2; hand-written adjoint (checkpoint every 50 steps) against
+    central FD (h=1e-6) and the FP64 CPU tangent-linear model."""
+    import numpy as np
+    import torch
+
+    import asd_amd
+    L, nsteps, every, T0, nl = 64, 4000, 10, 200, 200
+    q = gxmg(L)
+    ctx = asd_amd.Context(0)
+    S0 = torch.tensor(asd_amd.init_spins(3, L * L), device="cuda")
+    ts, th = (1.1, 0.12, 0.08, 0.01), [1.0, 0.1, 0.1, 0.01]
+    target, _ = ctx.sqt_vjp(asd_amd.params(*ts, 0.01), L, S0, nsteps, every, T0, nl, q)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    loss, grad = ctx.sqt_loss_grad(asd_amd.params(*th, 0.01), L, S0, nsteps, every, T0, nl, q, target, 50)
+    t_adj = time.perf_counter() - t0
+    h = 1e-6
+    fd = []
+    t0 = time.perf_counter()
+    for i in range(4):
+        tp, tm = list(th), list(th)
+        tp[i] += h
+        tm[i] -= h
+        sp, _ = ctx.sqt_vjp(asd_amd.params(*tp, 0.01), L, S0, nsteps, every, T0, nl, q)
+        sm, _ = ctx.sqt_vjp(asd_amd.params(*tm, 0.01), L, S0, nsteps, every, T0, nl, q)
+        fd.append((np.sum(np.abs(sp - target) ** 2) - np.sum(np.abs(sm - target) ** 2)) / (2 * h))
+    t_fd = time.perf_counter() - t0
+    fd = np.array(fd)
+    out = {"workload": "adjoint L=64", "loss": loss, "grad_adjoint": grad.tolist(), "grad_central_fd": fd.tolist(),
+           "max_rel_diff_fd": float(np.abs(grad - fd).max() / np.abs(fd).max()),
+           "adjoint_s": round(t_adj, 3), "fd_8_forward_s": round(t_fd, 3)}
+    if not args.no_cpu:
+        import oracle
+        t0 = time.perf_counter()
+        s_now, _ = ctx.sqt_vjp(asd_amd.params(*th, 0.01), L, S0, nsteps, every, T0, nl, q)
+        snaps, dsnaps = oracle.tlm_run(oracle.params(*th, dt=0.01), L, S0.cpu().numpy(), nsteps, every, T0 + nl - 1)
+        R = s_now - target
+        tlm = np.array([2 * np.sum((R.conj() * oracle.tlm_sqt(L, snaps, dsnaps[i], T0, nl, q)).real)
+                        for i in range(4)])
+        out.update(grad_tangent_linear_cpu=tlm.tolist(),
+                   max_rel_diff_tlm=float(np.abs(grad - tlm).max() / np.abs(tlm).max()),
+                   tlm_cpu_s=round(time.perf_counter() - t0, 2))
+    print(json.dumps(out), flush=True)
+
+
+def workload_rk4sweep(args):
+    """RK4 throughput against the HBM roofline (48 B per spin-step algorithmic) from the
+    latency-bound single L=128 lattice to replica batches whose state exceeds the caches."""
+    import torch
+
+    import asd_amd
+    ctx = asd_amd.Context(0)
+    ctx.set_timing(True)
+    p = asd_amd.params(J=1.0, D=0.1, dt=0.01)
+    rows = []
+    for L, nrep, steps in ((128, 1, 2000), (256, 1, 1000), (1024, 1, 200), (1024, 8, 100), (1024, 32, 50),
+                           (2048 // 2, 64, 30)):
+        n = L * L * nrep
+        S = torch.tensor(asd_amd.init_spins(5, L * L), device="cuda").repeat(nrep, 1)
+        ctx.rk4_run(p, L, S, 2)
+        torch.cuda.synchronize()
+        ctx.rk4_run(p, L, S, steps)
+        ms = ctx.stage_times()["rk4"]
+        rate = n * steps / (ms * 1e-3)
+        rows.append({"L": L, "replicas": nrep, "sites": n, "state_MB": round(n * 24 / 1e6, 1),
+                     "spin_steps_per_s": rate, "us_per_step": round(ms * 1e3 / steps, 2),
+                     "algorithmic_GBps": round(48 * rate / 1e9, 1),
+                     "hbm_frac": round(48 * rate / (HBM_PEAK_GBS * 1e9), 4)})
+        log(f"  {rows[-1]}")
+    print(json.dumps({"workload": "rk4 sweep", "rows": rows}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -103,6 +288,9 @@ def main():
     ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison pass")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--workload", default="plain128",
+                    choices=["plain128", "precision256", "batch96", "adjoint64", "rk4sweep"],
+                    help="plain128 is the benchmark line; the others report their own JSON")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 1 GiB-in / 1 GiB-out elementwise copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -111,6 +299,11 @@ def main():
         print("bench.py measures one GPU only: this build has no multi-GPU path "
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload != "plain128":
+        {"precision256": workload_precision256, "batch96": workload_batch96, "adjoint64": workload_adjoint64,
+         "rk4sweep": workload_rk4sweep}[args.workload](args)
+        return
 
     import numpy as np
     import torch

@@ -98,15 +98,16 @@ struct Work {
     int lag, ti, tj;
 };
 
-// Blocks are dispatched round-robin over the 8 XCDs (block b -> XCD b % 8).  Give
-// each XCD a contiguous slice of the work list, and order tiles in groups of
-// `group` tile-rows so that concurrently running tiles share A and B panels in the
-// XCD's L2.
-__device__ __forceinline__ bool decode_work(int nti, int ntj, int nlags, int group, Work &w)
+// Work slots are processed by a capped grid in a grid-stride loop (a grid of one
+// block per tile would overflow the 32-bit dispatch size at L = 256).  The grid is a
+// multiple of 8 and blocks are dispatched round-robin over the 8 XCDs (block b ->
+// XCD b % 8), so slot s always runs on XCD s % 8: give each XCD a contiguous slice of
+// the work list, and order tiles in groups of `group` tile-rows so that concurrently
+// running tiles share A and B panels in the XCD's L2.
+__device__ __forceinline__ bool decode_work(long slot, long nslots, int nti, int ntj, int nlags, int group, Work &w)
 {
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int per = nb / 8;
-    const long wid = (long)(b % 8) * per + b / 8;
+    const long per = nslots / 8;
+    const long wid = (slot % 8) * per + slot / 8;
     const long per_lag = (long)nti * ntj;
     if (wid >= per_lag * nlags) return false;
     w.lag = (int)(wid / per_lag);
@@ -119,6 +120,9 @@ __device__ __forceinline__ bool decode_work(int nti, int ntj, int nlags, int gro
     w.tj = rr / gsz;
     return true;
 }
+
+constexpr long kMaxGrid = 1L << 16;  // 65536 blocks x 256 threads
+__host__ __device__ inline long num_slots(long total) { return (total + 7) / 8 * 8; }
 
 // --------------------------------------------------------- binning epilogue
 
@@ -197,8 +201,11 @@ __global__ __launch_bounds__(256) void sqt_gemm_kmajor_kernel(GemmArgs args)
     T(*sB)[BK][LDS_LD] = reinterpret_cast<T(*)[BK][LDS_LD]>(smem_raw + 2 * BK * LDS_LD * sizeof(T));
 
     const int nt_tiles = (args.Ns + TILE - 1) / TILE;
+    const long nslots = num_slots((long)nt_tiles * nt_tiles * args.nlags);
+    for (long slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
     Work w;
-    if (!decode_work(nt_tiles, nt_tiles, args.nlags, args.group, w)) return;
+    if (!decode_work(slot, nslots, nt_tiles, nt_tiles, args.nlags, args.group, w)) continue;
+    __syncthreads();  // LDS of the previous work item fully consumed
     const int i0 = w.ti * TILE, j0 = w.tj * TILE;
     const long ld = args.ld;
     const T *Ap = static_cast<const T *>(args.A) + (long)(3 * w.lag) * ld + i0;
@@ -289,6 +296,7 @@ __global__ __launch_bounds__(256) void sqt_gemm_kmajor_kernel(GemmArgs args)
         }
     __syncthreads();
     flush_bins(sb, g, L, args.bins + (long)w.lag * Ns);
+    }
 }
 
 // ------------------------------------------------------------- BF16X3 / FP16X2
@@ -334,8 +342,11 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
     __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][BN][LDS_LD];
 
     const int nti = (args.Ns + BM - 1) / BM, ntj = (args.Ns + BN - 1) / BN;
+    const long nslots = num_slots((long)nti * ntj * args.nlags);
+    for (long slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
     Work w;
-    if (!decode_work(nti, ntj, args.nlags, args.group, w)) return;
+    if (!decode_work(slot, nslots, nti, ntj, args.nlags, args.group, w)) continue;
+    __syncthreads();  // LDS of the previous work item fully consumed
     const int i0 = w.ti * BM, j0 = w.tj * BN;
     const long ld = args.ld;
     const long koff = 3L * w.lag, shift = koff & 7;
@@ -484,6 +495,7 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
         }
     __syncthreads();
     flush_bins(sb, g, L, args.bins + (long)w.lag * Ns);
+    }
 }
 
 // ------------------------------------------------------------- configurations
@@ -525,7 +537,7 @@ static void launch_split(const SqtCfg &c, GemmArgs ga, hipStream_t st)
 {
     const int BM = 64 * c.wm, BN = 64 * c.wn;
     const long total = (long)((ga.Ns + BM - 1) / BM) * ((ga.Ns + BN - 1) / BN) * ga.nlags;
-    const dim3 grid((unsigned)((total + 7) / 8 * 8)), blk(256);
+    const dim3 grid((unsigned)std::min(num_slots(total), kMaxGrid)), blk(256);
     ga.group = c.group;
     if constexpr (PREC == ASD_FP16X2) {
         hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 2>), grid, blk, 0, st, ga);
@@ -760,8 +772,7 @@ static long round_up(long a, long b) { return (a + b - 1) / b * b; }
 
 static int grid_for(int nt_tiles, int nlags)
 {
-    const long total = (long)nt_tiles * nt_tiles * nlags;
-    return (int)round_up(total, 8);
+    return (int)std::min(num_slots((long)nt_tiles * nt_tiles * nlags), kMaxGrid);
 }
 
 int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_b, long tstride,

@@ -151,3 +151,20 @@ def test_full_grid_replica_form(gctx, orc):
     a = gctx.sqt_replicas(L, dn, dz, q, "fp64")
     b = np.concatenate([gctx.sqt_replicas(L, dn, dz, q[i:i + 2 * L], "fp64") for i in range(0, len(q), 2 * L)])
     assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp16x2", 1e-5)])
+def test_large_work_list_all_lags_computed(gctx, orc, prec, tol):
+    """L = 256 with 70 lags is 18.4M tile work items (4.7e9 work-items if launched one
+    workgroup per tile, beyond the 32-bit dispatch size); every lag must match the
+    separable oracle."""
+    torch = need_gpu()
+    L, T0, nlags = 256, 4, 70
+    snaps = np.stack([orc.init_spins(1000 + t, L * L) for t in range(T0 + nlags - 1)])
+    q = [(0, 0), (1, 0), (3, 5), (128, 0), (128, 128), (17, 200), (255, 1), (64, 32)]
+    ref = orc.sqt_separable(L, snaps, T0, nlags, q)
+    got = gctx.sqt(L, torch.tensor(snaps, device="cuda"), T0, nlags, q, prec)
+    err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    say(f"  {prec} L=256 per-lag max rel err: first {err[0]:.2e} last {err[-1]:.2e} max {err.max():.2e}")
+    assert err.max() <= tol

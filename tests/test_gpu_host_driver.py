@@ -32,7 +32,7 @@ def test_pipeline_mode_matches_api(gctx, tmp_path):
     import asd_amd
     L, steps, every, T0, G = 16, 300, 10, 20, 8
     r = subprocess.run([BIN, "-P", "-n", str(L), "-S", str(steps), "-E", str(every), "-T", str(T0), "-G", str(G),
-                        "-D", "0.1", "-s", "4", "-p", "fp64", "-O", "out.bin"], cwd=tmp_path, capture_output=True,
+                        "-e", "0.01", "-D", "0.1", "-s", "4", "-p", "fp64", "-O", "out.bin"], cwd=tmp_path, capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     hdr = json.load(open(tmp_path / "out.bin.json"))

@@ -47,6 +47,7 @@ struct LLGParams {
     double J, D, Bz, alpha, dt, len;
     double a_over_len;  // alpha / spin_length
     double den;         // 1 + alpha^2
+    double inv_den;     // 1 / (1 + alpha^2), formed once
 };
 LLGParams make_llg(const asd_params &p);
 

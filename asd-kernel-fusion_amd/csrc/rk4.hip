@@ -28,6 +28,7 @@ LLGParams make_llg(const asd_params &p)
     q.len = p.spin_length;
     q.a_over_len = p.alpha / p.spin_length;
     q.den = 1.0 + p.alpha * p.alpha;
+    q.inv_den = 1.0 / q.den;
     return q;
 }
 
@@ -50,9 +51,9 @@ __device__ __forceinline__ void llg_f(const LLGParams &p, double sx, double sy, 
     const double qx = sy * pz - sz * py;
     const double qy = sz * px - sx * pz;
     const double qz = sx * py - sy * px;
-    fx = (px + p.a_over_len * qx) / p.den;
-    fy = (py + p.a_over_len * qy) / p.den;
-    fz = (pz + p.a_over_len * qz) / p.den;
+    fx = (px + p.a_over_len * qx) * p.inv_den;
+    fy = (py + p.a_over_len * qy) * p.inv_den;
+    fz = (pz + p.a_over_len * qz) * p.inv_den;
 }
 
 __device__ __forceinline__ int wrap(int x, int L)
@@ -310,12 +311,22 @@ int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in, double *out,
                     double *snap_out, const int *snap_counter, long snap_stride, int bump_counter)
 {
-    // 16 x 16 tiles (256 threads, 41.5 KB LDS) for large lattices; 8 x 32 keeps
-    // the same thread count with more tiles per row when L is a multiple of 32.
-    constexpr int TR = 16, TC = 16;
-    const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);
-    hipLaunchKernelGGL((rk4_step_kernel<TR, TC>), grid, dim3(TR * TC), 0, ctx->stream, p, L, in, out,
-                       snap_out, snap_counter, snap_stride);
+    // 16 x 16 tiles (256 threads, 41.5 KB LDS: 2.25 sites loaded and ~7 f-evaluations
+    // per output site) while the lattice is small enough that tile count, not halo
+    // work, limits; 32 x 32 tiles (1024 threads, 115 KB LDS: 1.56 loads, ~4.8 f-evaluations)
+    // once there are enough tiles to fill the chip.
+    const long tiles32 = (long)((L + 31) / 32) * ((L + 31) / 32) * nrep;
+    if (tiles32 >= 2L * ctx->num_cu) {
+        constexpr int TR = 32, TC = 32;
+        const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);
+        hipLaunchKernelGGL((rk4_step_kernel<TR, TC>), grid, dim3(TR * TC), 0, ctx->stream, p, L, in, out,
+                           snap_out, snap_counter, snap_stride);
+    } else {
+        constexpr int TR = 16, TC = 16;
+        const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);
+        hipLaunchKernelGGL((rk4_step_kernel<TR, TC>), grid, dim3(TR * TC), 0, ctx->stream, p, L, in, out,
+                           snap_out, snap_counter, snap_stride);
+    }
     if (bump_counter)
         hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(1), 0, ctx->stream, (int *)snap_counter);
     ASD_HIP(hipGetLastError());

@@ -60,7 +60,8 @@ static const int ORC_NB[4][2] = {{1, 0}, {-1, 0}, {0, -1}, {0, 1}};
  * accumulated from 0.0 neighbour by neighbour as J*S_nb (:53, :74-76), then
  * p = h x S with v3d_cross's operand order (:82, v3d.h:48-53).  Extension:
  * h_z += 2 D S_z + B_z and Gilbert damping in Landau-Lifshitz form,
- * f = (p + (alpha/s) S x p) / (1 + alpha^2), which is p exactly at alpha = 0. */
+ * f = (p + (alpha/s) S x p) * (1 / (1 + alpha^2)), which is p exactly at alpha = 0
+ * (the reciprocal is formed once; the GPU kernels use the same rounding). */
 static void orc_site_rhs(const orc_params *p, int L, const double *S, long base, int i,
                          double *out)
 {
@@ -81,13 +82,13 @@ static void orc_site_rhs(const orc_params *p, int L, const double *S, long base,
     double py = hz * s[0] - hx * s[2];
     double pz = hx * s[1] - hy * s[0];
     double a = p->alpha / p->spin_length;
-    double den = 1.0 + p->alpha * p->alpha;
+    double inv_den = 1.0 / (1.0 + p->alpha * p->alpha);
     double qx = s[1] * pz - s[2] * py;
     double qy = s[2] * px - s[0] * pz;
     double qz = s[0] * py - s[1] * px;
-    out[0] = (px + a * qx) / den;
-    out[1] = (py + a * qy) / den;
-    out[2] = (pz + a * qz) / den;
+    out[0] = (px + a * qx) * inv_den;
+    out[1] = (py + a * qy) * inv_den;
+    out[2] = (pz + a * qz) * inv_den;
 }
 
 void orc_llg_rhs(const orc_params *p, int L, int nrep, const double *S, double *dS)

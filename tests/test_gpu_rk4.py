@@ -17,6 +17,10 @@ CASES = [
     (64, 2, dict(D=0.1, Bz=0.1, alpha=0.01)),
     (128, 1, dict(D=0.1)),                                 # config-2 lattice
     (48, 1, dict(J=-0.7, D=0.05, Bz=0.2, alpha=0.2, spin_length=1.5)),
+    # enough 32 x 32 tiles to take the large-tile kernel: one big lattice, and a ragged
+    # L = 40 (32 + 8) in 200 replicas
+    (768, 1, dict(D=0.1, alpha=0.05)),
+    (40, 200, dict(D=0.1, Bz=0.1)),
 ]
 
 
@@ -31,7 +35,7 @@ def test_rk4_trajectory_matches_oracle(gctx, orc, L, nrep, kw):
     S0 = S0 * length
     p_o = orc.params(dt=0.01, **kw)
     p_g = asd_amd.params(dt=0.01, **kw)
-    nsteps, every, nsnap = 100, 10, 11
+    nsteps, every, nsnap = (100, 10, 11) if L * L * nrep < 200_000 else (20, 10, 3)
     S_ref, snaps_ref = orc.run(p_o, L, S0, nsteps, every, nsnap)
     spins = torch.tensor(S0, device="cuda")
     snaps = torch.zeros((nsnap,) + S0.shape, dtype=torch.float64, device="cuda")

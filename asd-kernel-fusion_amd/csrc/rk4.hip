@@ -83,8 +83,18 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
     const int r0 = tile_r * TR - R, c0 = tile_c * TC - R;
     const int t = threadIdx.x;
 
+    // tile coordinates lie in [-R, L + RR); when L >= RR one correction wraps them,
+    // which avoids a runtime integer division per loaded site
+    const bool near = L >= RR;
     for (int idx = t; idx < NR; idx += NT) {
-        const int gr = wrap(r0 + idx / RC, L), gc = wrap(c0 + idx % RC, L);
+        int gr = r0 + idx / RC, gc = c0 + idx % RC;
+        if (near) {
+            gr += gr < 0 ? L : (gr >= L ? -L : 0);
+            gc += gc < 0 ? L : (gc >= L ? -L : 0);
+        } else {
+            gr = wrap(gr, L);
+            gc = wrap(gc, L);
+        }
         const double *s = in + 3 * ((long)gr * L + gc);
         const double x = s[0], y = s[1], z = s[2];
         sS[0][idx] = x; sS[1][idx] = y; sS[2][idx] = z;
@@ -103,7 +113,9 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
         o[0] = sS[0][own]; o[1] = sS[1][own]; o[2] = sS[2][own];
     }
 
-#pragma unroll 1
+    // unrolled: the region width w of each stage is a compile-time constant, so the
+    // e / w, e % w index arithmetic needs no runtime division
+#pragma unroll
     for (int stage = 1; stage <= 4; stage++) {
         // increments on the region within radius R - stage of the interior
         const int lo = stage, hr = RR - stage, hc = RC - stage;

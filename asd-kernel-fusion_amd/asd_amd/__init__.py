@@ -77,6 +77,7 @@ _SIGS = {
     "asd_memcpy": (C.c_int, [_vp, _vp, _vp, C.c_size_t, C.c_int]),
     "asd_sqt_vjp": (C.c_int, [_vp, C.POINTER(Params), C.c_int, _vp, C.c_long, C.c_int, C.c_int, C.c_int, _vp,
                               C.c_int, _vp, C.c_int, _dp, _vp]),
+    "asd_rk4_set_tile": (C.c_int, [_vp, C.c_int]),
     "asd_set_timing": (C.c_int, [_vp, C.c_int]),
     "asd_stage_times": (C.c_int, [_vp, _dp]),
     "asd_euler": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int, _vp, _vp]),
@@ -160,6 +161,9 @@ class Context:
 
     def synchronize(self):
         _check(lib().asd_synchronize(self.h))
+
+    def rk4_set_tile(self, tile: int = 0):
+        _check(lib().asd_rk4_set_tile(self.h, tile))
 
     def set_timing(self, on: bool = True):
         _check(lib().asd_set_timing(self.h, 1 if on else 0))

@@ -92,6 +92,8 @@ struct asd_ctx {
     // autotuner choice per precision (-1 = automatic); tuned per (L, K)
     int forced_cfg[4] = {-1, -1, -1, -1};
     std::map<std::pair<int, int>, int> tuned[4];
+    // RK4 tile edge: 0 = automatic, 16 or 32
+    int rk4_tile = 0;
     // 1: S(q,tau) always takes the per-q path (the adjoint needs its F_q(t) sums)
     int sqt_generic_only = 0;
     // stage timing (asd_set_timing): events around pack, gemm, fourier, finish, rk4

@@ -63,12 +63,15 @@ __device__ __forceinline__ int wrap(int x, int L)
 }
 
 // One full RK4 step (+ renormalisation) of one TR x TC output tile, one interior
-// site per thread.  LDS holds the tile + halo three times (S, the current stage
-// input Y, the current increment K), structure-of-arrays.  `in`/`out`/`snap` point
-// at this lattice's [site][xyz] arrays; snap (may be null) receives the input state.
+// site per thread.  LDS holds the tile + halo twice, structure-of-arrays: S and the
+// latest increment K.  The stage input Y_s = S + c k_{s-1} (c = 1/2, 1/2, 1) is formed
+// when read -- the same expression the reference stores, so the same rounding -- which
+// keeps LDS at two arrays (77 KB for 32 x 32: two workgroups per CU overlap each
+// other's load latency and barriers).  Each thread keeps its sites' new increments in
+// registers until every read of the previous K is done.
 template <int TR, int TC> struct TileLDS {
     static constexpr int R = 4, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
-    double S[3][NR], Y[3][NR], K[3][NR];
+    double S[3][NR], K[3][NR];
 };
 
 template <int TR, int TC>
@@ -77,8 +80,8 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
                                          double *__restrict__ snap)
 {
     constexpr int R = 4, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC, NT = TR * TC;
+    constexpr int MAXE = ((RR - 2) * (RC - 2) + NT - 1) / NT;  // region sites per thread, stage 1
     double(&sS)[3][NR] = sm.S;
-    double(&sY)[3][NR] = sm.Y;
     double(&sK)[3][NR] = sm.K;
     const int r0 = tile_r * TR - R, c0 = tile_c * TC - R;
     const int t = threadIdx.x;
@@ -96,9 +99,7 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
             gc = wrap(gc, L);
         }
         const double *s = in + 3 * ((long)gr * L + gc);
-        const double x = s[0], y = s[1], z = s[2];
-        sS[0][idx] = x; sS[1][idx] = y; sS[2][idx] = z;
-        sY[0][idx] = x; sY[1][idx] = y; sY[2][idx] = z;
+        sS[0][idx] = s[0]; sS[1][idx] = s[1]; sS[2][idx] = s[2];
     }
     // this thread's interior site
     const int ir = t / TC, ic = t % TC;
@@ -120,17 +121,37 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
         // increments on the region within radius R - stage of the interior
         const int lo = stage, hr = RR - stage, hc = RC - stage;
         const int w = hc - lo, nsite = (hr - lo) * w;
-        for (int e = t; e < nsite; e += NT) {
-            const int idx = (lo + e / w) * RC + lo + e % w;
-            double n[4][3];
-            const int nb[4] = {idx + RC, idx - RC, idx - 1, idx + 1};
+        // Y_stage at LDS index i (llg_sqt_fusion.c:111, :121, :131)
+        auto Y = [&](int d, int i) -> double {
+            if (stage == 1) return sS[d][i];
+            if (stage == 4) return sS[d][i] + sK[d][i];
+            return sS[d][i] + sK[d][i] / 2.0;
+        };
+        double kr[MAXE][3];
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                n[j][0] = sY[0][nb[j]]; n[j][1] = sY[1][nb[j]]; n[j][2] = sY[2][nb[j]];
+        for (int j = 0; j < MAXE; j++) {
+            const int e = t + j * NT;
+            if (e < nsite) {
+                const int idx = (lo + e / w) * RC + lo + e % w;
+                double n[4][3];
+                const int nb[4] = {idx + RC, idx - RC, idx - 1, idx + 1};
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    n[q][0] = Y(0, nb[q]); n[q][1] = Y(1, nb[q]); n[q][2] = Y(2, nb[q]);
+                }
+                double fx, fy, fz;
+                llg_f(p, Y(0, idx), Y(1, idx), Y(2, idx), n[0], n[1], n[2], n[3], fx, fy, fz);
+                kr[j][0] = fx * p.dt; kr[j][1] = fy * p.dt; kr[j][2] = fz * p.dt;
             }
-            double fx, fy, fz;
-            llg_f(p, sY[0][idx], sY[1][idx], sY[2][idx], n[0], n[1], n[2], n[3], fx, fy, fz);
-            sK[0][idx] = fx * p.dt; sK[1][idx] = fy * p.dt; sK[2][idx] = fz * p.dt;
+        }
+        __syncthreads();  // every read of the previous increment is done
+#pragma unroll
+        for (int j = 0; j < MAXE; j++) {
+            const int e = t + j * NT;
+            if (e < nsite) {
+                const int idx = (lo + e / w) * RC + lo + e % w;
+                sK[0][idx] = kr[j][0]; sK[1][idx] = kr[j][1]; sK[2][idx] = kr[j][2];
+            }
         }
         __syncthreads();
         const double kx = sK[0][own], ky = sK[1][own], kz = sK[2][own];
@@ -140,21 +161,6 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
             ax = ax + kx / 6.0; ay = ay + ky / 6.0; az = az + kz / 6.0;
         } else {
             ax = ax + kx / 3.0; ay = ay + ky / 3.0; az = az + kz / 3.0;
-        }
-        if (stage < 4) {
-            for (int e = t; e < nsite; e += NT) {
-                const int idx = (lo + e / w) * RC + lo + e % w;
-                if (stage < 3) {
-                    sY[0][idx] = sS[0][idx] + sK[0][idx] / 2.0;
-                    sY[1][idx] = sS[1][idx] + sK[1][idx] / 2.0;
-                    sY[2][idx] = sS[2][idx] + sK[2][idx] / 2.0;
-                } else {
-                    sY[0][idx] = sS[0][idx] + sK[0][idx];
-                    sY[1][idx] = sS[1][idx] + sK[1][idx];
-                    sY[2][idx] = sS[2][idx] + sK[2][idx];
-                }
-            }
-            __syncthreads();
         }
     }
     if (owner) {
@@ -323,12 +329,13 @@ int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in, double *out,
                     double *snap_out, const int *snap_counter, long snap_stride, int bump_counter)
 {
-    // 16 x 16 tiles (256 threads, 41.5 KB LDS: 2.25 sites loaded and ~7 f-evaluations
+    // 16 x 16 tiles (256 threads, 27.6 KB LDS: 2.25 sites loaded and ~7 f-evaluations
     // per output site) while the lattice is small enough that tile count, not halo
-    // work, limits; 32 x 32 tiles (1024 threads, 115 KB LDS: 1.56 loads, ~4.8 f-evaluations)
-    // once there are enough tiles to fill the chip.
+    // work, limits; 32 x 32 tiles (1024 threads, 76.8 KB LDS: 1.56 loads, ~4.8
+    // f-evaluations) once there are enough tiles to fill the chip.
     const long tiles32 = (long)((L + 31) / 32) * ((L + 31) / 32) * nrep;
-    if (tiles32 >= 2L * ctx->num_cu) {
+    const bool big = ctx->rk4_tile ? ctx->rk4_tile == 32 : tiles32 >= 2L * ctx->num_cu;
+    if (big) {
         constexpr int TR = 32, TC = 32;
         const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);
         hipLaunchKernelGGL((rk4_step_kernel<TR, TC>), grid, dim3(TR * TC), 0, ctx->stream, p, L, in, out,

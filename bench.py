@@ -280,18 +280,22 @@ def workload_rk4sweep(args):
     p = asd_amd.params(J=1.0, D=0.1, dt=0.01)
     rows = []
     for L, nrep, steps in ((128, 1, 2000), (256, 1, 1000), (1024, 1, 200), (1024, 8, 100), (1024, 32, 50),
-                           (2048 // 2, 64, 30)):
+                           (1024, 64, 30)):
         n = L * L * nrep
         S = torch.tensor(asd_amd.init_spins(5, L * L), device="cuda").repeat(nrep, 1)
-        ctx.rk4_run(p, L, S, 2)
-        torch.cuda.synchronize()
-        ctx.rk4_run(p, L, S, steps)
-        ms = ctx.stage_times()["rk4"]
-        rate = n * steps / (ms * 1e-3)
-        rows.append({"L": L, "replicas": nrep, "sites": n, "state_MB": round(n * 24 / 1e6, 1),
-                     "spin_steps_per_s": rate, "us_per_step": round(ms * 1e3 / steps, 2),
-                     "algorithmic_GBps": round(48 * rate / 1e9, 1),
-                     "hbm_frac": round(48 * rate / (HBM_PEAK_GBS * 1e9), 4)})
+        row = {"L": L, "replicas": nrep, "sites": n, "state_MB": round(n * 24 / 1e6, 1)}
+        for tile in (16, 32):
+            ctx.rk4_set_tile(tile)
+            ctx.rk4_run(p, L, S, 2)
+            torch.cuda.synchronize()
+            ctx.rk4_run(p, L, S, steps)
+            ms = ctx.stage_times()["rk4"]
+            rate = n * steps / (ms * 1e-3)
+            row[f"tile{tile}"] = {"spin_steps_per_s": rate, "us_per_step": round(ms * 1e3 / steps, 2),
+                                  "algorithmic_GBps": round(48 * rate / 1e9, 1),
+                                  "hbm_frac": round(48 * rate / (HBM_PEAK_GBS * 1e9), 4)}
+        ctx.rk4_set_tile(0)
+        rows.append(row)
         log(f"  {rows[-1]}")
     print(json.dumps({"workload": "rk4 sweep", "rows": rows}), flush=True)
 

@@ -101,6 +101,11 @@ int asd_memcpy(asd_ctx *ctx, void *dst, const void *src, size_t bytes, int kind)
 int asd_rk4_run(asd_ctx *ctx, const asd_params *p, int L, int nrep, double *d_spins,
                 long nsteps, int snap_every, int nsnap, double *d_snaps, int use_graph);
 
+/* RK4 tile edge for asd_rk4_run: 0 = automatic (32 x 32 once there are at least
+ * two such tiles per CU, else 16 x 16), or force 16 or 32.  Results do not depend
+ * on it (bit for bit). */
+int asd_rk4_set_tile(asd_ctx *ctx, int tile);
+
 /* ---------------------------------------- stage 2: S(q,tau) correlation --- */
 /* Native replacement for the reference's S(q,t) stage (llg_sqt_fusion.c:370-442
  * + fused_dgemm), with time origins in the role of the reference's replicas:

@@ -329,12 +329,12 @@ int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in, double *out,
                     double *snap_out, const int *snap_counter, long snap_stride, int bump_counter)
 {
-    // 16 x 16 tiles (256 threads, 27.6 KB LDS: 2.25 sites loaded and ~7 f-evaluations
-    // per output site) while the lattice is small enough that tile count, not halo
-    // work, limits; 32 x 32 tiles (1024 threads, 76.8 KB LDS: 1.56 loads, ~4.8
-    // f-evaluations) once there are enough tiles to fill the chip.
-    const long tiles32 = (long)((L + 31) / 32) * ((L + 31) / 32) * nrep;
-    const bool big = ctx->rk4_tile ? ctx->rk4_tile == 32 : tiles32 >= 2L * ctx->num_cu;
+    // 16 x 16 tiles (256 threads, 27.6 KB LDS, 5 workgroups per CU; 2.25 sites loaded
+    // and 5.7 f-evaluations per output site) are the default: measured faster at every
+    // size than 32 x 32 (1024 threads, 76.8 KB, 1.56 loads and 4.8 f-evaluations per
+    // site), which the 128-register cap of 1024-thread workgroups limits to one
+    // workgroup per CU.  32 x 32 stays selectable (asd_rk4_set_tile).
+    const bool big = ctx->rk4_tile == 32;
     if (big) {
         constexpr int TR = 32, TC = 32;
         const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);

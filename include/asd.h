@@ -101,9 +101,8 @@ int asd_memcpy(asd_ctx *ctx, void *dst, const void *src, size_t bytes, int kind)
 int asd_rk4_run(asd_ctx *ctx, const asd_params *p, int L, int nrep, double *d_spins,
                 long nsteps, int snap_every, int nsnap, double *d_snaps, int use_graph);
 
-/* RK4 tile edge for asd_rk4_run: 0 = automatic (32 x 32 once there are at least
- * two such tiles per CU, else 16 x 16), or force 16 or 32.  Results do not depend
- * on it (bit for bit). */
+/* RK4 tile edge for asd_rk4_run: 0 = automatic (16 x 16, measured fastest at all
+ * sizes), or force 16 or 32.  Results do not depend on it (bit for bit). */
 int asd_rk4_set_tile(asd_ctx *ctx, int tile);
 
 /* ---------------------------------------- stage 2: S(q,tau) correlation --- */

@@ -90,3 +90,21 @@ def test_full_size_invariants(gctx, orc):
     say(f"  Mz drift {abs(S[:, 2].sum() - S0[:, 2].sum()):.3e}  energy rel drift {abs(e1 - e0) / abs(e0):.3e}")
     assert abs(S[:, 2].sum() - S0[:, 2].sum()) < 1e-6 * L * L
     assert abs(e1 - e0) < 1e-5 * abs(e0)
+
+
+@pytest.mark.gpu
+def test_tile_edge_does_not_change_results(gctx, orc):
+    torch = need_gpu()
+    import asd_amd
+    L = 72
+    S0 = torch.tensor(orc.init_spins(8, L * L), device="cuda")
+    p = asd_amd.params(D=0.1, Bz=0.05, alpha=0.02, dt=0.01)
+    out = []
+    for tile in (16, 32, 0):
+        gctx.rk4_set_tile(tile)
+        s = S0.clone()
+        gctx.rk4_run(p, L, s, 30, 10, 0, None, use_graph=False)
+        gctx.synchronize()
+        out.append(s.cpu().numpy())
+    gctx.rk4_set_tile(0)
+    assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])

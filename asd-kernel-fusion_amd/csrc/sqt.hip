@@ -331,11 +331,12 @@ template <> struct SplitMfma<ASD_FP16X2> {
     static constexpr bool dual = true;
 };
 
-template <int PREC, int WM, int WN>
-__global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
+template <int PREC, int WM, int WN, int WVM = 2, int WVN = 2>
+__global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs args)
 {
     constexpr int BK = 32, LDS_LD = BK + 8;  // halves per LDS row (pad 16 B)
-    constexpr int BM = 2 * WM * 32, BN = 2 * WN * 32;
+    constexpr int NT = 64 * WVM * WVN;       // WVM x WVN waves of WM x WN 32x32 blocks
+    constexpr int BM = WVM * WM * 32, BN = WVN * WN * 32;
     typedef typename SplitMfma<PREC>::v_t v_t;
     // [buf][hi, lo][row][k]
     __shared__ __attribute__((aligned(16))) uint16_t sA[2][2][BM][LDS_LD];
@@ -357,10 +358,11 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
     const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (long)j0 * ld;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wi = wave >> 1, wj = wave & 1;
+    const int wi = wave / WVN, wj = wave % WVN;
     // global -> LDS: a row of one operand tile is 32 halves = 4 x 16 B; piece e of an
-    // operand is (row e/4, quarter e%4); thread t moves pieces t, t+256, ...
-    constexpr int PA = BM / 64, PB = BN / 64;  // pieces per thread per array
+    // operand is (row e/4, quarter e%4); thread t moves pieces t, t+NT, ...
+    constexpr int PA = BM * 4 / NT, PB = BN * 4 / NT;  // pieces per thread per array
+    static_assert(PA * NT == BM * 4 && PB * NT == BN * 4, "tile rows must split evenly over threads");
 
     constexpr bool DUAL = SplitMfma<PREC>::dual;
     f16v acc[WM][WN], acc2[DUAL ? WM : 1][DUAL ? WN : 1];
@@ -388,13 +390,13 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
     do {                                                                                 \
         const long k0_ = (long)(KT) * BK;                                                \
         _Pragma("unroll") for (int u = 0; u < PA; u++) {                                 \
-            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;                       \
+            const int e = tid + NT * u, row = e >> 2, qq = e & 3;                        \
             const long off = (long)row * ld + k0_ + qq * 8;                              \
             ra[ST][0][u] = *reinterpret_cast<const u32x4 *>(Ah + off);                   \
             ra[ST][1][u] = *reinterpret_cast<const u32x4 *>(Al + off);                   \
         }                                                                                \
         _Pragma("unroll") for (int u = 0; u < PB; u++) {                                 \
-            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;                       \
+            const int e = tid + NT * u, row = e >> 2, qq = e & 3;                        \
             const long off = (long)row * ld + k0_ + qq * 8;                              \
             rb[ST][0][u] = *reinterpret_cast<const u32x4 *>(Bh + off);                   \
             rb[ST][1][u] = *reinterpret_cast<const u32x4 *>(Bl + off);                   \
@@ -403,12 +405,12 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
 #define ASD_SPLIT_SSTORE(BUF, ST)                                                        \
     do {                                                                                 \
         _Pragma("unroll") for (int u = 0; u < PA; u++) {                                 \
-            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;                       \
+            const int e = tid + NT * u, row = e >> 2, qq = e & 3;                        \
             *reinterpret_cast<u32x4 *>(&sA[BUF][0][row][qq * 8]) = ra[ST][0][u];         \
             *reinterpret_cast<u32x4 *>(&sA[BUF][1][row][qq * 8]) = ra[ST][1][u];         \
         }                                                                                \
         _Pragma("unroll") for (int u = 0; u < PB; u++) {                                 \
-            const int e = tid + 256 * u, row = e >> 2, qq = e & 3;                       \
+            const int e = tid + NT * u, row = e >> 2, qq = e & 3;                        \
             *reinterpret_cast<u32x4 *>(&sB[BUF][0][row][qq * 8]) = rb[ST][0][u];         \
             *reinterpret_cast<u32x4 *>(&sB[BUF][1][row][qq * 8]) = rb[ST][1][u];         \
         }                                                                                \
@@ -466,7 +468,7 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
     const int L = args.L, Ns = args.Ns;
     const BinGeom g = bin_geom(i0, j0, Ns, L, BM, BN);
     double *sb = reinterpret_cast<double *>(&sA[0][0][0][0]);
-    for (int b = tid; b < g.nbins; b += 256) sb[b] = 0.0;
+    for (int b = tid; b < g.nbins; b += NT) sb[b] = 0.0;
     int rj[WN], cj[WN];
 #pragma unroll
     for (int n = 0; n < WN; n++) {
@@ -503,17 +505,19 @@ __global__ __launch_bounds__(256) void sqt_gemm_split_kernel(GemmArgs args)
 struct SqtCfg {
     int wm, wn, group;
     const char *name;
+    int wvm = 2, wvn = 2;  // waves per workgroup along M and N
 };
 static const SqtCfg kSplitCfgs[] = {
-    {2, 2, 8, "128x128 tiles, 64x64 per wave, group 8"},
-    {2, 2, 16, "128x128 tiles, 64x64 per wave, group 16"},
-    {2, 4, 8, "128x256 tiles, 64x128 per wave, group 8"},
-    {4, 2, 8, "256x128 tiles, 128x64 per wave, group 8"},
-    {4, 2, 16, "256x128 tiles, 128x64 per wave, group 16"},
+    {2, 2, 8, "128x128 tiles, 4 waves of 64x64, group 8"},
+    {1, 2, 8, "128x128 tiles, 8 waves of 32x64, group 8", 4, 2},
+    {2, 1, 8, "128x128 tiles, 8 waves of 64x32, group 8", 2, 4},
+    {2, 2, 16, "128x128 tiles, 4 waves of 64x64, group 16"},
+    {2, 4, 8, "128x256 tiles, 4 waves of 64x128, group 8"},
+    {4, 2, 8, "256x128 tiles, 4 waves of 128x64, group 8"},
 };
 // FP16X2 keeps two accumulators: the 64x128 / 128x64 wave tiles do not fit the
-// register file with the two-stage prefetch, so only 64x64 wave tiles are offered
-constexpr int kNumSplitFp16 = 2;
+// register file with the two-stage prefetch, so only the first four are offered
+constexpr int kNumSplitFp16 = 4;
 static const SqtCfg kKmajorCfgs[] = {
     {2, 2, 8, "128x128 tiles, 64x64 per wave, group 8"},
     {2, 2, 16, "128x128 tiles, 64x64 per wave, group 16"},
@@ -535,20 +539,22 @@ const char *sqt_config_name(asd_precision prec, int cfg)
 template <int PREC>
 static void launch_split(const SqtCfg &c, GemmArgs ga, hipStream_t st)
 {
-    const int BM = 64 * c.wm, BN = 64 * c.wn;
+    const int BM = 32 * c.wvm * c.wm, BN = 32 * c.wvn * c.wn;
     const long total = (long)((ga.Ns + BM - 1) / BM) * ((ga.Ns + BN - 1) / BN) * ga.nlags;
-    const dim3 grid((unsigned)std::min(num_slots(total), kMaxGrid)), blk(256);
+    const dim3 grid((unsigned)std::min(num_slots(total), kMaxGrid)), blk(64 * c.wvm * c.wvn);
     ga.group = c.group;
-    if constexpr (PREC == ASD_FP16X2) {
+    if (c.wvm == 4 && c.wvn == 2)
+        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 1, 2, 4, 2>), grid, blk, 0, st, ga);
+    else if (c.wvm == 2 && c.wvn == 4)
+        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 1, 2, 4>), grid, blk, 0, st, ga);
+    else if constexpr (PREC == ASD_FP16X2)
         hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 2>), grid, blk, 0, st, ga);
-    } else {
-        if (c.wm == 2 && c.wn == 2)
-            hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 2>), grid, blk, 0, st, ga);
-        else if (c.wm == 2 && c.wn == 4)
-            hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 4>), grid, blk, 0, st, ga);
-        else
-            hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 4, 2>), grid, blk, 0, st, ga);
-    }
+    else if (c.wm == 2 && c.wn == 2)
+        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 2>), grid, blk, 0, st, ga);
+    else if (c.wm == 2 && c.wn == 4)
+        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 4>), grid, blk, 0, st, ga);
+    else
+        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 4, 2>), grid, blk, 0, st, ga);
 }
 
 // ----------------------------------------------------------- FP64 finishing

@@ -168,3 +168,43 @@ def test_large_work_list_all_lags_computed(gctx, orc, prec, tol):
     err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
     say(f"  {prec} L=256 per-lag max rel err: first {err[0]:.2e} last {err[-1]:.2e} max {err.max():.2e}")
     assert err.max() <= tol
+
+
+# Oracle bounds at these small samples (9 or fewer time origins), relative to
+# max |S| over all (q, tau).  BF16X3 keeps about 17 significant bits per operand, so
+# with few origins to average over its error reaches ~3e-5; a wrong fragment layout
+# or a dropped tile gives errors of order 1.
+SMALL_SAMPLE_TOL = {"fp64": 1e-10, "fp32": 1e-5, "bf16x3": 1e-4, "fp16x2": 1e-5}
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3", "fp16x2"])
+def test_every_kernel_config_matches_oracle(gctx, orc, prec):
+    """Every tile / wave layout the autotuner may pick gives the oracle's S(q,tau):
+    lattices whose site count is not a multiple of any tile edge (400, 1296) and
+    one smaller than a tile (144), so partial tiles and masked rows are covered.
+    Each output element is accumulated over k in the same order by every layout,
+    so the layouts agree with each other up to the order of the FP64 bin sums."""
+    torch = need_gpu()
+    import asd_amd
+    n = asd_amd.lib().asd_sqt_num_configs(asd_amd.PRECISIONS[prec])
+    try:
+        for L, T0, nlags in ((20, 9, 5), (36, 6, 4), (12, 7, 3)):
+            snaps = trajectory(orc, L, T0 + nlags - 1, D=0.1, Bz=0.05)
+            q = gxmg(L)
+            ref = orc.sqt_gemm(L, snaps, T0, nlags, q)
+            d = torch.tensor(snaps, device="cuda")
+            first = None
+            for cfg in range(n):
+                name = asd_amd.lib().asd_sqt_config_name(asd_amd.PRECISIONS[prec], cfg).decode()
+                say(f"sqt {prec} L={L} config {cfg} ({name})")
+                gctx.set_config(prec, cfg)
+                got = gctx.sqt(L, d, T0, nlags, q, prec)
+                e = max_rel(got, ref)
+                first = got if first is None else first
+                ec = max_rel(got, first)
+                say(f"  max rel err vs oracle {e:.3e}, vs config 0 {ec:.3e}")
+                assert e <= SMALL_SAMPLE_TOL[prec], (L, cfg, name, e)
+                assert ec <= 1e-12, (L, cfg, name, ec)
+    finally:
+        gctx.set_config(prec, -1)

@@ -416,17 +416,25 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
         }                                                                                \
     } while (0)
 
+    // Kpad is a multiple of 2 BK (the host pads with zeros), so nk is even and the
+    // body below has no branch: the prefetch of the last two steps re-reads chunk
+    // nk - 1 (in bounds, never stored to a buffer that is read again).  Loads that
+    // are always issued let the compiler count outstanding loads exactly and wait
+    // only for the ones each LDS store needs.
     const int nk = args.Kpad / BK;
     ASD_SPLIT_GLOAD(0, 0);
-    if (nk > 1) ASD_SPLIT_GLOAD(1, 1);
+    ASD_SPLIT_GLOAD(1, 1);
     ASD_SPLIT_SSTORE(0, 0);
     __syncthreads();
     for (int kt = 0; kt < nk; kt += 2) {
 #pragma unroll
         for (int h = 0; h < 2; h++) {
             const int k = kt + h;
-            if (k < nk) {
-                if (k + 2 < nk) ASD_SPLIT_GLOAD(k + 2, h);
+            {
+                ASD_SPLIT_GLOAD(min(k + 2, nk - 1), h);
+                // keep the prefetch ahead of this step's MFMAs (the scheduler would
+                // otherwise sink it below the LDS stores to reuse their registers)
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int kk = 0; kk < BK / 16; kk++) {
                     const int kc = kk * 16 + 8 * (lane / 32);
@@ -457,7 +465,7 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
                             }
                         }
                 }
-                if (k + 1 < nk) ASD_SPLIT_SSTORE(h ^ 1, h ^ 1);
+                ASD_SPLIT_SSTORE(h ^ 1, h ^ 1);
                 __syncthreads();
             }
         }
@@ -469,32 +477,63 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
     const BinGeom g = bin_geom(i0, j0, Ns, L, BM, BN);
     double *sb = reinterpret_cast<double *>(&sA[0][0][0][0]);
     for (int b = tid; b < g.nbins; b += NT) sb[b] = 0.0;
-    int rj[WN], cj[WN];
-#pragma unroll
-    for (int n = 0; n < WN; n++) {
-        const int j = j0 + wj * WN * 32 + n * 32 + lane % 32;
-        rj[n] = j / L;
-        cj[n] = j - rj[n] * L;
-    }
     __syncthreads();
+    if (i0 + BM <= Ns && j0 + BN <= Ns && L >= 32) {
+        // full tile: with i = ri L + ci and j = rj L + cj the bin
+        // (ri - rj - drmin) L + (ci - cj mod L) is (i - j) - drmin L + (ci < cj ? L : 0).
+        // A lane's rows are ib + 32 m + off(v) with off(v) < 32 <= L, so ci needs at
+        // most one wrap and no division per element.
+        const int ib = i0 + wi * WM * 32 + 4 * (lane / 32);
+        int jb[WN], cj[WN];
 #pragma unroll
-    for (int m = 0; m < WM; m++)
+        for (int n = 0; n < WN; n++) {
+            const int j = j0 + wj * WN * 32 + n * 32 + lane % 32;
+            jb[n] = j + g.drmin * L;
+            cj[n] = j % L;
+        }
 #pragma unroll
-        for (int v = 0; v < 16; v++) {
-            const int i = i0 + wi * WM * 32 + m * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-            if (i >= Ns) continue;
-            const int ri = i / L, ci = i - ri * L;
+        for (int m = 0; m < WM; m++) {
+            const int im = ib + m * 32, cim = im % L;
 #pragma unroll
-            for (int n = 0; n < WN; n++) {
-                const int j = j0 + wj * WN * 32 + n * 32 + lane % 32;
-                if (j >= Ns) continue;
-                int dc = ci - cj[n];
-                dc += dc < 0 ? L : 0;
-                double val = (double)acc[m][n][v];
-                if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
-                atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], val);
+            for (int v = 0; v < 16; v++) {
+                const int off = 8 * (v / 4) + (v % 4);
+                int ci = cim + off;
+                ci -= ci >= L ? L : 0;
+#pragma unroll
+                for (int n = 0; n < WN; n++) {
+                    double val = (double)acc[m][n][v];
+                    if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
+                    atomicAdd(&sb[im - jb[n] + (ci < cj[n] ? L : 0) + off], val);
+                }
             }
         }
+    } else {
+        int rj[WN], cj[WN];
+#pragma unroll
+        for (int n = 0; n < WN; n++) {
+            const int j = j0 + wj * WN * 32 + n * 32 + lane % 32;
+            rj[n] = j / L;
+            cj[n] = j - rj[n] * L;
+        }
+#pragma unroll
+        for (int m = 0; m < WM; m++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int i = i0 + wi * WM * 32 + m * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+                if (i >= Ns) continue;
+                const int ri = i / L, ci = i - ri * L;
+#pragma unroll
+                for (int n = 0; n < WN; n++) {
+                    const int j = j0 + wj * WN * 32 + n * 32 + lane % 32;
+                    if (j >= Ns) continue;
+                    int dc = ci - cj[n];
+                    dc += dc < 0 ? L : 0;
+                    double val = (double)acc[m][n][v];
+                    if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
+                    atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], val);
+                }
+            }
+    }
     __syncthreads();
     flush_bins(sb, g, L, args.bins + (long)w.lag * Ns);
     }
@@ -855,7 +894,7 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         else
             hipLaunchKernelGGL(sqt_gemm_kmajor_kernel<float>, dim3(nblocks), dim3(256), 0, st, ga);
     } else {
-        const long Kpad = round_up(K, 32);
+        const long Kpad = round_up(K, 64);  // an even number of 32-wide k steps
         const long ldk = round_up(3L * (nlags - 1) + Kpad + 8, 8);  // halves per i-row
         const long cs = ldk * ldi;                                   // one shifted copy
         ASD_TRY(ctx->opA.ensure(2 * 8 * cs));

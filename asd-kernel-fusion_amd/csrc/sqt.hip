@@ -23,6 +23,8 @@
 
 #include <algorithm>
 #include <cmath>
+#include <map>
+#include <mutex>
 
 #include "asd_internal.h"
 
@@ -98,31 +100,32 @@ struct Work {
     int lag, ti, tj;
 };
 
-// Work slots are processed by a capped grid in a grid-stride loop (a grid of one
-// block per tile would overflow the 32-bit dispatch size at L = 256).  The grid is a
-// multiple of 8 and blocks are dispatched round-robin over the 8 XCDs (block b ->
-// XCD b % 8), so slot s always runs on XCD s % 8: give each XCD a contiguous slice of
-// the work list, and order tiles in groups of `group` tile-rows so that concurrently
-// running tiles share A and B panels in the XCD's L2.
-__device__ __forceinline__ bool decode_work(long slot, long nslots, int nti, int ntj, int nlags, int group, Work &w)
+// Work slots are processed by a persistent grid (as many blocks as are resident at
+// once, a multiple of 8) in a grid-stride loop.  Blocks are dispatched round-robin
+// over the 8 XCDs (block b -> XCD b % 8), so slot s always runs on XCD s % 8.  XCD x
+// takes the x-th eighth of every lag's tiles and all XCDs walk the lags in step: the
+// operands in flight are one lag's A window plus B, which the shared last-level cache
+// holds.  Within an XCD, consecutive slots run together; tiles are ordered in groups
+// of `group` tile-rows so that they share A and B panels in the XCD's L2.
+__device__ __forceinline__ bool decode_work(long slot, int nti, int ntj, int nlags, int group, Work &w)
 {
-    const long per = nslots / 8;
-    const long wid = (slot % 8) * per + slot / 8;
-    const long per_lag = (long)nti * ntj;
-    if (wid >= per_lag * nlags) return false;
-    w.lag = (int)(wid / per_lag);
-    const int r = (int)(wid - (long)w.lag * per_lag);
-    const int span = group * ntj;
-    const int g = r / span, first = g * group;
-    const int gsz = min(nti - first, group);
-    const int rr = r - g * span;
+    const long per_lag = (long)nti * ntj, per_x = (per_lag + 7) / 8;
+    const long u = slot / 8, lag = u / per_x;
+    if (lag >= nlags) return false;
+    const long r = (slot % 8) * per_x + (u - lag * per_x);
+    if (r >= per_lag) return false;
+    w.lag = (int)lag;
+    const int span = group * ntj, g = (int)(r / span), first = g * group;
+    const int gsz = min(nti - first, group), rr = (int)(r - (long)g * span);
     w.ti = first + rr % gsz;
     w.tj = rr / gsz;
     return true;
 }
 
-constexpr long kMaxGrid = 1L << 16;  // 65536 blocks x 256 threads
-__host__ __device__ inline long num_slots(long total) { return (total + 7) / 8 * 8; }
+constexpr long kMaxGrid = 1L << 16;  // cap on the persistent grid (blocks)
+
+// slots for nlags lags of per_lag tiles (each XCD gets ceil(per_lag / 8) per lag)
+__host__ __device__ inline long num_slots(long per_lag, int nlags) { return 8 * ((per_lag + 7) / 8) * nlags; }
 
 // --------------------------------------------------------- binning epilogue
 
@@ -201,10 +204,10 @@ __global__ __launch_bounds__(256) void sqt_gemm_kmajor_kernel(GemmArgs args)
     T(*sB)[BK][LDS_LD] = reinterpret_cast<T(*)[BK][LDS_LD]>(smem_raw + 2 * BK * LDS_LD * sizeof(T));
 
     const int nt_tiles = (args.Ns + TILE - 1) / TILE;
-    const long nslots = num_slots((long)nt_tiles * nt_tiles * args.nlags);
+    const long nslots = num_slots((long)nt_tiles * nt_tiles, args.nlags);
     for (long slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
     Work w;
-    if (!decode_work(slot, nslots, nt_tiles, nt_tiles, args.nlags, args.group, w)) continue;
+    if (!decode_work(slot, nt_tiles, nt_tiles, args.nlags, args.group, w)) continue;
     __syncthreads();  // LDS of the previous work item fully consumed
     const int i0 = w.ti * TILE, j0 = w.tj * TILE;
     const long ld = args.ld;
@@ -331,6 +334,81 @@ template <> struct SplitMfma<ASD_FP16X2> {
     static constexpr bool dual = true;
 };
 
+// Epilogue of the 32x32x16 split kernels: bins one wave's accumulators (rows
+// i0 + r0 + 32 m + 8 (v / 4) + 4 (lane / 32) + v % 4, columns j0 + c0 + 32 n + lane % 32)
+// into the tile's displacement bins in LDS (sb, zeroed here; the caller has finished
+// with that LDS) and adds them to the lag's global bins.  FP16X2 keeps the cross
+// products, scaled by 2^11, in acc2.
+template <int WM, int WN, bool DUAL>
+__device__ __forceinline__ void bin_split_tile(const f16v (&acc)[WM][WN],
+                                               const f16v (&acc2)[DUAL ? WM : 1][DUAL ? WN : 1], double *sb,
+                                               int i0, int j0, int r0, int c0, int BM, int BN, int Ns, int L,
+                                               double *gbins)
+{
+    const int lane = threadIdx.x & 63;
+    const BinGeom g = bin_geom(i0, j0, Ns, L, BM, BN);
+    for (int b = threadIdx.x; b < g.nbins; b += blockDim.x) sb[b] = 0.0;
+    __syncthreads();
+    if (i0 + BM <= Ns && j0 + BN <= Ns && L >= 32) {
+        // full tile: with i = ri L + ci and j = rj L + cj the bin
+        // (ri - rj - drmin) L + (ci - cj mod L) is (i - j) - drmin L + (ci < cj ? L : 0).
+        // A lane's rows are ib + 32 m + off(v) with off(v) < 32 <= L, so ci needs at
+        // most one wrap and no division per element.
+        const int ib = i0 + r0 + 4 * (lane / 32);
+        int jb[WN], cj[WN];
+#pragma unroll
+        for (int n = 0; n < WN; n++) {
+            const int j = j0 + c0 + n * 32 + lane % 32;
+            jb[n] = j + g.drmin * L;
+            cj[n] = j % L;
+        }
+#pragma unroll
+        for (int m = 0; m < WM; m++) {
+            const int im = ib + m * 32, cim = im % L;
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int off = 8 * (v / 4) + (v % 4);
+                int ci = cim + off;
+                ci -= ci >= L ? L : 0;
+#pragma unroll
+                for (int n = 0; n < WN; n++) {
+                    double val = (double)acc[m][n][v];
+                    if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
+                    atomicAdd(&sb[im - jb[n] + (ci < cj[n] ? L : 0) + off], val);
+                }
+            }
+        }
+    } else {
+        int rj[WN], cj[WN];
+#pragma unroll
+        for (int n = 0; n < WN; n++) {
+            const int j = j0 + c0 + n * 32 + lane % 32;
+            rj[n] = j / L;
+            cj[n] = j - rj[n] * L;
+        }
+#pragma unroll
+        for (int m = 0; m < WM; m++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int i = i0 + r0 + m * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+                if (i >= Ns) continue;
+                const int ri = i / L, ci = i - ri * L;
+#pragma unroll
+                for (int n = 0; n < WN; n++) {
+                    const int j = j0 + c0 + n * 32 + lane % 32;
+                    if (j >= Ns) continue;
+                    int dc = ci - cj[n];
+                    dc += dc < 0 ? L : 0;
+                    double val = (double)acc[m][n][v];
+                    if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
+                    atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], val);
+                }
+            }
+    }
+    __syncthreads();
+    flush_bins(sb, g, L, gbins);
+}
+
 template <int PREC, int WM, int WN, int WVM = 2, int WVN = 2>
 __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs args)
 {
@@ -343,10 +421,10 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
     __shared__ __attribute__((aligned(16))) uint16_t sB[2][2][BN][LDS_LD];
 
     const int nti = (args.Ns + BM - 1) / BM, ntj = (args.Ns + BN - 1) / BN;
-    const long nslots = num_slots((long)nti * ntj * args.nlags);
+    const long nslots = num_slots((long)nti * ntj, args.nlags);
     for (long slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
     Work w;
-    if (!decode_work(slot, nslots, nti, ntj, args.nlags, args.group, w)) continue;
+    if (!decode_work(slot, nti, ntj, args.nlags, args.group, w)) continue;
     __syncthreads();  // LDS of the previous work item fully consumed
     const int i0 = w.ti * BM, j0 = w.tj * BN;
     const long ld = args.ld;
@@ -473,69 +551,168 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
 #undef ASD_SPLIT_GLOAD
 #undef ASD_SPLIT_SSTORE
 
-    const int L = args.L, Ns = args.Ns;
-    const BinGeom g = bin_geom(i0, j0, Ns, L, BM, BN);
-    double *sb = reinterpret_cast<double *>(&sA[0][0][0][0]);
-    for (int b = tid; b < g.nbins; b += NT) sb[b] = 0.0;
-    __syncthreads();
-    if (i0 + BM <= Ns && j0 + BN <= Ns && L >= 32) {
-        // full tile: with i = ri L + ci and j = rj L + cj the bin
-        // (ri - rj - drmin) L + (ci - cj mod L) is (i - j) - drmin L + (ci < cj ? L : 0).
-        // A lane's rows are ib + 32 m + off(v) with off(v) < 32 <= L, so ci needs at
-        // most one wrap and no division per element.
-        const int ib = i0 + wi * WM * 32 + 4 * (lane / 32);
-        int jb[WN], cj[WN];
-#pragma unroll
-        for (int n = 0; n < WN; n++) {
-            const int j = j0 + wj * WN * 32 + n * 32 + lane % 32;
-            jb[n] = j + g.drmin * L;
-            cj[n] = j % L;
-        }
-#pragma unroll
-        for (int m = 0; m < WM; m++) {
-            const int im = ib + m * 32, cim = im % L;
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int off = 8 * (v / 4) + (v % 4);
-                int ci = cim + off;
-                ci -= ci >= L ? L : 0;
-#pragma unroll
-                for (int n = 0; n < WN; n++) {
-                    double val = (double)acc[m][n][v];
-                    if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
-                    atomicAdd(&sb[im - jb[n] + (ci < cj[n] ? L : 0) + off], val);
-                }
-            }
-        }
-    } else {
-        int rj[WN], cj[WN];
-#pragma unroll
-        for (int n = 0; n < WN; n++) {
-            const int j = j0 + wj * WN * 32 + n * 32 + lane % 32;
-            rj[n] = j / L;
-            cj[n] = j - rj[n] * L;
-        }
-#pragma unroll
-        for (int m = 0; m < WM; m++)
-#pragma unroll
-            for (int v = 0; v < 16; v++) {
-                const int i = i0 + wi * WM * 32 + m * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
-                if (i >= Ns) continue;
-                const int ri = i / L, ci = i - ri * L;
-#pragma unroll
-                for (int n = 0; n < WN; n++) {
-                    const int j = j0 + wj * WN * 32 + n * 32 + lane % 32;
-                    if (j >= Ns) continue;
-                    int dc = ci - cj[n];
-                    dc += dc < 0 ? L : 0;
-                    double val = (double)acc[m][n][v];
-                    if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
-                    atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], val);
-                }
-            }
+    bin_split_tile<WM, WN, DUAL>(acc, acc2, reinterpret_cast<double *>(&sA[0][0][0][0]), i0, j0,
+                                 wi * WM * 32, wj * WN * 32, BM, BN, args.Ns, args.L,
+                                 args.bins + (long)w.lag * args.Ns);
     }
+}
+
+// s_waitcnt immediates (gfx9 encoding: vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt[5:4]
+// in [15:14]) that wait on one counter only
+constexpr int wait_vmcnt(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14); }
+constexpr int kWaitLgkm0 = 15 | (7 << 4) | (3 << 14);
+
+// The split-precision GEMM with operand tiles moved by direct global -> LDS loads
+// (global_load_lds_dwordx4: 64 lanes x 16 B into 1 KiB of LDS at a wave-uniform base)
+// through a ring of NS stages, so no staging registers and no LDS stores are needed
+// and every chunk is requested NS - 1 steps before it is used.  A stage holds
+// [A hi][A lo][B hi][B lo], rows of 32 halves (64 B) with no padding; the 16-byte
+// quarter q of row r sits in slot q ^ ((r >> 2) & 3), which spreads any 16
+// consecutive rows read at one quarter over all 64 LDS banks.
+//
+// Synchronisation per k step (one barrier): wait for this wave's loads of the step's
+// stage (the loads of the NS - 2 later stages may stay in flight) and for its LDS
+// reads of the previous step, barrier, then issue the stage that reuses the previous
+// step's buffer, then compute.  The barrier is the bare s_barrier: the workgroup fence
+// of __syncthreads would wait for every load in flight.  The loads are always issued
+// (the last ones re-read chunk nk - 1) so the counts waited for are exact.
+template <int PREC, int WM, int WN, int WVM, int WVN, int NS>
+__global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs args)
+{
+    constexpr int BK = 32;                      // halves per row of a stage
+    constexpr int NW = WVM * WVN;
+    constexpr int BM = WVM * WM * 32, BN = WVN * WN * 32;
+    constexpr int CA = BM / 16 / NW, CB = BN / 16 / NW;  // 1 KiB chunks per wave per array
+    static_assert(CA * 16 * NW == BM && CB * 16 * NW == BN, "tile rows must split evenly over waves");
+    constexpr int PER_STAGE = 2 * (CA + CB);    // loads per wave per stage
+    static_assert(PER_STAGE * (NS - 2) <= 63, "vmcnt range");
+    constexpr int OFF_AL = BM * BK, OFF_BH = 2 * BM * BK, OFF_BL = 2 * BM * BK + BN * BK;
+    constexpr int STAGE = 2 * (BM + BN) * BK;   // halves
+    typedef typename SplitMfma<PREC>::v_t v_t;
+    __shared__ __attribute__((aligned(1024))) uint16_t lds[NS * STAGE];
+
+    const int nti = (args.Ns + BM - 1) / BM, ntj = (args.Ns + BN - 1) / BN;
+    const long nslots = num_slots((long)nti * ntj, args.nlags);
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wi = wave / WVN, wj = wave % WVN;
+    // load side: lane l of a chunk fills slot (row l / 4, quarter l % 4), so it fetches
+    // global quarter (l % 4) ^ (l / 16) of that row (rows of a chunk start at a multiple
+    // of 16, so (r >> 2) & 3 is l / 16)
+    const int lrow = lane >> 2, lq = (lane & 3) ^ (lane >> 4);
+    // read side: fragment lane l reads row l % 32 of a 32-row block at quarter
+    // 2 kk + l / 32, found in slot quarter (2 kk + l / 32) ^ ((l >> 2) & 3)
+    const int rrow = lane & 31, rx = (lane >> 2) & 3, rhalf = lane >> 5;
+
+    for (long slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
+    Work w;
+    if (!decode_work(slot, nti, ntj, args.nlags, args.group, w)) continue;
+    __syncthreads();  // LDS (bins) of the previous work item fully consumed
+    const int i0 = w.ti * BM, j0 = w.tj * BN;
+    const long ld = args.ld;
+    const long koff = 3L * w.lag, shift = koff & 7;
+    const long acopy = shift * args.copy_stride + (long)i0 * ld + (koff - shift);
+    const uint16_t *Ah = static_cast<const uint16_t *>(args.A) + acopy;
+    const uint16_t *Al = static_cast<const uint16_t *>(args.A2) + acopy;
+    const uint16_t *Bh = static_cast<const uint16_t *>(args.B) + (long)j0 * ld;
+    const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (long)j0 * ld;
+    // per-lane source offsets of this wave's chunks (chunk c = wave * C + u: rows 16 c ..)
+    long srcA[CA], srcB[CB];
+#pragma unroll
+    for (int u = 0; u < CA; u++) srcA[u] = (long)(16 * (wave * CA + u) + lrow) * ld + 8 * lq;
+#pragma unroll
+    for (int u = 0; u < CB; u++) srcB[u] = (long)(16 * (wave * CB + u) + lrow) * ld + 8 * lq;
+
+    auto issue = [&](int kt, int st) {
+        const long k0 = (long)kt * BK;
+        uint16_t *base = lds + st * STAGE;
+#pragma unroll
+        for (int u = 0; u < CA; u++) {
+            const int c = (wave * CA + u) * 512;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(Ah + srcA[u] + k0),
+                                             (__attribute__((address_space(3))) void *)(base + c), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(Al + srcA[u] + k0),
+                                             (__attribute__((address_space(3))) void *)(base + OFF_AL + c), 16, 0, 0);
+        }
+#pragma unroll
+        for (int u = 0; u < CB; u++) {
+            const int c = (wave * CB + u) * 512;
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(Bh + srcB[u] + k0),
+                                             (__attribute__((address_space(3))) void *)(base + OFF_BH + c), 16, 0, 0);
+            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(Bl + srcB[u] + k0),
+                                             (__attribute__((address_space(3))) void *)(base + OFF_BL + c), 16, 0, 0);
+        }
+    };
+
+    constexpr bool DUAL = SplitMfma<PREC>::dual;
+    f16v acc[WM][WN], acc2[DUAL ? WM : 1][DUAL ? WN : 1];
+#pragma unroll
+    for (int a = 0; a < WM; a++)
+#pragma unroll
+        for (int b = 0; b < WN; b++)
+#pragma unroll
+            for (int e = 0; e < 16; e++) acc[a][b][e] = 0.0f;
+    if constexpr (DUAL) {
+#pragma unroll
+        for (int a = 0; a < WM; a++)
+#pragma unroll
+            for (int b = 0; b < WN; b++)
+#pragma unroll
+                for (int e = 0; e < 16; e++) acc2[a][b][e] = 0.0f;
+    }
+
+    const int nk = args.Kpad / BK;
+#pragma unroll
+    for (int st = 0; st < NS - 1; st++) issue(min(st, nk - 1), st);
+    for (int kt = 0; kt < nk; kt += NS) {
+#pragma unroll
+        for (int h = 0; h < NS; h++) {
+            const int k = kt + h;
+            if (k < nk) {
+                __builtin_amdgcn_s_waitcnt(wait_vmcnt(PER_STAGE * (NS - 2)));
+                __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+                __builtin_amdgcn_s_barrier();
+                asm volatile("" ::: "memory");
+                issue(min(k + NS - 1, nk - 1), (h + NS - 1) % NS);
+                const uint16_t *sb = lds + h * STAGE;
+#pragma unroll
+                for (int kk = 0; kk < BK / 16; kk++) {
+                    const int qo = 8 * ((2 * kk + rhalf) ^ rx);
+                    v_t ah[WM], al[WM], bh[WN], bl[WN];
+#pragma unroll
+                    for (int m = 0; m < WM; m++) {
+                        const int r = wi * WM * 32 + m * 32 + rrow;
+                        ah[m] = *reinterpret_cast<const v_t *>(sb + r * BK + qo);
+                        al[m] = *reinterpret_cast<const v_t *>(sb + OFF_AL + r * BK + qo);
+                    }
+#pragma unroll
+                    for (int n = 0; n < WN; n++) {
+                        const int r = wj * WN * 32 + n * 32 + rrow;
+                        bh[n] = *reinterpret_cast<const v_t *>(sb + OFF_BH + r * BK + qo);
+                        bl[n] = *reinterpret_cast<const v_t *>(sb + OFF_BL + r * BK + qo);
+                    }
+#pragma unroll
+                    for (int m = 0; m < WM; m++)
+#pragma unroll
+                        for (int n = 0; n < WN; n++) {
+                            acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bh[n], acc[m][n]);
+                            if constexpr (DUAL) {
+                                acc2[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc2[m][n]);
+                                acc2[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc2[m][n]);
+                            } else {
+                                acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc[m][n]);
+                                acc[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc[m][n]);
+                            }
+                        }
+                }
+                asm volatile("" ::: "memory");
+            }
+        }
+    }
+    // every load in flight lands before the bins reuse the LDS
+    __builtin_amdgcn_s_waitcnt(wait_vmcnt(0));
     __syncthreads();
-    flush_bins(sb, g, L, args.bins + (long)w.lag * Ns);
+    bin_split_tile<WM, WN, DUAL>(acc, acc2, reinterpret_cast<double *>(lds), i0, j0, wi * WM * 32,
+                                 wj * WN * 32, BM, BN, args.Ns, args.L, args.bins + (long)w.lag * args.Ns);
     }
 }
 
@@ -545,18 +722,22 @@ struct SqtCfg {
     int wm, wn, group;
     const char *name;
     int wvm = 2, wvn = 2;  // waves per workgroup along M and N
+    int dma = 0;           // stages of the direct-to-LDS ring (0: register-staged loads)
 };
 static const SqtCfg kSplitCfgs[] = {
     {2, 2, 8, "128x128 tiles, 4 waves of 64x64, group 8"},
     {1, 2, 8, "128x128 tiles, 8 waves of 32x64, group 8", 4, 2},
     {2, 1, 8, "128x128 tiles, 8 waves of 64x32, group 8", 2, 4},
+    {1, 2, 8, "128x128 tiles, 8 waves of 32x64, 4-stage LDS ring, group 8", 4, 2, 4},
+    {2, 1, 8, "128x128 tiles, 8 waves of 64x32, 4-stage LDS ring, group 8", 2, 4, 4},
+    {2, 2, 8, "256x128 tiles, 8 waves of 64x64, 3-stage LDS ring, group 8", 4, 2, 3},
     {2, 2, 16, "128x128 tiles, 4 waves of 64x64, group 16"},
     {2, 4, 8, "128x256 tiles, 4 waves of 64x128, group 8"},
     {4, 2, 8, "256x128 tiles, 4 waves of 128x64, group 8"},
 };
-// FP16X2 keeps two accumulators: the 64x128 / 128x64 wave tiles do not fit the
-// register file with the two-stage prefetch, so only the first four are offered
-constexpr int kNumSplitFp16 = 4;
+// FP16X2 keeps two accumulators: the 64x128 / 128x64 register-staged wave tiles do not
+// fit the register file, so only the first six are offered
+constexpr int kNumSplitFp16 = 6;
 static const SqtCfg kKmajorCfgs[] = {
     {2, 2, 8, "128x128 tiles, 64x64 per wave, group 8"},
     {2, 2, 16, "128x128 tiles, 64x64 per wave, group 16"},
@@ -575,25 +756,55 @@ const char *sqt_config_name(asd_precision prec, int cfg)
     return prec <= ASD_FP32 ? kKmajorCfgs[cfg].name : kSplitCfgs[cfg].name;
 }
 
+using GemmKernel = void (*)(GemmArgs);
+
+// persistent grid: the blocks of `k` resident at once on the device (a multiple of 8,
+// so that slot s stays on XCD s % 8), at most one per slot
+static int resident_grid(GemmKernel k, int threads, long nslots, int device)
+{
+    static std::mutex mu;
+    static std::map<std::pair<int, const void *>, int> cache;
+    std::lock_guard<std::mutex> lock(mu);
+    const auto key = std::make_pair(device, reinterpret_cast<const void *>(k));
+    auto it = cache.find(key);
+    if (it == cache.end()) {
+        int per_cu = 0, cus = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k, threads, 0) != hipSuccess) per_cu = 1;
+        if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) != hipSuccess) cus = 8;
+        const long n = (long)std::max(per_cu, 1) * std::max(cus, 8) / 8 * 8;
+        it = cache.emplace(key, (int)std::min(n, kMaxGrid)).first;
+    }
+    return (int)std::min<long>(it->second, nslots);
+}
+
 template <int PREC>
-static void launch_split(const SqtCfg &c, GemmArgs ga, hipStream_t st)
+static void launch_split(const SqtCfg &c, GemmArgs ga, hipStream_t st, int device)
 {
     const int BM = 32 * c.wvm * c.wm, BN = 32 * c.wvn * c.wn;
-    const long total = (long)((ga.Ns + BM - 1) / BM) * ((ga.Ns + BN - 1) / BN) * ga.nlags;
-    const dim3 grid((unsigned)std::min(num_slots(total), kMaxGrid)), blk(64 * c.wvm * c.wvn);
+    const long per_lag = (long)((ga.Ns + BM - 1) / BM) * ((ga.Ns + BN - 1) / BN);
+    const int threads = 64 * c.wvm * c.wvn;
     ga.group = c.group;
-    if (c.wvm == 4 && c.wvn == 2)
-        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 1, 2, 4, 2>), grid, blk, 0, st, ga);
+    GemmKernel k;
+    if (c.dma == 4 && c.wvm == 4)
+        k = sqt_gemm_dma_kernel<PREC, 1, 2, 4, 2, 4>;
+    else if (c.dma == 4)
+        k = sqt_gemm_dma_kernel<PREC, 2, 1, 2, 4, 4>;
+    else if (c.dma == 3)
+        k = sqt_gemm_dma_kernel<PREC, 2, 2, 4, 2, 3>;
+    else if (c.wvm == 4 && c.wvn == 2)
+        k = sqt_gemm_split_kernel<PREC, 1, 2, 4, 2>;
     else if (c.wvm == 2 && c.wvn == 4)
-        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 1, 2, 4>), grid, blk, 0, st, ga);
+        k = sqt_gemm_split_kernel<PREC, 2, 1, 2, 4>;
     else if constexpr (PREC == ASD_FP16X2)
-        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 2>), grid, blk, 0, st, ga);
+        k = sqt_gemm_split_kernel<PREC, 2, 2>;
     else if (c.wm == 2 && c.wn == 2)
-        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 2>), grid, blk, 0, st, ga);
+        k = sqt_gemm_split_kernel<PREC, 2, 2>;
     else if (c.wm == 2 && c.wn == 4)
-        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 2, 4>), grid, blk, 0, st, ga);
+        k = sqt_gemm_split_kernel<PREC, 2, 4>;
     else
-        hipLaunchKernelGGL((sqt_gemm_split_kernel<PREC, 4, 2>), grid, blk, 0, st, ga);
+        k = sqt_gemm_split_kernel<PREC, 4, 2>;
+    const dim3 grid(resident_grid(k, threads, num_slots(per_lag, ga.nlags), device)), blk(threads);
+    hipLaunchKernelGGL(k, grid, blk, 0, st, ga);
 }
 
 // ----------------------------------------------------------- FP64 finishing
@@ -815,10 +1026,6 @@ __global__ void zero_kernel(double *p, long n)
 
 static long round_up(long a, long b) { return (a + b - 1) / b * b; }
 
-static int grid_for(int nt_tiles, int nlags)
-{
-    return (int)std::min(num_slots((long)nt_tiles * nt_tiles * nlags), kMaxGrid);
-}
 
 int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_b, long tstride,
             int nsnap_a, int T0, int nlags, const int *h_q, int nq, asd_precision prec, double *d_out,
@@ -863,7 +1070,6 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
     ga.nt = nt_tiles;
     ga.nlags = nlags;
     ga.bins = ctx->bins.as<double>();
-    const int nblocks = grid_for(nt_tiles, nlags);
     ga.group = kKmajorCfgs[prec <= ASD_FP32 ? cfg : 0].group;
 
     if (prec == ASD_FP64 || prec == ASD_FP32) {
@@ -889,10 +1095,9 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ga.ld = ldi;
         ga.Kpad = (int)Kpad;
         ASD_TRY(mark(ctx, 1));
-        if (prec == ASD_FP64)
-            hipLaunchKernelGGL(sqt_gemm_kmajor_kernel<double>, dim3(nblocks), dim3(256), 0, st, ga);
-        else
-            hipLaunchKernelGGL(sqt_gemm_kmajor_kernel<float>, dim3(nblocks), dim3(256), 0, st, ga);
+        GemmKernel k = prec == ASD_FP64 ? sqt_gemm_kmajor_kernel<double> : sqt_gemm_kmajor_kernel<float>;
+        const int nblocks = resident_grid(k, 256, num_slots((long)nt_tiles * nt_tiles, nlags), ctx->device);
+        hipLaunchKernelGGL(k, dim3(nblocks), dim3(256), 0, st, ga);
     } else {
         const long Kpad = round_up(K, 64);  // an even number of 32-wide k steps
         const long ldk = round_up(3L * (nlags - 1) + Kpad + 8, 8);  // halves per i-row
@@ -916,9 +1121,9 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ga.Kpad = (int)Kpad;
         ASD_TRY(mark(ctx, 1));
         if (prec == ASD_BF16X3)
-            launch_split<ASD_BF16X3>(kSplitCfgs[cfg], ga, st);
+            launch_split<ASD_BF16X3>(kSplitCfgs[cfg], ga, st, ctx->device);
         else
-            launch_split<ASD_FP16X2>(kSplitCfgs[cfg], ga, st);
+            launch_split<ASD_FP16X2>(kSplitCfgs[cfg], ga, st, ctx->device);
     }
     ASD_HIP(hipGetLastError());
     ASD_TRY(mark(ctx, 2));

@@ -593,7 +593,10 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
 
     const int nti = (args.Ns + BM - 1) / BM, ntj = (args.Ns + BN - 1) / BN;
     const long nslots = num_slots((long)nti * ntj, args.nlags);
-    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int tid = threadIdx.x, lane = tid & 63;
+    // wave-uniform by construction; readfirstlane lets the compiler keep it (and the
+    // LDS bases of the loads) in scalar registers
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wi = wave / WVN, wj = wave % WVN;
     // load side: lane l of a chunk fills slot (row l / 4, quarter l % 4), so it fetches
     // global quarter (l % 4) ^ (l / 16) of that row (rows of a chunk start at a multiple
@@ -674,35 +677,53 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
                 asm volatile("" ::: "memory");
                 issue(min(k + NS - 1, nk - 1), (h + NS - 1) % NS);
                 const uint16_t *sb = lds + h * STAGE;
+                // the reads of the second 16-deep half overlap the MFMAs of the first
+                constexpr int KH = BK / 16;
+                v_t ah[KH][WM], al[KH][WM], bh[KH][WN], bl[KH][WN];
 #pragma unroll
-                for (int kk = 0; kk < BK / 16; kk++) {
+                for (int kk = 0; kk < KH; kk++) {
                     const int qo = 8 * ((2 * kk + rhalf) ^ rx);
-                    v_t ah[WM], al[WM], bh[WN], bl[WN];
 #pragma unroll
                     for (int m = 0; m < WM; m++) {
                         const int r = wi * WM * 32 + m * 32 + rrow;
-                        ah[m] = *reinterpret_cast<const v_t *>(sb + r * BK + qo);
-                        al[m] = *reinterpret_cast<const v_t *>(sb + OFF_AL + r * BK + qo);
+                        ah[kk][m] = *reinterpret_cast<const v_t *>(sb + r * BK + qo);
+                        al[kk][m] = *reinterpret_cast<const v_t *>(sb + OFF_AL + r * BK + qo);
                     }
 #pragma unroll
                     for (int n = 0; n < WN; n++) {
                         const int r = wj * WN * 32 + n * 32 + rrow;
-                        bh[n] = *reinterpret_cast<const v_t *>(sb + OFF_BH + r * BK + qo);
-                        bl[n] = *reinterpret_cast<const v_t *>(sb + OFF_BL + r * BK + qo);
+                        bh[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BH + r * BK + qo);
+                        bl[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BL + r * BK + qo);
                     }
+                }
+#pragma unroll
+                for (int kk = 0; kk < KH; kk++)
 #pragma unroll
                     for (int m = 0; m < WM; m++)
 #pragma unroll
                         for (int n = 0; n < WN; n++) {
-                            acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bh[n], acc[m][n]);
+                            acc[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bh[kk][n], acc[m][n]);
                             if constexpr (DUAL) {
-                                acc2[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc2[m][n]);
-                                acc2[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc2[m][n]);
+                                acc2[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bl[kk][n], acc2[m][n]);
+                                acc2[m][n] = SplitMfma<PREC>::mfma(al[kk][m], bh[kk][n], acc2[m][n]);
                             } else {
-                                acc[m][n] = SplitMfma<PREC>::mfma(ah[m], bl[n], acc[m][n]);
-                                acc[m][n] = SplitMfma<PREC>::mfma(al[m], bh[n], acc[m][n]);
+                                acc[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bl[kk][n], acc[m][n]);
+                                acc[m][n] = SplitMfma<PREC>::mfma(al[kk][m], bh[kk][n], acc[m][n]);
                             }
                         }
+                // issue order: the first half's R reads, then one MFMA and one read of
+                // the second half in turn, then the remaining MFMAs (at most R + 1 reads
+                // in flight, so the LDS counter can wait for exactly the ones needed)
+                {
+                    constexpr int R = 2 * (WM + WN), M = 3 * WM * WN;
+                    static_assert(M >= R, "interleave needs at least R MFMAs per half");
+                    __builtin_amdgcn_sched_group_barrier(0x100, R, 0);
+#pragma unroll
+                    for (int i = 0; i < R; i++) {
+                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                    }
+                    __builtin_amdgcn_sched_group_barrier(0x008, 2 * M - R, 0);
                 }
                 asm volatile("" ::: "memory");
             }

@@ -22,6 +22,7 @@ timed on this host's cores on a bounded sample of the same workload.
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -300,6 +301,23 @@ def workload_rk4sweep(args):
     print(json.dumps({"workload": "rk4 sweep", "rows": rows}), flush=True)
 
 
+def profiled_traffic(cfg_name, precision, L, T0, nlags):
+    """HBM bytes per GEMM launch (FETCH_SIZE + WRITE_SIZE, scaled by the plain-copy calibration) from the
+    committed rocprofv3 summary, when that summary was taken with this configuration on this workload;
+    None otherwise (the counters need a profiler run of their own)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                        "sqt_gemm_dma_fp16x2_L128_hbm_traffic.md")
+    try:
+        with open(path) as f:
+            kv = dict(re.findall(r"^- (\w+): (.+)$", f.read(), re.M))
+    except OSError:
+        return None
+    if (kv.get("config"), kv.get("precision"), kv.get("workload")) != (cfg_name, precision,
+                                                                       f"L={L} T0={T0} nlags={nlags}"):
+        return None
+    return int(kv["traffic_bytes_per_launch"])
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -401,9 +419,10 @@ def main():
                    "gemm_precision": args.precision, "rk4_precision": "fp64"},
         "roofline": {"bound": "mfma", "achieved": round(gemm_tflops, 3), "peak": PEAK_TFLOPS[args.precision],
                      "unit": "TFLOP/s", "frac": round(gemm_tflops / PEAK_TFLOPS[args.precision], 4),
-                     "traffic": None,
-                     "kernel": f"sqt_gemm_split_kernel<{args.precision}> (all {nlags} lags in one launch)",
-                     "mfma_products_per_flop": 3},
+                     "traffic": profiled_traffic(tuned[args.precision]["chosen"], args.precision, L, T0, nlags),
+                     "kernel": f"S(q,t) GEMM, {args.precision}, {tuned[args.precision]['chosen']} "
+                               f"(all {nlags} lags in one launch)",
+                     "mfma_products_per_flop": 3 if args.precision in ("fp16x2", "bf16x3") else 1},
         "stage_ms": {k: round(v, 3) for k, v in avg.items()},
         "autotune": tuned,
         "rk4_spin_steps_per_s": rk_rate,

@@ -1117,7 +1117,9 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ga.Kpad = (int)Kpad;
         ASD_TRY(mark(ctx, 1));
         GemmKernel k = prec == ASD_FP64 ? sqt_gemm_kmajor_kernel<double> : sqt_gemm_kmajor_kernel<float>;
-        const int nblocks = resident_grid(k, 256, num_slots((long)nt_tiles * nt_tiles, nlags), ctx->device);
+        // a capped grid of many more blocks than fit at once measured 2-4% faster than the
+        // persistent grid for this kernel (FP32 / FP64 at L = 256)
+        const int nblocks = (int)std::min(num_slots((long)nt_tiles * nt_tiles, nlags), kMaxGrid);
         hipLaunchKernelGGL(k, dim3(nblocks), dim3(256), 0, st, ga);
     } else {
         const long Kpad = round_up(K, 64);  // an even number of 32-wide k steps

@@ -100,8 +100,9 @@ struct Work {
     int lag, ti, tj;
 };
 
-// Work slots are processed by a persistent grid (as many blocks as are resident at
-// once, a multiple of 8) in a grid-stride loop.  Blocks are dispatched round-robin
+// Work slots are processed in a grid-stride loop by a grid that is a multiple of 8: a
+// persistent one (as many blocks as are resident at once) for the split-precision
+// kernels, a capped one for the k-major kernels.  Blocks are dispatched round-robin
 // over the 8 XCDs (block b -> XCD b % 8), so slot s always runs on XCD s % 8.  XCD x
 // takes the x-th eighth of every lag's tiles and all XCDs walk the lags in step: the
 // operands in flight are one lag's A window plus B, which the shared last-level cache

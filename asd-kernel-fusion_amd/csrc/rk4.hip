@@ -11,8 +11,11 @@
 //
 // This file is compiled with -ffp-contract=off and keeps the reference's operation
 // order (field accumulated neighbour by neighbour from 0.0, J multiplied per term,
-// v3d_cross operand order, the k/6 + k/3 + k/3 + k/6 left-to-right sum, true
-// divisions), so trajectories match the FP64 CPU restatement bit for bit.
+// v3d_cross operand order, the k/6 + k/3 + k/3 + k/6 left-to-right sum, correctly
+// rounded divisions), so trajectories match the FP64 CPU restatement bit for bit.
+// Two exact shortcuts apply when the parameters allow: J == 1 drops the multiplies
+// by J (bit for bit), and alpha == 0 drops the damping term, which leaves every value
+// equal (IEEE ==) and can only turn an exact -0 into +0.
 #include "asd_internal.h"
 
 namespace asd {
@@ -34,27 +37,71 @@ LLGParams make_llg(const asd_params &p)
 
 // f(S) at one site given its four neighbours in the reference order
 // (row+1, row-1, col-1, col+1; llg_sqt_fusion.c:181-184).
+// UNIT_J (J == 1): n * J is n exactly, so the multiplies are dropped.
+// UNDAMPED (alpha == 0): (p + 0 q) * 1 is p, so the damping term is dropped; the value
+// is the same (IEEE ==), only an exact -0 in p may come out as +0 the other way.
+template <bool UNIT_J = false, bool UNDAMPED = false>
 __device__ __forceinline__ void llg_f(const LLGParams &p, double sx, double sy, double sz,
                                       const double *n0, const double *n1, const double *n2,
                                       const double *n3, double &fx, double &fy, double &fz)
 {
     double hx = 0.0, hy = 0.0, hz = 0.0;
-    hx += n0[0] * p.J; hy += n0[1] * p.J; hz += n0[2] * p.J;
-    hx += n1[0] * p.J; hy += n1[1] * p.J; hz += n1[2] * p.J;
-    hx += n2[0] * p.J; hy += n2[1] * p.J; hz += n2[2] * p.J;
-    hx += n3[0] * p.J; hy += n3[1] * p.J; hz += n3[2] * p.J;
+    if constexpr (UNIT_J) {
+        hx += n0[0]; hy += n0[1]; hz += n0[2];
+        hx += n1[0]; hy += n1[1]; hz += n1[2];
+        hx += n2[0]; hy += n2[1]; hz += n2[2];
+        hx += n3[0]; hy += n3[1]; hz += n3[2];
+    } else {
+        hx += n0[0] * p.J; hy += n0[1] * p.J; hz += n0[2] * p.J;
+        hx += n1[0] * p.J; hy += n1[1] * p.J; hz += n1[2] * p.J;
+        hx += n2[0] * p.J; hy += n2[1] * p.J; hz += n2[2] * p.J;
+        hx += n3[0] * p.J; hy += n3[1] * p.J; hz += n3[2] * p.J;
+    }
     hz += 2.0 * p.D * sz;
     hz += p.Bz;
     const double px = hy * sz - hz * sy;
     const double py = hz * sx - hx * sz;
     const double pz = hx * sy - hy * sx;
-    const double qx = sy * pz - sz * py;
-    const double qy = sz * px - sx * pz;
-    const double qz = sx * py - sy * px;
-    fx = (px + p.a_over_len * qx) * p.inv_den;
-    fy = (py + p.a_over_len * qy) * p.inv_den;
-    fz = (pz + p.a_over_len * qz) * p.inv_den;
+    if constexpr (UNDAMPED) {
+        fx = px; fy = py; fz = pz;
+    } else {
+        const double qx = sy * pz - sz * py;
+        const double qy = sz * px - sx * pz;
+        const double qz = sx * py - sy * px;
+        fx = (px + p.a_over_len * qx) * p.inv_den;
+        fy = (py + p.a_over_len * qy) * p.inv_den;
+        fz = (pz + p.a_over_len * qz) * p.inv_den;
+    }
 }
+
+// a + k / d, with k / d correctly rounded by the FMA correction of Markstein's theorem
+// (z = RN(1/d) within half an ulp, q = RN(k z) within one ulp of k/d, r = k - d q exact,
+// so RN(q + r z) = RN(k/d)).  Written as -(fma(-r, z, -q)), whose value is RN(q + r z)
+// for k != 0 and which keeps the sign of k = +-0; the negation folds into the add.
+// Three instructions instead of the IEEE division sequence; same bits as k / d
+// (checked against division on 2.2e9 random and near-multiple operands).
+__device__ __forceinline__ double add_div(double a, double k, double d, double z)
+{
+    const double q = k * z;
+    const double r = __builtin_fma(-q, d, k);
+    return a - __builtin_fma(-r, z, -q);
+}
+
+// (len * a) / nrm for the three components with one IEEE division (y = 1 / nrm) and the
+// same correction per component: bit-identical to three divisions.
+__device__ __forceinline__ void renorm(double len, double ax, double ay, double az, double &ox, double &oy,
+                                       double &oz)
+{
+    const double nrm = sqrt(ax * ax + ay * ay + az * az);
+    const double y = 1.0 / nrm;
+    auto one = [&](double a) {
+        const double n = len * a, q = n * y;
+        const double r = __builtin_fma(-q, nrm, n);
+        return -__builtin_fma(-r, y, -q);
+    };
+    ox = one(ax); oy = one(ay); oz = one(az);
+}
+constexpr double kSixth = 1.0 / 6.0, kThird = 1.0 / 3.0;
 
 __device__ __forceinline__ int wrap(int x, int L)
 {
@@ -74,7 +121,7 @@ template <int TR, int TC> struct TileLDS {
     double S[3][NR], K[3][NR];
 };
 
-template <int TR, int TC>
+template <int TR, int TC, bool UJ = false, bool UD = false>
 __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p, int L, int tile_r, int tile_c,
                                          const double *__restrict__ in, double *__restrict__ out,
                                          double *__restrict__ snap)
@@ -140,7 +187,7 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
                     n[q][0] = Y(0, nb[q]); n[q][1] = Y(1, nb[q]); n[q][2] = Y(2, nb[q]);
                 }
                 double fx, fy, fz;
-                llg_f(p, Y(0, idx), Y(1, idx), Y(2, idx), n[0], n[1], n[2], n[3], fx, fy, fz);
+                llg_f<UJ, UD>(p, Y(0, idx), Y(1, idx), Y(2, idx), n[0], n[1], n[2], n[3], fx, fy, fz);
                 kr[j][0] = fx * p.dt; kr[j][1] = fy * p.dt; kr[j][2] = fz * p.dt;
             }
         }
@@ -156,25 +203,24 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
         __syncthreads();
         const double kx = sK[0][own], ky = sK[1][own], kz = sK[2][own];
         if (stage == 1) {
-            ax = sS[0][own] + kx / 6.0; ay = sS[1][own] + ky / 6.0; az = sS[2][own] + kz / 6.0;
+            ax = add_div(sS[0][own], kx, 6.0, kSixth);
+            ay = add_div(sS[1][own], ky, 6.0, kSixth);
+            az = add_div(sS[2][own], kz, 6.0, kSixth);
         } else if (stage == 4) {
-            ax = ax + kx / 6.0; ay = ay + ky / 6.0; az = az + kz / 6.0;
+            ax = add_div(ax, kx, 6.0, kSixth); ay = add_div(ay, ky, 6.0, kSixth); az = add_div(az, kz, 6.0, kSixth);
         } else {
-            ax = ax + kx / 3.0; ay = ay + ky / 3.0; az = az + kz / 3.0;
+            ax = add_div(ax, kx, 3.0, kThird); ay = add_div(ay, ky, 3.0, kThird); az = add_div(az, kz, 3.0, kThird);
         }
     }
     if (owner) {
-        const double nrm = sqrt(ax * ax + ay * ay + az * az);
         double *o = out + 3 * ((long)grow * L + gcol);
-        o[0] = p.len * ax / nrm;
-        o[1] = p.len * ay / nrm;
-        o[2] = p.len * az / nrm;
+        renorm(p.len, ax, ay, az, o[0], o[1], o[2]);
     }
 }
 
 // single lattice (nrep replicas along blockIdx.z); snapshot slot from *snap_counter
 // when given (graph replay), else slot 0 of snap
-template <int TR, int TC>
+template <int TR, int TC, bool UJ, bool UD>
 __global__ __launch_bounds__(TR *TC) void rk4_step_kernel(LLGParams p, int L,
                                                           const double *__restrict__ in,
                                                           double *__restrict__ out,
@@ -186,7 +232,7 @@ __global__ __launch_bounds__(TR *TC) void rk4_step_kernel(LLGParams p, int L,
     const long base = (long)blockIdx.z * L * L * 3;
     double *sn = nullptr;
     if (snap) sn = snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride + base;
-    rk4_tile<TR, TC>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
+    rk4_tile<TR, TC, UJ, UD>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
 }
 
 // ------------------------------------------------------------------ batching
@@ -208,7 +254,157 @@ __global__ __launch_bounds__(TR *TC) void rk4_batch_kernel(const BatchTraj *__re
         double *out = parity ? tr.a : tr.b;
         double *sn = (write_snap && tr.snaps) ? tr.snaps + (long)*snap_counter * 3L * tr.L * tr.L : nullptr;
         __syncthreads();  // LDS of the previous item fully consumed
-        rk4_tile<TR, TC>(sm, tr.p, tr.L, w.y / ntc, w.y % ntc, in, out, sn);
+        if (tr.p.J == 1.0 && tr.p.alpha == 0.0)
+            rk4_tile<TR, TC, true, true>(sm, tr.p, tr.L, w.y / ntc, w.y % ntc, in, out, sn);
+        else
+            rk4_tile<TR, TC>(sm, tr.p, tr.L, w.y / ntc, w.y % ntc, in, out, sn);
+    }
+}
+
+// ------------------------------------------------- register-streaming step
+// For lattices far larger than the caches.  One wave owns 64 consecutive columns of one
+// band of R rows: lanes 4..59 are output columns, lanes 0..3 and 60..63 a 4-column halo
+// (each RK stage widens the dependency by one site).  The wave streams down the rows:
+// at step t it loads row t and evaluates stage 1 at row t-1, stage 2 at row t-2, stage
+// 3 at row t-3 and stage 4 at row t-4, so each stage finds its row+1 / row-1 inputs in
+// the same lane's registers and its col-1 / col+1 inputs in the neighbouring lanes (DPP
+// whole-wave shifts).  No LDS and no barriers; each site is read once and written once
+// (plus the halo: 64/56 columns, (R+8)/R rows).  Rows are kept in 4-slot rings indexed
+// by the step modulo 4, so the 4-way unrolled loop keeps every slot in a register.
+// Every site sees the same expressions as rk4_tile (llg_f, Y = S + k/2.0 or S + k,
+// k/6.0 + k/3.0 + k/3.0 + k/6.0 left to right, len * a / |a|): bit-identical results.
+constexpr int kStreamHalo = 4, kStreamCols = 64 - 2 * kStreamHalo;
+
+// bound_ctrl: the edge lane (a halo lane) receives 0, so no old value is needed
+__device__ __forceinline__ double lane_from_left(double v)  // lane l receives lane l-1
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x138, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x138, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_from_right(double v)  // lane l receives lane l+1
+{
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), 0x130, 0xf, 0xf, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), 0x130, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+// k = dt f(c) with row+1 = up, row-1 = dn and the column neighbours from the adjacent lanes
+template <bool UJ, bool UD>
+__device__ __forceinline__ void stream_k(const LLGParams &p, const double (&c)[3], const double (&up)[3],
+                                         const double (&dn)[3], double (&k)[3])
+{
+    double lf[3], rt[3];
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        lf[d] = lane_from_left(c[d]);
+        rt[d] = lane_from_right(c[d]);
+    }
+    double fx, fy, fz;
+    llg_f<UJ, UD>(p, c[0], c[1], c[2], up, dn, lf, rt, fx, fy, fz);
+    k[0] = fx * p.dt; k[1] = fy * p.dt; k[2] = fz * p.dt;
+}
+
+// grid: 4 waves per block, one work item (replica, band, strip) per wave; R % 4 == 0,
+// 8 <= R <= L - 8, L >= 64 (checked by the launcher)
+template <bool UJ, bool UD>
+__global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int nrep, int R, int nband,
+                                                         int nstrip, const double *__restrict__ in,
+                                                         double *__restrict__ out, double *__restrict__ snap,
+                                                         const int *__restrict__ snap_counter, long snap_stride)
+{
+    const int lane = threadIdx.x & 63;
+    const long item = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (item >= (long)nrep * nband * nstrip) return;  // whole wave; the kernel has no barriers
+    const int strip = (int)(item % nstrip), band = (int)((item / nstrip) % nband);
+    const int rep = (int)(item / ((long)nstrip * nband));
+    const long ns = (long)L * L;
+    const double *src = in + rep * ns * 3;
+    double *dst = out + rep * ns * 3;
+    double *sn = nullptr;
+    if (snap) sn = snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride + rep * ns * 3;
+
+    const int c = strip * kStreamCols - kStreamHalo + lane;  // in [-4, L + 116)
+    const int gc = ((c % L) + L) % L;
+    const bool ocol = lane >= kStreamHalo && lane < 64 - kStreamHalo && c < L;
+    const int r0 = band * R;                                 // first output row
+    const int rows = min(R, L - r0);                         // output rows of this band
+    auto row_of = [&](int t) {                               // lattice row loaded at step t
+        int y = r0 - kStreamHalo + t;                        // in [-4, L + R + 4) with R <= L - 8
+        y += y < 0 ? L : 0;
+        y -= y >= L ? L : 0;
+        return y;
+    };
+    auto load = [&](int t, double (&v)[3]) {
+        const double *q = src + 3 * ((long)row_of(t) * L + gc);
+        v[0] = q[0]; v[1] = q[1]; v[2] = q[2];
+    };
+
+    double S[4][3], Y2[4][3], Y3[4][3], Y4[4][3], A[4][3], pf[4][3];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            S[a][d] = 0.0; Y2[a][d] = 0.0; Y3[a][d] = 0.0; Y4[a][d] = 0.0; A[a][d] = 0.0; pf[a][d] = 0.0;
+        }
+    const int nstep = rows + 2 * kStreamHalo;  // steps t = 0 .. nstep-1; output row t-8 at step t
+    // prefetch PF rows ahead (3: 186 VGPRs, two waves per SIMD, no scratch; asking
+    // the compiler for three waves per SIMD spills)
+    constexpr int PF = 3;
+    load(0, pf[0]);
+    if (PF > 1 && nstep > 1) load(1, pf[1]);
+    if (PF > 2 && nstep > 2) load(2, pf[2]);
+    for (int t0 = 0; t0 < nstep; t0 += 4) {
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int t = t0 + u;
+            if (t < nstep) {
+                // ring slots of rows t, t-1, .., t-5 (t-4 shares the slot of t)
+                constexpr int s0 = 0;
+                const int i0 = (u + 4 - 0) & 3, i1 = (u + 4 - 1) & 3, i2 = (u + 4 - 2) & 3, i3 = (u + 4 - 3) & 3;
+                const int i4 = i0, i5 = i1;
+                (void)s0;
+#pragma unroll
+                for (int d = 0; d < 3; d++) S[i0][d] = pf[i0][d];
+                if (t + PF < nstep) load(t + PF, pf[(u + PF) & 3]);
+                const int tr = t - kStreamHalo;  // band-relative row of the state loaded now
+                if (sn && ocol && tr >= 0 && tr < rows) {
+                    double *o = sn + 3 * ((long)(r0 + tr) * L + gc);
+                    o[0] = S[i0][0]; o[1] = S[i0][1]; o[2] = S[i0][2];
+                }
+                double k[3];
+                // stage 1 at row t-1: Y1 = S
+                stream_k<UJ, UD>(p, S[i1], S[i0], S[i2], k);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    A[i1][d] = add_div(S[i1][d], k[d], 6.0, kSixth);
+                    Y2[i1][d] = S[i1][d] + k[d] / 2.0;
+                }
+                // stage 2 at row t-2
+                stream_k<UJ, UD>(p, Y2[i2], Y2[i1], Y2[i3], k);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    A[i2][d] = add_div(A[i2][d], k[d], 3.0, kThird);
+                    Y3[i2][d] = S[i2][d] + k[d] / 2.0;
+                }
+                // stage 3 at row t-3
+                stream_k<UJ, UD>(p, Y3[i3], Y3[i2], Y3[i4], k);
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    A[i3][d] = add_div(A[i3][d], k[d], 3.0, kThird);
+                    Y4[i3][d] = S[i3][d] + k[d];
+                }
+                // stage 4 at row t-4, then the new state of that row
+                stream_k<UJ, UD>(p, Y4[i4], Y4[i3], Y4[i5], k);
+                const int orow = t - 2 * kStreamHalo;  // band-relative output row
+#pragma unroll
+                for (int d = 0; d < 3; d++) A[i4][d] = add_div(A[i4][d], k[d], 6.0, kSixth);
+                if (ocol && orow >= 0) {
+                    double *o = dst + 3 * ((long)(r0 + orow) * L + gc);
+                    renorm(p.len, A[i4][0], A[i4][1], A[i4][2], o[0], o[1], o[2]);
+                }
+            }
+        }
     }
 }
 
@@ -329,22 +525,43 @@ int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in, double *out,
                     double *snap_out, const int *snap_counter, long snap_stride, int bump_counter)
 {
-    // 16 x 16 tiles (256 threads, 27.6 KB LDS, 5 workgroups per CU; 2.25 sites loaded
-    // and 5.7 f-evaluations per output site) are the default: measured faster at every
-    // size than 32 x 32 (1024 threads, 76.8 KB, 1.56 loads and 4.8 f-evaluations per
-    // site), which the 128-register cap of 1024-thread workgroups limits to one
-    // workgroup per CU.  32 x 32 stays selectable (asd_rk4_set_tile).
+    // LDS tiles: 16 x 16 (256 threads, 27.6 KB LDS, 5 workgroups per CU; 2.25 sites
+    // loaded and 5.7 f-evaluations per output site) measured faster at every size than
+    // 32 x 32 (1024 threads, 76.8 KB, 1.56 loads and 4.8 f-evaluations per site), which
+    // the 128-register cap of 1024-thread workgroups limits to one workgroup per CU.
+    // 32 x 32 stays selectable (asd_rk4_set_tile).
     const bool big = ctx->rk4_tile == 32;
-    if (big) {
+    // J == 1 and alpha == 0 (the plain workloads) take the specialised arithmetic
+    const bool uj = p.J == 1.0, ud = p.alpha == 0.0;
+    // streaming strips: forced (64), or automatic once there are at least four waves per
+    // CU (each wave's rows form one serial chain, so small lattices are faster in tiles;
+    // measured: L = 1024 x 8 replicas 205 us vs 275 us per step, x 1 replica 114 vs 41)
+    const int R = std::min(120, L - 8);
+    const int nband = L >= 64 ? (L + R - 1) / R : 0, nstrip = (L + kStreamCols - 1) / kStreamCols;
+    const long waves = (long)nrep * nband * nstrip;
+    const bool stream = L >= 64 && (ctx->rk4_tile == 64 || (ctx->rk4_tile == 0 && waves >= 4L * ctx->num_cu));
+    if (stream) {
+        auto k = uj && ud ? rk4_stream_kernel<true, true>
+                          : (uj ? rk4_stream_kernel<true, false>
+                                : (ud ? rk4_stream_kernel<false, true> : rk4_stream_kernel<false, false>));
+        hipLaunchKernelGGL(k, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, p, L, nrep, R, nband,
+                           nstrip, in, out, snap_out, snap_counter, snap_stride);
+    } else if (big) {
         constexpr int TR = 32, TC = 32;
         const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);
-        hipLaunchKernelGGL((rk4_step_kernel<TR, TC>), grid, dim3(TR * TC), 0, ctx->stream, p, L, in, out,
-                           snap_out, snap_counter, snap_stride);
+        auto k = uj && ud ? rk4_step_kernel<TR, TC, true, true>
+                          : (uj ? rk4_step_kernel<TR, TC, true, false>
+                                : (ud ? rk4_step_kernel<TR, TC, false, true> : rk4_step_kernel<TR, TC, false, false>));
+        hipLaunchKernelGGL(k, grid, dim3(TR * TC), 0, ctx->stream, p, L, in, out, snap_out, snap_counter,
+                           snap_stride);
     } else {
         constexpr int TR = 16, TC = 16;
         const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);
-        hipLaunchKernelGGL((rk4_step_kernel<TR, TC>), grid, dim3(TR * TC), 0, ctx->stream, p, L, in, out,
-                           snap_out, snap_counter, snap_stride);
+        auto k = uj && ud ? rk4_step_kernel<TR, TC, true, true>
+                          : (uj ? rk4_step_kernel<TR, TC, true, false>
+                                : (ud ? rk4_step_kernel<TR, TC, false, true> : rk4_step_kernel<TR, TC, false, false>));
+        hipLaunchKernelGGL(k, grid, dim3(TR * TC), 0, ctx->stream, p, L, in, out, snap_out, snap_counter,
+                           snap_stride);
     }
     if (bump_counter)
         hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(1), 0, ctx->stream, (int *)snap_counter);

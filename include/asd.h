@@ -101,8 +101,11 @@ int asd_memcpy(asd_ctx *ctx, void *dst, const void *src, size_t bytes, int kind)
 int asd_rk4_run(asd_ctx *ctx, const asd_params *p, int L, int nrep, double *d_spins,
                 long nsteps, int snap_every, int nsnap, double *d_snaps, int use_graph);
 
-/* RK4 tile edge for asd_rk4_run: 0 = automatic (16 x 16, measured fastest at all
- * sizes), or force 16 or 32.  Results do not depend on it (bit for bit). */
+/* RK4 kernel for asd_rk4_run: 0 = automatic (16 x 16 LDS tiles, or the streaming
+ * strips once nrep * ceil(L / 120) * ceil(L / 56) >= 4 waves per CU), 16 or 32 = LDS
+ * tiles of that edge, 64 = register-streaming column strips (one 64-lane wave per 56
+ * columns of a band of rows; lattices with L < 64 use 16 x 16 tiles).  Results do not
+ * depend on the choice (bit for bit). */
 int asd_rk4_set_tile(asd_ctx *ctx, int tile);
 
 /* ---------------------------------------- stage 2: S(q,tau) correlation --- */

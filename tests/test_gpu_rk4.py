@@ -100,11 +100,70 @@ def test_tile_edge_does_not_change_results(gctx, orc):
     S0 = torch.tensor(orc.init_spins(8, L * L), device="cuda")
     p = asd_amd.params(D=0.1, Bz=0.05, alpha=0.02, dt=0.01)
     out = []
-    for tile in (16, 32, 0):
+    for tile in (16, 32, 0, 64):
         gctx.rk4_set_tile(tile)
         s = S0.clone()
         gctx.rk4_run(p, L, s, 30, 10, 0, None, use_graph=False)
         gctx.synchronize()
         out.append(s.cpu().numpy())
     gctx.rk4_set_tile(0)
+    assert all(np.array_equal(out[0], o) for o in out[1:])
+
+
+STREAM_CASES = [
+    # L, nrep, params: strips of 56 columns and bands of min(120, L - 8) rows, ragged at
+    # both ends; L = 48 is below the 64-column wave and takes the tile kernel
+    (64, 2, dict(D=0.1, Bz=0.1, alpha=0.01)),
+    (100, 1, dict(D=0.1, Bz=0.05)),
+    (130, 3, dict(J=-0.7, D=0.05, Bz=0.2, alpha=0.2, spin_length=1.5)),
+    (257, 1, dict(D=0.1, alpha=0.05)),
+    (128, 1, dict(D=0.1)),
+    (48, 1, dict(D=0.1)),
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,nrep,kw", STREAM_CASES)
+def test_streaming_kernel_bit_exact(gctx, orc, L, nrep, kw):
+    """The register-streaming step (asd_rk4_set_tile 64) gives the oracle's trajectory and
+    snapshots bit for bit."""
+    torch = need_gpu()
+    import asd_amd
+    say(f"rk4 streaming L={L} nrep={nrep} {kw}")
+    S0 = np.stack([orc.init_spins(11 + r, L * L) for r in range(nrep)]) * kw.get("spin_length", 1.0)
+    nsteps, every, nsnap = 40, 10, 5
+    S_ref, snaps_ref = orc.run(orc.params(dt=0.01, **kw), L, S0, nsteps, every, nsnap)
+    spins = torch.tensor(S0, device="cuda")
+    snaps = torch.zeros((nsnap,) + S0.shape, dtype=torch.float64, device="cuda")
+    gctx.rk4_set_tile(64)
+    try:
+        gctx.rk4_run(asd_amd.params(dt=0.01, **kw), L, spins, nsteps, every, nsnap, snaps, use_graph=False)
+        gctx.synchronize()
+    finally:
+        gctx.rk4_set_tile(0)
+    S_gpu = spins.cpu().numpy()
+    say(f"  max |S_gpu - S_oracle| = {np.abs(S_gpu - S_ref).max():.3e}")
+    assert np.array_equal(S_gpu, S_ref)
+    assert np.array_equal(snaps.cpu().numpy(), snaps_ref)
+
+
+@pytest.mark.gpu
+def test_automatic_choice_takes_streaming_for_many_replicas(gctx, orc):
+    """At L = 128 x 180 replicas (1080 waves) the automatic choice runs the streaming
+    strips; the result equals both forced kernels bit for bit."""
+    torch = need_gpu()
+    import asd_amd
+    L, nrep = 128, 180
+    S0 = torch.tensor(np.stack([orc.init_spins(40 + r % 7, L * L) for r in range(nrep)]), device="cuda")
+    p = asd_amd.params(D=0.1, dt=0.01)
+    out = []
+    try:
+        for tile in (0, 16, 64):
+            gctx.rk4_set_tile(tile)
+            s = S0.clone()
+            gctx.rk4_run(p, L, s, 20, 10, 0, None, use_graph=False)
+            gctx.synchronize()
+            out.append(s.cpu().numpy())
+    finally:
+        gctx.rk4_set_tile(0)
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])

@@ -335,6 +335,8 @@ def main():
     ap.add_argument("--workload", default="plain128",
                     choices=["plain128", "precision256", "batch96", "adjoint64", "rk4sweep"],
                     help="plain128 is the benchmark line; the others report their own JSON")
+    ap.add_argument("--no-rk4-large", action="store_true",
+                    help="skip the RK4 measurement on the 1024 x 1024 x 64-replica batch")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 1 GiB-in / 1 GiB-out elementwise copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -451,6 +453,25 @@ def main():
         res["err_vs_fp64"] = {"max": float(e.max()), "rms": float(np.sqrt((e ** 2).mean())),
                               "floor": "relative to max_q |S(q,tau)| of each lag"}
         log(f"  fp64 pass: {st64}")
+
+    # the same RK4 step on a lattice batch far larger than the caches (L = 1024, 64
+    # replicas: 1.6 GB of state), the HBM-bound regime; the automatic kernel choice
+    # takes the register-streaming strips here
+    if not args.no_rk4_large:
+        Lb, nb, steps_b = 1024, 64, 30
+        Sb = torch.tensor(asd_amd.init_spins(5, Lb * Lb), device="cuda").repeat(nb, 1)
+        ctx.rk4_run(prm, Lb, Sb, 2)
+        torch.cuda.synchronize()
+        ctx.rk4_run(prm, Lb, Sb, steps_b)
+        ms_b = ctx.stage_times()["rk4"]
+        rate_b = nb * Lb * Lb * steps_b / (ms_b * 1e-3)
+        res["rk4_bandwidth_bound"] = {
+            "L": Lb, "replicas": nb, "sites": nb * Lb * Lb, "steps": steps_b,
+            "us_per_step": round(ms_b * 1e3 / steps_b, 2), "spin_steps_per_s": rate_b,
+            "algorithmic_GBps": round(48.0 * rate_b / 1e9, 1),
+            "hbm_frac": round(48.0 * rate_b / (HBM_PEAK_GBS * 1e9), 4)}
+        del Sb
+        log(f"  rk4 bandwidth-bound: {res['rk4_bandwidth_bound']}")
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)

@@ -301,6 +301,36 @@ def workload_rk4sweep(args):
     print(json.dumps({"workload": "rk4 sweep", "rows": rows}), flush=True)
 
 
+def workload_rk4large(args):
+    """The RK4 step alone on the HBM-bound batch (L = 1024, 64 replicas, 1.6 GB of state;
+    automatic kernel choice), for profiling its traffic."""
+    import torch
+
+    import asd_amd
+    ctx = asd_amd.Context(0)
+    ctx.set_timing(True)
+    p = asd_amd.params(J=1.0, D=0.1, dt=0.01)
+    L, nrep, steps = 1024, 64, 30
+    S = torch.tensor(asd_amd.init_spins(5, L * L), device="cuda").repeat(nrep, 1)
+    ctx.rk4_run(p, L, S, 2)
+    torch.cuda.synchronize()
+    ctx.rk4_run(p, L, S, steps)
+    ms = ctx.stage_times()["rk4"]
+    rate = nrep * L * L * steps / (ms * 1e-3)
+    res = {"workload": "rk4 large", "L": L, "replicas": nrep, "steps": steps,
+           "us_per_step": round(ms * 1e3 / steps, 2), "spin_steps_per_s": rate,
+           "algorithmic_GBps": round(48.0 * rate / 1e9, 1), "hbm_frac": round(48.0 * rate / (HBM_PEAK_GBS * 1e9), 4)}
+    if args.calib_copy:
+        del S
+        x = torch.ones(1 << 27, dtype=torch.float64, device="cuda")      # 1 GiB
+        y = torch.empty_like(x)
+        for _ in range(2):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        res["calib_copy_bytes"] = {"read": x.numel() * 8, "write": y.numel() * 8}
+    print(json.dumps(res), flush=True)
+
+
 def profiled_traffic(cfg_name, precision, L, T0, nlags):
     """HBM bytes per GEMM launch (FETCH_SIZE + WRITE_SIZE, scaled by the plain-copy calibration) from the
     committed rocprofv3 summary, when that summary was taken with this configuration on this workload;
@@ -333,7 +363,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="plain128",
-                    choices=["plain128", "precision256", "batch96", "adjoint64", "rk4sweep"],
+                    choices=["plain128", "precision256", "batch96", "adjoint64", "rk4sweep", "rk4large"],
                     help="plain128 is the benchmark line; the others report their own JSON")
     ap.add_argument("--no-rk4-large", action="store_true",
                     help="skip the RK4 measurement on the 1024 x 1024 x 64-replica batch")
@@ -348,7 +378,7 @@ def main():
 
     if args.workload != "plain128":
         {"precision256": workload_precision256, "batch96": workload_batch96, "adjoint64": workload_adjoint64,
-         "rk4sweep": workload_rk4sweep}[args.workload](args)
+         "rk4sweep": workload_rk4sweep, "rk4large": workload_rk4large}[args.workload](args)
         return
 
     import numpy as np

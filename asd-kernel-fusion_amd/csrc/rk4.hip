@@ -340,16 +340,23 @@ __global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int
         v[0] = q[0]; v[1] = q[1]; v[2] = q[2];
     };
 
-    double S[4][3], Y2[4][3], Y3[4][3], Y4[4][3], A[4][3], pf[4][3];
+    double S[4][3], Y2[4][3], Y3[4][3], Y4[4][3], pf[4][3];
+    // the partial sums S + k1/6 + k2/3 + k3/3 of the four rows in flight live in LDS
+    // (each lane only touches its own doubles: no barriers), which keeps the register
+    // file at three waves per SIMD
+    __shared__ double sA[4][4][3][64];  // [wave][ring slot][component][lane]
+    double(&A)[4][3][64] = sA[threadIdx.x >> 6];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            S[a][d] = 0.0; Y2[a][d] = 0.0; Y3[a][d] = 0.0; Y4[a][d] = 0.0; A[a][d] = 0.0; pf[a][d] = 0.0;
+            S[a][d] = 0.0; Y2[a][d] = 0.0; Y3[a][d] = 0.0; Y4[a][d] = 0.0; pf[a][d] = 0.0;
+            A[a][d][lane] = 0.0;
         }
     const int nstep = rows + 2 * kStreamHalo;  // steps t = 0 .. nstep-1; output row t-8 at step t
-    // prefetch PF rows ahead (3: 186 VGPRs, two waves per SIMD, no scratch; asking
-    // the compiler for three waves per SIMD spills)
+    // prefetch PF rows ahead (with the partial sums in LDS: 158 VGPRs, three waves per
+    // SIMD, no scratch; with them in registers 186 VGPRs and two waves: 8% slower at
+    // 1024 x 1024 x 64)
     constexpr int PF = 3;
     load(0, pf[0]);
     if (PF > 1 && nstep > 1) load(1, pf[1]);
@@ -377,31 +384,31 @@ __global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int
                 stream_k<UJ, UD>(p, S[i1], S[i0], S[i2], k);
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    A[i1][d] = add_div(S[i1][d], k[d], 6.0, kSixth);
+                    A[i1][d][lane] = add_div(S[i1][d], k[d], 6.0, kSixth);
                     Y2[i1][d] = S[i1][d] + k[d] / 2.0;
                 }
                 // stage 2 at row t-2
                 stream_k<UJ, UD>(p, Y2[i2], Y2[i1], Y2[i3], k);
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    A[i2][d] = add_div(A[i2][d], k[d], 3.0, kThird);
+                    A[i2][d][lane] = add_div(A[i2][d][lane], k[d], 3.0, kThird);
                     Y3[i2][d] = S[i2][d] + k[d] / 2.0;
                 }
                 // stage 3 at row t-3
                 stream_k<UJ, UD>(p, Y3[i3], Y3[i2], Y3[i4], k);
 #pragma unroll
                 for (int d = 0; d < 3; d++) {
-                    A[i3][d] = add_div(A[i3][d], k[d], 3.0, kThird);
+                    A[i3][d][lane] = add_div(A[i3][d][lane], k[d], 3.0, kThird);
                     Y4[i3][d] = S[i3][d] + k[d];
                 }
                 // stage 4 at row t-4, then the new state of that row
                 stream_k<UJ, UD>(p, Y4[i4], Y4[i3], Y4[i5], k);
                 const int orow = t - 2 * kStreamHalo;  // band-relative output row
 #pragma unroll
-                for (int d = 0; d < 3; d++) A[i4][d] = add_div(A[i4][d], k[d], 6.0, kSixth);
+                for (int d = 0; d < 3; d++) A[i4][d][lane] = add_div(A[i4][d][lane], k[d], 6.0, kSixth);
                 if (ocol && orow >= 0) {
                     double *o = dst + 3 * ((long)(r0 + orow) * L + gc);
-                    renorm(p.len, A[i4][0], A[i4][1], A[i4][2], o[0], o[1], o[2]);
+                    renorm(p.len, A[i4][0][lane], A[i4][1][lane], A[i4][2][lane], o[0], o[1], o[2]);
                 }
             }
         }

@@ -354,13 +354,14 @@ __global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int
             A[a][d][lane] = 0.0;
         }
     const int nstep = rows + 2 * kStreamHalo;  // steps t = 0 .. nstep-1; output row t-8 at step t
-    // prefetch PF rows ahead (with the partial sums in LDS: 158 VGPRs, three waves per
-    // SIMD, no scratch; with them in registers 186 VGPRs and two waves: 8% slower at
-    // 1024 x 1024 x 64)
-    constexpr int PF = 3;
+    // prefetch PF rows ahead (with the partial sums in LDS and PF = 4: 164 VGPRs, three
+    // waves per SIMD, no scratch; PF = 3 measured 11% slower at 1024 x 1024 x 64, the
+    // partial sums in registers another 8%)
+    constexpr int PF = 4;
     load(0, pf[0]);
     if (PF > 1 && nstep > 1) load(1, pf[1]);
     if (PF > 2 && nstep > 2) load(2, pf[2]);
+    if (PF > 3 && nstep > 3) load(3, pf[3]);
     for (int t0 = 0; t0 < nstep; t0 += 4) {
 #pragma unroll
         for (int u = 0; u < 4; u++) {

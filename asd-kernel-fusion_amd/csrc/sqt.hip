@@ -752,7 +752,10 @@ static const SqtCfg kSplitCfgs[] = {
     {2, 1, 8, "128x128 tiles, 8 waves of 64x32, group 8", 2, 4},
     {1, 2, 8, "128x128 tiles, 8 waves of 32x64, 4-stage LDS ring, group 8", 4, 2, 4},
     {2, 1, 8, "128x128 tiles, 8 waves of 64x32, 4-stage LDS ring, group 8", 2, 4, 4},
-    {2, 2, 8, "256x128 tiles, 8 waves of 64x64, 3-stage LDS ring, group 8", 4, 2, 3},
+    // 32 tiles run at once per XCD: 4 tile-rows x 8 tile-columns share 4 A panels of 256
+    // rows and 8 B panels of 128 rows (256 KB per k step, 83% L2 hits at best); group 8
+    // (8 x 4) needs 320 KB (79%; 78% measured)
+    {2, 2, 4, "256x128 tiles, 8 waves of 64x64, 3-stage LDS ring, group 4", 4, 2, 3},
     {2, 2, 16, "128x128 tiles, 4 waves of 64x64, group 16"},
     {2, 4, 8, "128x256 tiles, 4 waves of 64x128, group 8"},
     {4, 2, 8, "256x128 tiles, 4 waves of 128x64, group 8"},

@@ -455,6 +455,13 @@ def main():
                      "kernel": f"S(q,t) GEMM, {args.precision}, {tuned[args.precision]['chosen']} "
                                f"(all {nlags} lags in one launch)",
                      "mfma_products_per_flop": 3 if args.precision in ("fp16x2", "bf16x3") else 1},
+        # the reference applies the phase in the GEMM epilogue: 4 flops per G element and q; here the
+        # epilogue bins G by displacement (one FP64 add per element) and the phases act on the L^2
+        # real bins per lag (a multiply-add each for cos and sin: 4 flops per bin and q)
+        "epilogue": {"reference_fused_flops": 4.0 * nq * float(ns) * ns * nlags,
+                     "binning_adds": float(ns) * ns * nlags,
+                     "bin_dft_flops": 4.0 * ns * nq * nlags,
+                     "fourier_ms": round(avg["fourier"], 3), "finish_ms": round(avg["finish"], 3)},
         "stage_ms": {k: round(v, 3) for k, v in avg.items()},
         "autotune": tuned,
         "rk4_spin_steps_per_s": rk_rate,

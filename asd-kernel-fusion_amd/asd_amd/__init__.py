@@ -65,6 +65,8 @@ _SIGS = {
                               C.c_int]),
     "asd_sqt": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int]),
     "asd_sqt_replicas": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp]),
+    "asd_sqt_unfused": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int]),
+    "asd_sqt_bruteforce": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int]),
     "asd_batch_rk4_run": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_long, C.c_int, C.c_int, _vp]),
     "asd_autotune_sqt": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _ip, _dp]),
     "asd_sqt_num_configs": (C.c_int, [C.c_int]),
@@ -266,6 +268,23 @@ class Context:
         _check(lib().asd_sqt(self.h, L, C.c_void_p(snaps.data_ptr()), nsnap, T0, nlags, _np_ptr(q), len(q), prec,
                              _np_ptr(res), 0))
         return res[..., 0] + 1j * res[..., 1]
+
+    def _sqt_other(self, fn, L, snaps, T0, nlags, q):
+        t = self._torch
+        q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
+        assert snaps.dtype == t.float64 and snaps.is_cuda and snaps.is_contiguous()
+        res = np.empty((nlags, len(q), 2))
+        _check(fn(self.h, L, C.c_void_p(snaps.data_ptr()), snaps.shape[0], T0, nlags, _np_ptr(q), len(q),
+                  _np_ptr(res), 0))
+        return res[..., 0] + 1j * res[..., 1]
+
+    def sqt_unfused(self, L: int, snaps, T0: int, nlags: int, q):
+        """The unfused FP64 path (explicit correlation matrix, library GEMMs); as sqt()."""
+        return self._sqt_other(lib().asd_sqt_unfused, L, snaps, T0, nlags, q)
+
+    def sqt_bruteforce(self, L: int, snaps, T0: int, nlags: int, q):
+        """Direct double sum over sites for L <= 16 (independent check); as sqt()."""
+        return self._sqt_other(lib().asd_sqt_bruteforce, L, snaps, T0, nlags, q)
 
     def sqt_replicas(self, L: int, now, zero, q, precision="fp64"):
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)

@@ -150,7 +150,8 @@ int asd_destroy(asd_ctx *c)
     DevBuf *bufs[] = {&c->state_b, &c->counter, &c->opA, &c->opB, &c->opA2, &c->opB2, &c->bins, &c->fq,
                       &c->twiddle, &c->qdev, &c->outdev, &c->stats, &c->c0, &c->c1, &c->c2, &c->c3, &c->c4,
                       &c->c5, &c->c6, &c->work, &c->tabs, &c->batch_b, &c->adj0, &c->adj1, &c->adj2, &c->adj3, &c->adj4,
-                      &c->adj5};
+                      &c->adj5, &c->ref0, &c->ref1, &c->ref2, &c->ref3, &c->ref4, &c->ref5};
+    blas_destroy(c);
     for (DevBuf *b : bufs) b->release();
     delete c;
     return ASD_OK;
@@ -363,6 +364,44 @@ int asd_sqt(asd_ctx *c, int L, const double *d_snaps, int nsnap, int T0, int nla
         ASD_HIP(hipStreamSynchronize(c->stream));
     }
     return ASD_OK;
+}
+
+// the unfused (library GEMM + explicit correlation matrix) and brute-force paths share
+// asd_sqt's arguments, checks and output handling
+static int sqt_other(asd_ctx *c, int L, const double *d_snaps, int nsnap, int T0, int nlags, const int *h_q, int nq,
+                     double *out, int out_on_device, bool brute)
+{
+    ASD_CHECK(c && d_snaps && out, "NULL argument");
+    ASD_TRY(check_L(L));
+    ASD_CHECK(T0 >= 1 && nlags >= 1, "T0 and nlags must be >= 1");
+    ASD_TRY(check_q(L, h_q, nq));
+    ASD_HIP(hipSetDevice(c->device));
+    double *dout = out;
+    if (!out_on_device) {
+        ASD_TRY(c->outdev.ensure(sizeof(double) * 2 * (long)nlags * nq));
+        dout = c->outdev.as<double>();
+    }
+    if (brute)
+        ASD_TRY(sqt_bruteforce_run(c, L, d_snaps, nsnap, T0, nlags, h_q, nq, dout));
+    else
+        ASD_TRY(sqt_unfused_run(c, L, d_snaps, nsnap, T0, nlags, h_q, nq, dout));
+    if (!out_on_device) {
+        ASD_HIP(hipMemcpyAsync(out, dout, sizeof(double) * 2 * (long)nlags * nq, hipMemcpyDeviceToHost, c->stream));
+        ASD_HIP(hipStreamSynchronize(c->stream));
+    }
+    return ASD_OK;
+}
+
+int asd_sqt_unfused(asd_ctx *c, int L, const double *d_snaps, int nsnap, int T0, int nlags, const int *h_q, int nq,
+                    double *out, int out_on_device)
+{
+    return sqt_other(c, L, d_snaps, nsnap, T0, nlags, h_q, nq, out, out_on_device, false);
+}
+
+int asd_sqt_bruteforce(asd_ctx *c, int L, const double *d_snaps, int nsnap, int T0, int nlags, const int *h_q,
+                       int nq, double *out, int out_on_device)
+{
+    return sqt_other(c, L, d_snaps, nsnap, T0, nlags, h_q, nq, out, out_on_device, true);
 }
 
 int asd_sqt_replicas(asd_ctx *c, int L, int nrep, const double *d_now, const double *d_zero, const int *h_q,

@@ -101,6 +101,9 @@ struct asd_ctx {
     hipEvent_t ev[8] = {};
     // adjoint workspace
     asd::DevBuf adj0, adj1, adj2, adj3, adj4, adj5;
+    // unfused / brute-force S(q,tau) paths (sqt_ref.hip): rocBLAS handle and buffers
+    void *blas = nullptr;
+    asd::DevBuf ref0, ref1, ref2, ref3, ref4, ref5;
 };
 
 namespace asd {
@@ -120,6 +123,11 @@ int rhs_launch(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *
 int sqt_num_configs(asd_precision prec);
 const char *sqt_config_name(asd_precision prec, int cfg);
 int sqt_autotune(asd_ctx *ctx, int L, int T0, asd_precision prec, int *best, double *times_ms);
+int sqt_unfused_run(asd_ctx *ctx, int L, const double *d_snaps, int nsnap, int T0, int nlags, const int *h_q,
+                    int nq, double *d_out);
+int sqt_bruteforce_run(asd_ctx *ctx, int L, const double *d_snaps, int nsnap, int T0, int nlags, const int *h_q,
+                       int nq, double *d_out);
+void blas_destroy(asd_ctx *ctx);
 int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_b, long ta_stride,
             int nsnap_a, int T0, int nlags, const int *h_q, int nq, asd_precision prec,
             double *d_out, int cfg);

@@ -331,6 +331,43 @@ def workload_rk4large(args):
     print(json.dumps(res), flush=True)
 
 
+def workload_unfused128(args):
+    """What fusing saves: the L=128 S(q,t) of the benchmark line (400 lags x 400 origins, 192 q)
+    through the unfused FP64 path (library GEMM into an explicit 16384 x 16384 correlation matrix
+    per lag, outer subtract, phase reduction as a second GEMM: the reference's
+    gemm/llg_sqt_mkl.c data flow) beside the fused FP64 and FP16X2 paths, on the same snapshots."""
+    import numpy as np
+    import torch
+
+    import asd_amd
+    L, T0, nlags, every, nsteps = 128, 400, 400, 10, 8000
+    ns, nsnap = L * L, nsteps // every
+    q = asd_amd.qpath_gxmg(L)
+    ctx = asd_amd.Context(0)
+    S = torch.tensor(asd_amd.init_spins(1, ns), device="cuda")
+    snaps = torch.empty((nsnap, ns, 3), dtype=torch.float64, device="cuda")
+    ctx.rk4_run(asd_amd.params(J=1.0, D=0.1, dt=0.01), L, S, nsteps, every, nsnap, snaps)
+    torch.cuda.synchronize()
+    ctx.autotune(L, T0, "fp16x2")
+    res, out = {}, {}
+    for name, fn in (("fused_fp64", lambda: ctx.sqt(L, snaps, T0, nlags, q, "fp64")),
+                     ("fused_fp16x2", lambda: ctx.sqt(L, snaps, T0, nlags, q, "fp16x2")),
+                     ("unfused_fp64", lambda: ctx.sqt_unfused(L, snaps, T0, nlags, q))):
+        fn()  # warm (workspaces, library handle)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        out[name] = fn()
+        torch.cuda.synchronize()
+        res[name + "_s"] = round(time.perf_counter() - t0, 4)
+        log(f"  {name}: {res[name + '_s']} s")
+    scale = np.abs(out["fused_fp64"]).max(axis=1, keepdims=True)
+    res["unfused_vs_fused_fp64_max_rel_err"] = float((np.abs(out["unfused_fp64"] - out["fused_fp64"]) / scale).max())
+    res["fusion_gain_fp64"] = round(res["unfused_fp64_s"] / res["fused_fp64_s"], 3)
+    res["fusion_gain_vs_fp16x2"] = round(res["unfused_fp64_s"] / res["fused_fp16x2_s"], 3)
+    res["unfused_explicit_matrix_GB"] = round(ns * ns * 8 / 1e9, 2)
+    print(json.dumps({"workload": "unfused vs fused S(q,t), L=128, 400 lags x 400 origins, 192 q", **res}), flush=True)
+
+
 def profiled_traffic(cfg_name, precision, L, T0, nlags):
     """HBM bytes per GEMM launch (FETCH_SIZE + WRITE_SIZE, scaled by the plain-copy calibration) from the
     committed rocprofv3 summary, when that summary was taken with this configuration on this workload;
@@ -363,7 +400,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="plain128",
-                    choices=["plain128", "precision256", "batch96", "adjoint64", "rk4sweep", "rk4large"],
+                    choices=["plain128", "precision256", "batch96", "adjoint64", "rk4sweep", "rk4large", "unfused128"],
                     help="plain128 is the benchmark line; the others report their own JSON")
     ap.add_argument("--no-rk4-large", action="store_true",
                     help="skip the RK4 measurement on the 1024 x 1024 x 64-replica batch")
@@ -378,7 +415,8 @@ def main():
 
     if args.workload != "plain128":
         {"precision256": workload_precision256, "batch96": workload_batch96, "adjoint64": workload_adjoint64,
-         "rk4sweep": workload_rk4sweep, "rk4large": workload_rk4large}[args.workload](args)
+         "rk4sweep": workload_rk4sweep, "rk4large": workload_rk4large,
+         "unfused128": workload_unfused128}[args.workload](args)
         return
 
     import numpy as np

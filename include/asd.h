@@ -124,6 +124,23 @@ int asd_rk4_set_tile(asd_ctx *ctx, int tile);
 int asd_sqt(asd_ctx *ctx, int L, const double *d_snaps, int nsnap, int T0, int nlags,
             const int *h_q, int nq, asd_precision prec, double *out, int out_on_device);
 
+/* The reference's unfused S(q,t) data flow (gemm/llg_sqt_mkl.c:369-423) on the GPU,
+ * FP64: per lag G = (1/T0) A_tau B^T by rocBLAS dgemm into an explicit Ns x Ns
+ * buffer, C = G - fbar zbar^T, then sum_ij C_ij e^{-iq.(r_i - r_j)} for every q as a
+ * second dgemm against the per-site cos/sin table plus an FP64 Kahan sum.  Same
+ * arguments, output layout and results (to rounding) as asd_sqt with ASD_FP64; it
+ * materialises Ns^2 doubles (2.1 GB at L = 128) and exists to measure what the fused
+ * path saves. */
+int asd_sqt_unfused(asd_ctx *ctx, int L, const double *d_snaps, int nsnap, int T0, int nlags, const int *h_q,
+                    int nq, double *out, int out_on_device);
+
+/* Brute-force S(q,t) for L <= 16 (baseline/llg_sqt_baseline.c:340-418 with its
+ * disconnected term applied): per (tau, q) the double site sum and the time-origin sum
+ * evaluated directly, phases from FP64 sincos (no lattice tables).  An independent
+ * device-side check of the fused path; same arguments and output layout as asd_sqt. */
+int asd_sqt_bruteforce(asd_ctx *ctx, int L, const double *d_snaps, int nsnap, int T0, int nlags, const int *h_q,
+                       int nq, double *out, int out_on_device);
+
 /* Same contraction for the reference's replica form (llg_sqt_fusion.c:235-268,
  * :370-398): A = nrep replicas at time t, B = the same replicas at time 0, one lag.
  * d_now, d_zero: device [nrep][L*L][3].  out: [nq][2] host doubles. */

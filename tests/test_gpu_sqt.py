@@ -208,3 +208,40 @@ def test_every_kernel_config_matches_oracle(gctx, orc, prec):
                 assert ec <= 1e-12, (L, cfg, name, ec)
     finally:
         gctx.set_config(prec, -1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,T0,nlags,qset", [(16, 20, 8, "full"), (20, 12, 6, "gxmg"), (32, 10, 4, "gxmg")])
+def test_unfused_path_matches_fused_and_oracle(gctx, orc, L, T0, nlags, qset):
+    """asd_sqt_unfused (rocBLAS GEMM into an explicit Ns x Ns matrix, outer subtract,
+    phase reduction as a second GEMM) gives the fused FP64 path's and the oracle's
+    S(q,tau) to 1e-10."""
+    torch = need_gpu()
+    say(f"sqt unfused L={L} T0={T0} nlags={nlags} q={qset}")
+    snaps = trajectory(orc, L, T0 + nlags - 1, D=0.1, Bz=0.05)
+    q = full_grid(L) if qset == "full" else gxmg(L)
+    d = torch.tensor(snaps, device="cuda")
+    ref = orc.sqt_gemm(L, snaps, T0, nlags, q)
+    un = gctx.sqt_unfused(L, d, T0, nlags, q)
+    fu = gctx.sqt(L, d, T0, nlags, q, "fp64")
+    e_o, e_f = max_rel(un, ref), max_rel(un, fu)
+    say(f"  max rel err vs oracle {e_o:.3e}, vs fused {e_f:.3e}")
+    assert e_o <= 1e-10 and e_f <= 1e-10
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,T0,nlags", [(6, 9, 4), (8, 10, 5), (16, 6, 3)])
+def test_bruteforce_matches_oracle_and_fused(gctx, orc, L, T0, nlags):
+    """The direct double sum with sincos phases (no lattice tables, no binning) agrees
+    with the oracle and with the fused path on the full momentum grid."""
+    torch = need_gpu()
+    say(f"sqt brute force L={L} T0={T0} nlags={nlags}")
+    snaps = trajectory(orc, L, T0 + nlags - 1, D=0.1, Bz=0.05, alpha=0.02)
+    q = full_grid(L)
+    d = torch.tensor(snaps, device="cuda")
+    ref = orc.sqt_gemm(L, snaps, T0, nlags, q)
+    bf = gctx.sqt_bruteforce(L, d, T0, nlags, q)
+    fu = gctx.sqt(L, d, T0, nlags, q, "fp64")
+    e_o, e_f = max_rel(bf, ref), max_rel(bf, fu)
+    say(f"  max rel err vs oracle {e_o:.3e}, vs fused {e_f:.3e}")
+    assert e_o <= 1e-10 and e_f <= 1e-10

@@ -232,8 +232,8 @@ int asd_memcpy(asd_ctx *c, void *dst, const void *src, size_t bytes, int kind)
 
 int asd_rk4_set_tile(asd_ctx *c, int tile)
 {
-    ASD_CHECK(c && (tile == 0 || tile == 16 || tile == 32 || tile == 64),
-              "tile must be 0 (automatic), 16, 32 or 64");
+    ASD_CHECK(c && (tile == 0 || tile == 8 || tile == 16 || tile == 32 || tile == 64),
+              "tile must be 0 (automatic), 8, 16, 32 or 64");
     c->rk4_tile = tile;
     if (c->graph.exec) {  // captured launches use the previous tile
         (void)hipGraphExecDestroy(c->graph.exec);

@@ -121,12 +121,15 @@ template <int TR, int TC> struct TileLDS {
     double S[3][NR], K[3][NR];
 };
 
-template <int TR, int TC, bool UJ = false, bool UD = false>
+template <int TR, int TC, bool UJ = false, bool UD = false, int NT = TR * TC>
 __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p, int L, int tile_r, int tile_c,
                                          const double *__restrict__ in, double *__restrict__ out,
                                          double *__restrict__ snap)
 {
-    constexpr int R = 4, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC, NT = TR * TC;
+    // NT threads; threads t < TR * TC own output site t of the tile, all of them share the
+    // stage regions and the tile load
+    constexpr int R = 4, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
+    static_assert(NT >= TR * TC, "one output site per thread at most");
     constexpr int MAXE = ((RR - 2) * (RC - 2) + NT - 1) / NT;  // region sites per thread, stage 1
     double(&sS)[3][NR] = sm.S;
     double(&sK)[3][NR] = sm.K;
@@ -149,9 +152,10 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
         sS[0][idx] = s[0]; sS[1][idx] = s[1]; sS[2][idx] = s[2];
     }
     // this thread's interior site
-    const int ir = t / TC, ic = t % TC;
+    const bool tile_site = t < TR * TC;
+    const int ir = tile_site ? t / TC : 0, ic = tile_site ? t % TC : 0;
     const int grow = tile_r * TR + ir, gcol = tile_c * TC + ic;
-    const bool owner = grow < L && gcol < L;
+    const bool owner = tile_site && grow < L && gcol < L;
     const int own = (ir + R) * RC + (ic + R);
     __syncthreads();
 
@@ -201,6 +205,7 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
             }
         }
         __syncthreads();
+        if (!tile_site) continue;
         const double kx = sK[0][own], ky = sK[1][own], kz = sK[2][own];
         if (stage == 1) {
             ax = add_div(sS[0][own], kx, 6.0, kSixth);
@@ -220,8 +225,8 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
 
 // single lattice (nrep replicas along blockIdx.z); snapshot slot from *snap_counter
 // when given (graph replay), else slot 0 of snap
-template <int TR, int TC, bool UJ, bool UD>
-__global__ __launch_bounds__(TR *TC) void rk4_step_kernel(LLGParams p, int L,
+template <int TR, int TC, bool UJ, bool UD, int NT = TR * TC>
+__global__ __launch_bounds__(NT) void rk4_step_kernel(LLGParams p, int L,
                                                           const double *__restrict__ in,
                                                           double *__restrict__ out,
                                                           double *__restrict__ snap,
@@ -232,7 +237,7 @@ __global__ __launch_bounds__(TR *TC) void rk4_step_kernel(LLGParams p, int L,
     const long base = (long)blockIdx.z * L * L * 3;
     double *sn = nullptr;
     if (snap) sn = snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride + base;
-    rk4_tile<TR, TC, UJ, UD>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
+    rk4_tile<TR, TC, UJ, UD, NT>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
 }
 
 // ------------------------------------------------------------------ batching
@@ -530,6 +535,17 @@ int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items
     return ASD_OK;
 }
 
+template <int TR, int TC, int NT>
+static void launch_tiles(asd_ctx *ctx, const LLGParams &p, bool uj, bool ud, int L, int nrep, const double *in,
+                         double *out, double *snap_out, const int *snap_counter, long snap_stride)
+{
+    const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);
+    auto k = uj && ud ? rk4_step_kernel<TR, TC, true, true, NT>
+                      : (uj ? rk4_step_kernel<TR, TC, true, false, NT>
+                            : (ud ? rk4_step_kernel<TR, TC, false, true, NT> : rk4_step_kernel<TR, TC, false, false, NT>));
+    hipLaunchKernelGGL(k, grid, dim3(NT), 0, ctx->stream, p, L, in, out, snap_out, snap_counter, snap_stride);
+}
+
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in, double *out,
                     double *snap_out, const int *snap_counter, long snap_stride, int bump_counter)
 {
@@ -555,21 +571,21 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
         hipLaunchKernelGGL(k, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, p, L, nrep, R, nband,
                            nstrip, in, out, snap_out, snap_counter, snap_stride);
     } else if (big) {
-        constexpr int TR = 32, TC = 32;
-        const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);
-        auto k = uj && ud ? rk4_step_kernel<TR, TC, true, true>
-                          : (uj ? rk4_step_kernel<TR, TC, true, false>
-                                : (ud ? rk4_step_kernel<TR, TC, false, true> : rk4_step_kernel<TR, TC, false, false>));
-        hipLaunchKernelGGL(k, grid, dim3(TR * TC), 0, ctx->stream, p, L, in, out, snap_out, snap_counter,
-                           snap_stride);
+        launch_tiles<32, 32, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
     } else {
-        constexpr int TR = 16, TC = 16;
-        const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);
-        auto k = uj && ud ? rk4_step_kernel<TR, TC, true, true>
-                          : (uj ? rk4_step_kernel<TR, TC, true, false>
-                                : (ud ? rk4_step_kernel<TR, TC, false, true> : rk4_step_kernel<TR, TC, false, false>));
-        hipLaunchKernelGGL(k, grid, dim3(TR * TC), 0, ctx->stream, p, L, in, out, snap_out, snap_counter,
-                           snap_stride);
+        // tiles by occupancy (measured at L = 128 / 256 / 1024, one lattice: 8 x 8 with 256
+        // threads 4.8 us vs 6.7 us for 16 x 16; 16 x 16 with 1024 threads 6.3 vs 7.2 us;
+        // 16 x 16 with 256 threads 38 us, the others 39-62 us): 8 x 8 tiles (four threads per
+        // output site) while 16 x 16 tiles would not give every CU a workgroup, 16 x 16 with
+        // four threads per site below four workgroups per CU, then 16 x 16 with one
+        const long n16 = (long)nrep * ((L + 15) / 16) * ((L + 15) / 16);
+        const int cus = std::max(ctx->num_cu, 1);
+        if (ctx->rk4_tile == 8 || (ctx->rk4_tile == 0 && n16 < cus))
+            launch_tiles<8, 8, 256>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+        else if (ctx->rk4_tile == 0 && n16 < 4L * cus)
+            launch_tiles<16, 16, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+        else
+            launch_tiles<16, 16, 256>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
     }
     if (bump_counter)
         hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(1), 0, ctx->stream, (int *)snap_counter);

@@ -285,7 +285,7 @@ def workload_rk4sweep(args):
         n = L * L * nrep
         S = torch.tensor(asd_amd.init_spins(5, L * L), device="cuda").repeat(nrep, 1)
         row = {"L": L, "replicas": nrep, "sites": n, "state_MB": round(n * 24 / 1e6, 1)}
-        for tile in (16, 32, 64):
+        for tile in (0, 8, 16, 32, 64):   # 0: the automatic choice
             ctx.rk4_set_tile(tile)
             ctx.rk4_run(p, L, S, 2)
             torch.cuda.synchronize()

@@ -101,8 +101,9 @@ int asd_memcpy(asd_ctx *ctx, void *dst, const void *src, size_t bytes, int kind)
 int asd_rk4_run(asd_ctx *ctx, const asd_params *p, int L, int nrep, double *d_spins,
                 long nsteps, int snap_every, int nsnap, double *d_snaps, int use_graph);
 
-/* RK4 kernel for asd_rk4_run: 0 = automatic (16 x 16 LDS tiles, or the streaming
- * strips once nrep * ceil(L / 120) * ceil(L / 56) >= 4 waves per CU), 16 or 32 = LDS
+/* RK4 kernel for asd_rk4_run: 0 = automatic (LDS tiles sized by occupancy: 8 x 8
+ * while 16 x 16 tiles would leave CUs idle, 16 x 16 otherwise; the streaming strips
+ * once nrep * ceil(L / 120) * ceil(L / 56) >= 4 waves per CU), 8, 16 or 32 = LDS
  * tiles of that edge, 64 = register-streaming column strips (one 64-lane wave per 56
  * columns of a band of rows; lattices with L < 64 use 16 x 16 tiles).  Results do not
  * depend on the choice (bit for bit). */

@@ -21,6 +21,8 @@ CASES = [
     # L = 40 (32 + 8) in 200 replicas
     (768, 1, dict(D=0.1, alpha=0.05)),
     (40, 200, dict(D=0.1, Bz=0.1)),
+    # 256 tiles of 16 x 16: the automatic choice takes 16 x 16 tiles with 1024 threads
+    (64, 16, dict(D=0.1, Bz=0.05, alpha=0.01)),
 ]
 
 
@@ -100,7 +102,7 @@ def test_tile_edge_does_not_change_results(gctx, orc):
     S0 = torch.tensor(orc.init_spins(8, L * L), device="cuda")
     p = asd_amd.params(D=0.1, Bz=0.05, alpha=0.02, dt=0.01)
     out = []
-    for tile in (16, 32, 0, 64):
+    for tile in (16, 32, 0, 64, 8):
         gctx.rk4_set_tile(tile)
         s = S0.clone()
         gctx.rk4_run(p, L, s, 30, 10, 0, None, use_graph=False)

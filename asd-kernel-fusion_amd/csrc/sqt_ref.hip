@@ -163,6 +163,10 @@ int sqt_unfused_run(asd_ctx *ctx, int L, const double *d_snaps, int nsnap, int T
     const int Ns = L * L, K = 3 * T0;
     const long nk = 3L * (T0 + nlags - 1);
     ASD_CHECK(nsnap >= T0 + nlags - 1, "need nsnap >= T0 + nlags - 1 (%d < %d)", nsnap, T0 + nlags - 1);
+    // the explicit correlation matrix and the per-site phase table (built on the host)
+    // bound this path: Ns^2 and 2 nq Ns doubles
+    ASD_CHECK(8.0 * Ns * Ns <= 64e9, "the unfused path stores Ns^2 doubles: L = %d is too large", L);
+    ASD_CHECK(16.0 * nq * Ns <= 8e9, "the unfused path's phase table (2 nq Ns doubles) is too large: nq = %d", nq);
     hipStream_t st = ctx->stream;
     if (!ctx->blas) {
         rocblas_handle h = nullptr;

@@ -59,3 +59,4 @@ def test_error_path_without_device_work():
     import asd_amd
     with pytest.raises(asd_amd.AsdError):
         asd_amd.qpath_gxmg(7)      # odd L rejected on the host
+

@@ -245,3 +245,16 @@ def test_bruteforce_matches_oracle_and_fused(gctx, orc, L, T0, nlags):
     e_o, e_f = max_rel(bf, ref), max_rel(bf, fu)
     say(f"  max rel err vs oracle {e_o:.3e}, vs fused {e_f:.3e}")
     assert e_o <= 1e-10 and e_f <= 1e-10
+
+
+@pytest.mark.gpu
+def test_unfused_path_rejects_sizes_it_cannot_hold(gctx):
+    """The unfused path stores Ns^2 doubles and a 2 nq Ns phase table; beyond its bounds it
+    returns an error before allocating anything."""
+    torch = need_gpu()
+    snaps = torch.zeros((5, 256 * 256, 3), dtype=torch.float64, device="cuda")
+    with pytest.raises(RuntimeError, match="phase table"):
+        gctx.sqt_unfused(256, snaps, 3, 2, full_grid(256))            # 2 nq Ns = 8.6e9 doubles
+    big = torch.zeros((3, 1024 * 1024, 3), dtype=torch.float64, device="cuda")
+    with pytest.raises(RuntimeError, match="Ns\\^2"):
+        gctx.sqt_unfused(1024, big, 2, 1, [(0, 0)])                   # Ns^2 = 1.1e12 doubles

@@ -373,17 +373,22 @@ static void orc_site_rhs_tlm(const orc_params *p, int L, const double *S, const 
 {
     int row = i / L, col = i % L;
     const double *s = S + 3L * i;
-    double hx = 0, hy = 0, hz = 0, sx = 0, sy = 0, sz = 0; /* h and sum of neighbours */
+    /* f itself with orc_site_rhs's rounding steps, so the linearisation runs along
+     * exactly the trajectory orc_run produces (in a chaotic run a 1-ulp difference
+     * in the base trajectory grows into a visible change of the derivative) */
+    double hx = 0.0, hy = 0.0, hz = 0.0, sx = 0, sy = 0, sz = 0; /* h and sum of neighbours */
     for (int j = 0; j < 4; j++) {
         int nr = (row + ORC_NB[j][0] + L) % L, nc = (col + ORC_NB[j][1] + L) % L;
         const double *t = S + 3L * (nr * L + nc);
+        hx += (t[0] * p->J); hy += (t[1] * p->J); hz += (t[2] * p->J);
         sx += t[0]; sy += t[1]; sz += t[2];
     }
-    hx = p->J * sx; hy = p->J * sy; hz = p->J * sz + 2.0 * p->D * s[2] + p->Bz;
+    hz += 2.0 * p->D * s[2];
+    hz += p->Bz;
     double px = hy * s[2] - hz * s[1], py = hz * s[0] - hx * s[2], pz = hx * s[1] - hy * s[0];
-    double a = p->alpha / p->spin_length, den = 1.0 + p->alpha * p->alpha;
+    double a = p->alpha / p->spin_length, inv_den = 1.0 / (1.0 + p->alpha * p->alpha);
     double qx = s[1] * pz - s[2] * py, qy = s[2] * px - s[0] * pz, qz = s[0] * py - s[1] * px;
-    f[0] = (px + a * qx) / den; f[1] = (py + a * qy) / den; f[2] = (pz + a * qz) / den;
+    f[0] = (px + a * qx) * inv_den; f[1] = (py + a * qy) * inv_den; f[2] = (pz + a * qz) * inv_den;
     for (int ip = 0; ip < 4; ip++) {
         const double *d = dS + ip * ns * 3;
         const double *ds = d + 3L * i;
@@ -405,9 +410,10 @@ static void orc_site_rhs_tlm(const orc_params *p, int L, const double *S, const 
         double dqz = ds[0] * py - ds[1] * px + s[0] * dpy - s[1] * dpx;
         double da = dA / p->spin_length, dden = 2.0 * p->alpha * dA;
         double *o = df + ip * ns * 3 + 3L * i;
-        o[0] = (dpx + da * qx + a * dqx) / den - f[0] * dden / den;
-        o[1] = (dpy + da * qy + a * dqy) / den - f[1] * dden / den;
-        o[2] = (dpz + da * qz + a * dqz) / den - f[2] * dden / den;
+        /* d[(p + a q) / den] = (dp + da q + a dq) / den - f dden / den */
+        o[0] = (dpx + da * qx + a * dqx) * inv_den - f[0] * dden * inv_den;
+        o[1] = (dpy + da * qy + a * dqy) * inv_den - f[1] * dden * inv_den;
+        o[2] = (dpz + da * qz + a * dqz) * inv_den - f[2] * dden * inv_den;
     }
 }
 

@@ -118,7 +118,7 @@ def test_qmat_phase_convention(orc):
         assert abs(complex(Q[pr], Q[pr + 4]) - e) < 1e-14
 
 
-@pytest.mark.parametrize("th", [(1.0, 0.1, 0.1, 0.01), (1.1, 0.12, 0.08, 0.0)])
+@pytest.mark.parametrize("th", [(1.0, 0.1, 0.1, 0.01), (1.1, 0.12, 0.08, 0.0), (-0.7, 0.05, 0.2, 0.2)])
 def test_tlm_matches_finite_differences(orc, th):
     L, every, nsnap, T0, nlags = 4, 5, 9, 5, 5
     S0 = orc.init_spins(3, L * L)
@@ -126,7 +126,10 @@ def test_tlm_matches_finite_differences(orc, th):
     p = orc.params(J=J, D=D, Bz=Bz, alpha=al, dt=0.01)
     snaps, dsnaps = orc.tlm_run(p, L, S0, (nsnap - 1) * every, every, nsnap)
     _, s_chk = orc.run(p, L, S0, (nsnap - 1) * every, every, nsnap)
-    assert np.abs(snaps - s_chk).max() < 1e-13
+    # the tangent model linearises along orc_run's own trajectory, bit for bit: over
+    # thousands of chaotic steps a 1-ulp offset in the base trajectory grows to ~1e-4
+    # in the derivative
+    assert np.array_equal(snaps, s_chk)
     q = gxmg(L)
     h = 1e-6
     for ip in range(4):

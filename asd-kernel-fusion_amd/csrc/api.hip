@@ -51,7 +51,9 @@ static int check_params(const asd_params *p)
     ASD_CHECK(std::isfinite(p->J) && std::isfinite(p->D) && std::isfinite(p->Bz) &&
                   std::isfinite(p->alpha) && std::isfinite(p->dt) && std::isfinite(p->spin_length),
               "non-finite parameter");
-    ASD_CHECK(p->spin_length > 0, "spin_length must be > 0 (got %g)", p->spin_length);
+    // the RK4 kernels' renormalisation assumes |S|^2 within 2^(+-900) of 1
+    ASD_CHECK(p->spin_length >= 1e-100 && p->spin_length <= 1e100, "spin_length must be in [1e-100, 1e100] (got %g)",
+              p->spin_length);
     ASD_CHECK(p->alpha >= 0, "alpha must be >= 0 (got %g)", p->alpha);
     return ASD_OK;
 }

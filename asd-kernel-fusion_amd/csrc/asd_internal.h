@@ -48,6 +48,10 @@ struct LLGParams {
     double a_over_len;  // alpha / spin_length
     double den;         // 1 + alpha^2
     double inv_den;     // 1 / (1 + alpha^2), formed once
+    // the divisors of the k/6 and k/3 terms as kernel arguments: held in SGPRs they can be
+    // operands of a VOP3 FMA, where a literal cannot (gfx9 encodings)
+    double six, three, three_halves;
+    double half_dt;     // dt / 2 (exact)
 };
 LLGParams make_llg(const asd_params &p);
 

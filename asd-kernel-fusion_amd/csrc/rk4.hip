@@ -32,6 +32,10 @@ LLGParams make_llg(const asd_params &p)
     q.a_over_len = p.alpha / p.spin_length;
     q.den = 1.0 + p.alpha * p.alpha;
     q.inv_den = 1.0 / q.den;
+    q.six = 6.0;
+    q.three = 3.0;
+    q.three_halves = 1.5;
+    q.half_dt = 0.5 * p.dt;
     return q;
 }
 
@@ -45,7 +49,11 @@ __device__ __forceinline__ void llg_f(const LLGParams &p, double sx, double sy, 
                                       const double *n0, const double *n1, const double *n2,
                                       const double *n3, double &fx, double &fy, double &fz)
 {
-    double hx = 0.0, hy = 0.0, hz = 0.0;
+    // the sums start from -0.0, the additive identity: -0.0 + x is x for every x, so the
+    // first add folds away.  The reference starts from 0.0; 0.0 + x differs from x only for
+    // x = -0.0, so the sums are equal (IEEE ==) and can differ only in the sign of an exact
+    // zero, which no later operation turns into a different value.
+    double hx = -0.0, hy = -0.0, hz = -0.0;
     if constexpr (UNIT_J) {
         hx += n0[0]; hy += n0[1]; hz += n0[2];
         hx += n1[0]; hy += n1[1]; hz += n1[2];
@@ -101,7 +109,46 @@ __device__ __forceinline__ void renorm(double len, double ax, double ay, double 
     };
     ox = one(ax); oy = one(ay); oz = one(az);
 }
-constexpr double kSixth = 1.0 / 6.0, kThird = 1.0 / 3.0;
+
+// The same three values (len a) / sqrt(a.a) for operands in the normal range, in fewer
+// instructions.  gfx950's correctly rounded f64 square root and division are Newton
+// sequences around v_rsq_f64 / v_rcp_f64, wrapped in operand scaling and special-case
+// selection for zeros, infinities and exponents beyond about +-900; here |a|^2 is near
+// len^2 (the RK4 increment is a small rotation), so the wrappers are identities and are
+// left out:
+//   sqrt(x): y = rsq(x); g = x y; h = y / 2; r = 1/2 - h g; g += g r; h += h r;
+//            twice: d = x - g g; g += d h                                (as the compiler's)
+//   n / b:   y0 = rcp(b); twice: e = 1 - b y; y += y e;   q = n y; r = n - b q;  q + r y
+// The reciprocal iterate y depends on b alone, so one serves all three quotients, each
+// of which is then exactly the compiler's IEEE division.  Valid while |a|^2 and every
+// nonzero len a_i stay above 2^-900 and below 2^900 (the launcher's parameter range
+// keeps spin_length there).
+__device__ __forceinline__ void renorm_fast(double len, double ax, double ay, double az, double &ox, double &oy,
+                                            double &oz)
+{
+    const double x = ax * ax + ay * ay + az * az;
+    const double rs = __builtin_amdgcn_rsq(x);
+    double g = x * rs, h = rs * 0.5;
+    const double r0 = __builtin_fma(-h, g, 0.5);
+    g = __builtin_fma(g, r0, g);
+    h = __builtin_fma(h, r0, h);
+    double d = __builtin_fma(-g, g, x);
+    g = __builtin_fma(d, h, g);
+    d = __builtin_fma(-g, g, x);
+    const double nrm = __builtin_fma(d, h, g);
+    double y = __builtin_amdgcn_rcp(nrm);
+    double e = __builtin_fma(-nrm, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    e = __builtin_fma(-nrm, y, 1.0);
+    y = __builtin_fma(y, e, y);
+    auto one = [&](double a) {
+        const double n = len * a, q = n * y;
+        const double r = __builtin_fma(-nrm, q, n);
+        return __builtin_fma(r, y, q);
+    };
+    ox = one(ax); oy = one(ay); oz = one(az);
+}
+constexpr double kSixth = 1.0 / 6.0, kThird = 1.0 / 3.0, kTwoThirds = 2.0 / 3.0;
 
 __device__ __forceinline__ int wrap(int x, int L)
 {
@@ -208,13 +255,13 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
         if (!tile_site) continue;
         const double kx = sK[0][own], ky = sK[1][own], kz = sK[2][own];
         if (stage == 1) {
-            ax = add_div(sS[0][own], kx, 6.0, kSixth);
-            ay = add_div(sS[1][own], ky, 6.0, kSixth);
-            az = add_div(sS[2][own], kz, 6.0, kSixth);
+            ax = add_div(sS[0][own], kx, p.six, kSixth);
+            ay = add_div(sS[1][own], ky, p.six, kSixth);
+            az = add_div(sS[2][own], kz, p.six, kSixth);
         } else if (stage == 4) {
-            ax = add_div(ax, kx, 6.0, kSixth); ay = add_div(ay, ky, 6.0, kSixth); az = add_div(az, kz, 6.0, kSixth);
+            ax = add_div(ax, kx, p.six, kSixth); ay = add_div(ay, ky, p.six, kSixth); az = add_div(az, kz, p.six, kSixth);
         } else {
-            ax = add_div(ax, kx, 3.0, kThird); ay = add_div(ay, ky, 3.0, kThird); az = add_div(az, kz, 3.0, kThird);
+            ax = add_div(ax, kx, p.three, kThird); ay = add_div(ay, ky, p.three, kThird); az = add_div(az, kz, p.three, kThird);
         }
     }
     if (owner) {
@@ -294,10 +341,11 @@ __device__ __forceinline__ double lane_from_right(double v)  // lane l receives 
     return __hiloint2double(hi, lo);
 }
 
-// k = dt f(c) with row+1 = up, row-1 = dn and the column neighbours from the adjacent lanes
+// k = h f(c) (h = dt, or dt/2 for the halved increment) with row+1 = up, row-1 = dn and
+// the column neighbours from the adjacent lanes
 template <bool UJ, bool UD>
 __device__ __forceinline__ void stream_k(const LLGParams &p, const double (&c)[3], const double (&up)[3],
-                                         const double (&dn)[3], double (&k)[3])
+                                         const double (&dn)[3], double h, double (&k)[3])
 {
     double lf[3], rt[3];
 #pragma unroll
@@ -307,7 +355,7 @@ __device__ __forceinline__ void stream_k(const LLGParams &p, const double (&c)[3
     }
     double fx, fy, fz;
     llg_f<UJ, UD>(p, c[0], c[1], c[2], up, dn, lf, rt, fx, fy, fz);
-    k[0] = fx * p.dt; k[1] = fy * p.dt; k[2] = fz * p.dt;
+    k[0] = fx * h; k[1] = fy * h; k[2] = fz * h;
 }
 
 // grid: 4 waves per block, one work item (replica, band, strip) per wave; R % 4 == 0,
@@ -319,7 +367,10 @@ __global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int
                                                          const int *__restrict__ snap_counter, long snap_stride)
 {
     const int lane = threadIdx.x & 63;
-    const long item = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    // the wave's index as a scalar: everything derived from it (band, rows, the row of
+    // each load and store) is then scalar arithmetic instead of per-lane VALU work
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long item = (long)blockIdx.x * 4 + wave;
     if (item >= (long)nrep * nband * nstrip) return;  // whole wave; the kernel has no barriers
     const int strip = (int)(item % nstrip), band = (int)((item / nstrip) % nband);
     const int rep = (int)(item / ((long)nstrip * nband));
@@ -340,82 +391,110 @@ __global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int
         y -= y >= L ? L : 0;
         return y;
     };
+    // addresses as a scalar row base plus this lane's 32-bit byte offset (3 gc doubles),
+    // which the loads and stores take as SGPR base + VGPR offset with no per-lane
+    // 64-bit arithmetic
+    const unsigned lane_off = 24u * (unsigned)gc;
+    auto at = [&](const double *row_base) {
+        return (const double *)((const char *)row_base + lane_off);
+    };
+    auto at_w = [&](double *row_base) { return (double *)((char *)row_base + lane_off); };
+    const long row_stride = 3L * L;
     auto load = [&](int t, double (&v)[3]) {
-        const double *q = src + 3 * ((long)row_of(t) * L + gc);
+        const double *q = at(src + row_of(t) * row_stride);
         v[0] = q[0]; v[1] = q[1]; v[2] = q[2];
     };
 
-    double S[4][3], Y2[4][3], Y3[4][3], Y4[4][3], pf[4][3];
+    // B: ring of the rows loaded, row t in slot t % 8 from its load (PF = 4 steps ahead)
+    // to its last use as a state at step t + 3.  Eight slots and an eight-step unroll give
+    // every row fixed registers for its whole life: with separate prefetch and state rings
+    // the compiler moves each loaded row between them at the loop latch, and that move
+    // waits for every load in flight.  Y2/Y3/Y4: stage inputs of the rows in flight.
+    double B[8][3], Y2[4][3], Y3[4][3], Y4[4][3];
     // the partial sums S + k1/6 + k2/3 + k3/3 of the four rows in flight live in LDS
     // (each lane only touches its own doubles: no barriers), which keeps the register
     // file at three waves per SIMD
     __shared__ double sA[4][4][3][64];  // [wave][ring slot][component][lane]
-    double(&A)[4][3][64] = sA[threadIdx.x >> 6];
+    double(&A)[4][3][64] = sA[wave];
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
         for (int d = 0; d < 3; d++) {
-            S[a][d] = 0.0; Y2[a][d] = 0.0; Y3[a][d] = 0.0; Y4[a][d] = 0.0; pf[a][d] = 0.0;
+            B[a + 4][d] = 0.0; Y2[a][d] = 0.0; Y3[a][d] = 0.0; Y4[a][d] = 0.0;
             A[a][d][lane] = 0.0;
         }
-    const int nstep = rows + 2 * kStreamHalo;  // steps t = 0 .. nstep-1; output row t-8 at step t
-    // prefetch PF rows ahead (with the partial sums in LDS and PF = 4: 164 VGPRs, three
-    // waves per SIMD, no scratch; PF = 3 measured 11% slower at 1024 x 1024 x 64, the
-    // partial sums in registers another 8%)
+    // steps t = 0 .. nstep-1 (output row t-8 at step t), padded to whole groups of eight
+    // so the unrolled loop has no branch around a load (a conditional load makes the
+    // compiler copy the old value into the load's destination first, which again waits
+    // for every load in flight).  Padding steps compute on valid rows and store nothing;
+    // loads past the last step repeat row nstep-1.
+    const int nstep = rows + 2 * kStreamHalo;
+    const int nstep8 = (nstep + 7) & ~7;
+    // prefetch PF = 4 rows ahead (with the partial sums in LDS: three waves per SIMD, no
+    // scratch; PF = 3 measured 11% slower at 1024 x 1024 x 64, the partial sums in
+    // registers another 8%)
     constexpr int PF = 4;
-    load(0, pf[0]);
-    if (PF > 1 && nstep > 1) load(1, pf[1]);
-    if (PF > 2 && nstep > 2) load(2, pf[2]);
-    if (PF > 3 && nstep > 3) load(3, pf[3]);
-    for (int t0 = 0; t0 < nstep; t0 += 4) {
+    // issued in row order, as the loop issues them: the wait at the loop head is the
+    // worst case over its entries, and rows loaded in another order here would make it
+    // wait for the most recent loads
 #pragma unroll
-        for (int u = 0; u < 4; u++) {
+    for (int a = 0; a < PF; a++) {
+        __builtin_amdgcn_sched_barrier(0);
+        load(a, B[a]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    for (int t0 = 0; t0 < nstep8; t0 += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
             const int t = t0 + u;
-            if (t < nstep) {
-                // ring slots of rows t, t-1, .., t-5 (t-4 shares the slot of t)
-                constexpr int s0 = 0;
-                const int i0 = (u + 4 - 0) & 3, i1 = (u + 4 - 1) & 3, i2 = (u + 4 - 2) & 3, i3 = (u + 4 - 3) & 3;
-                const int i4 = i0, i5 = i1;
-                (void)s0;
+            // row slots of rows t, t-1, t-2, t-3
+            const int b0 = u, b1 = (u + 7) & 7, b2 = (u + 6) & 7, b3 = (u + 5) & 7;
+            // stage-input slots of rows t, t-1, .., t-5 (t-4 shares the slot of t)
+            const int i0 = u & 3, i1 = (u + 3) & 3, i2 = (u + 2) & 3, i3 = (u + 1) & 3;
+            const int i4 = i0, i5 = i1;
+            // keep each step's instructions within the step: a load hoisted above the last
+            // use of its slot's previous row needs another register, and then a move at the
+            // loop latch that waits for every load in flight
+            __builtin_amdgcn_sched_barrier(0);
+            load(min(t + PF, nstep - 1), B[(u + PF) & 7]);  // the slot of row t-4, now free
+            const int tr = t - kStreamHalo;                  // band-relative row of row t
+            if (sn && ocol && tr >= 0 && tr < rows) {
+                double *o = at_w(sn + (r0 + tr) * row_stride);
+                o[0] = B[b0][0]; o[1] = B[b0][1]; o[2] = B[b0][2];
+            }
+            // Stages 1 and 2 form the halved increment kh = f (dt/2) directly: it is k/2
+            // exactly (RN(f dt)/2 = RN(f dt/2) while k is above the subnormal range), so
+            // Y = S + k/2 is S + kh and k/6, k/3 are the correctly rounded kh/3, kh/1.5.
+            double k[3];
+            // stage 1 at row t-1: Y1 = S
+            stream_k<UJ, UD>(p, B[b1], B[b0], B[b2], p.half_dt, k);
 #pragma unroll
-                for (int d = 0; d < 3; d++) S[i0][d] = pf[i0][d];
-                if (t + PF < nstep) load(t + PF, pf[(u + PF) & 3]);
-                const int tr = t - kStreamHalo;  // band-relative row of the state loaded now
-                if (sn && ocol && tr >= 0 && tr < rows) {
-                    double *o = sn + 3 * ((long)(r0 + tr) * L + gc);
-                    o[0] = S[i0][0]; o[1] = S[i0][1]; o[2] = S[i0][2];
-                }
-                double k[3];
-                // stage 1 at row t-1: Y1 = S
-                stream_k<UJ, UD>(p, S[i1], S[i0], S[i2], k);
+            for (int d = 0; d < 3; d++) {
+                A[i1][d][lane] = add_div(B[b1][d], k[d], p.three, kThird);  // + k1/6
+                Y2[i1][d] = B[b1][d] + k[d];                                // S + k1/2
+            }
+            // stage 2 at row t-2
+            stream_k<UJ, UD>(p, Y2[i2], Y2[i1], Y2[i3], p.half_dt, k);
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    A[i1][d][lane] = add_div(S[i1][d], k[d], 6.0, kSixth);
-                    Y2[i1][d] = S[i1][d] + k[d] / 2.0;
-                }
-                // stage 2 at row t-2
-                stream_k<UJ, UD>(p, Y2[i2], Y2[i1], Y2[i3], k);
+            for (int d = 0; d < 3; d++) {
+                A[i2][d][lane] = add_div(A[i2][d][lane], k[d], p.three_halves, kTwoThirds);  // + k2/3
+                Y3[i2][d] = B[b2][d] + k[d];                                                  // S + k2/2
+            }
+            // stage 3 at row t-3
+            stream_k<UJ, UD>(p, Y3[i3], Y3[i2], Y3[i4], p.dt, k);
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    A[i2][d][lane] = add_div(A[i2][d][lane], k[d], 3.0, kThird);
-                    Y3[i2][d] = S[i2][d] + k[d] / 2.0;
-                }
-                // stage 3 at row t-3
-                stream_k<UJ, UD>(p, Y3[i3], Y3[i2], Y3[i4], k);
+            for (int d = 0; d < 3; d++) {
+                A[i3][d][lane] = add_div(A[i3][d][lane], k[d], p.three, kThird);
+                Y4[i3][d] = B[b3][d] + k[d];
+            }
+            // stage 4 at row t-4, then the new state of that row
+            stream_k<UJ, UD>(p, Y4[i4], Y4[i3], Y4[i5], p.dt, k);
+            const int orow = t - 2 * kStreamHalo;  // band-relative output row
 #pragma unroll
-                for (int d = 0; d < 3; d++) {
-                    A[i3][d][lane] = add_div(A[i3][d][lane], k[d], 3.0, kThird);
-                    Y4[i3][d] = S[i3][d] + k[d];
-                }
-                // stage 4 at row t-4, then the new state of that row
-                stream_k<UJ, UD>(p, Y4[i4], Y4[i3], Y4[i5], k);
-                const int orow = t - 2 * kStreamHalo;  // band-relative output row
-#pragma unroll
-                for (int d = 0; d < 3; d++) A[i4][d][lane] = add_div(A[i4][d][lane], k[d], 6.0, kSixth);
-                if (ocol && orow >= 0) {
-                    double *o = dst + 3 * ((long)(r0 + orow) * L + gc);
-                    renorm(p.len, A[i4][0][lane], A[i4][1][lane], A[i4][2][lane], o[0], o[1], o[2]);
-                }
+            for (int d = 0; d < 3; d++) A[i4][d][lane] = add_div(A[i4][d][lane], k[d], p.six, kSixth);
+            if (ocol && orow >= 0 && orow < rows) {
+                double *o = at_w(dst + (r0 + orow) * row_stride);
+                renorm_fast(p.len, A[i4][0][lane], A[i4][1][lane], A[i4][2][lane], o[0], o[1], o[2]);
             }
         }
     }

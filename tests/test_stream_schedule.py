@@ -1,8 +1,10 @@
 """The register-streaming RK4 step (rk4_stream_kernel in rk4.hip) evaluates stage s
-of row t-s while loading row t, keeps rows in 4-slot rings indexed by the step mod 4,
-takes column neighbours from the adjacent lanes (whole-wave DPP shifts; the edge lane
-reads 0) and covers the lattice with 56-column strips and bands of min(120, L - 8)
-rows.  This emulates exactly that schedule with one numpy vector per wave (64 lanes),
+of row t-s at step t, keeps the loaded rows in an 8-slot ring indexed by the row mod 8
+(each loaded 4 steps ahead) and the stage inputs in 4-slot rings indexed by the step
+mod 4, pads the steps to a multiple of 8 (padding steps store nothing; loads past the
+last step repeat its row), takes column neighbours from the adjacent lanes (whole-wave
+DPP shifts; the edge lane reads 0) and covers the lattice with 56-column strips and
+bands of min(120, L - 8) rows.  This emulates exactly that schedule with one numpy vector per wave (64 lanes),
 in IEEE double with no fused operations, and checks one step against the oracle bit for
 bit, plus that every site is written once and every load stays inside the lattice.
 It pins the schedule; the GPU tests pin the kernel itself (test_gpu_rk4.py).
@@ -66,31 +68,32 @@ def stream_step(p, L, S_in):
                 return [S_in[y * L + gc, d].copy() for d in range(3)]
 
             z = lambda: [np.zeros(64) for _ in range(3)]
-            S, Y2, Y3, Y4, A, pf = ([z() for _ in range(4)] for _ in range(6))
+            Y2, Y3, Y4, A = ([z() for _ in range(4)] for _ in range(4))
+            B = [z() for _ in range(8)]
             nstep = rows + 2 * HALO
+            nstep8 = (nstep + 7) & ~7
             PF = 4
-            for st in range(min(PF, nstep)):
-                pf[st] = load(st)
-            for t in range(nstep):
-                u = t % 4
-                i0, i1, i2, i3 = u, (u + 3) & 3, (u + 2) & 3, (u + 1) & 3
+            for st in range(PF):
+                B[st] = load(st)
+            for t in range(nstep8):
+                u = t % 8
+                b0, b1, b2, b3 = u, (u + 7) & 7, (u + 6) & 7, (u + 5) & 7
+                i0, i1, i2, i3 = u & 3, (u + 3) & 3, (u + 2) & 3, (u + 1) & 3
                 i4, i5 = i0, i1
-                S[i0] = [x.copy() for x in pf[i0]]
-                if t + PF < nstep:
-                    pf[(u + PF) & 3] = load(t + PF)
-                k = _k(p, S[i1], S[i0], S[i2])
-                A[i1] = [S[i1][d] + k[d] / 6.0 for d in range(3)]
-                Y2[i1] = [S[i1][d] + k[d] / 2.0 for d in range(3)]
+                B[(u + PF) & 7] = load(min(t + PF, nstep - 1))   # the slot of row t-4
+                k = _k(p, B[b1], B[b0], B[b2])
+                A[i1] = [B[b1][d] + k[d] / 6.0 for d in range(3)]
+                Y2[i1] = [B[b1][d] + k[d] / 2.0 for d in range(3)]
                 k = _k(p, Y2[i2], Y2[i1], Y2[i3])
                 A[i2] = [A[i2][d] + k[d] / 3.0 for d in range(3)]
-                Y3[i2] = [S[i2][d] + k[d] / 2.0 for d in range(3)]
+                Y3[i2] = [B[b2][d] + k[d] / 2.0 for d in range(3)]
                 k = _k(p, Y3[i3], Y3[i2], Y3[i4])
                 A[i3] = [A[i3][d] + k[d] / 3.0 for d in range(3)]
-                Y4[i3] = [S[i3][d] + k[d] for d in range(3)]
+                Y4[i3] = [B[b3][d] + k[d] for d in range(3)]
                 k = _k(p, Y4[i4], Y4[i3], Y4[i5])
                 A[i4] = [A[i4][d] + k[d] / 6.0 for d in range(3)]
                 orow = t - 2 * HALO
-                if orow >= 0:
+                if 0 <= orow < rows:
                     ax, ay, az = A[i4]
                     nrm = np.sqrt(ax * ax + ay * ay + az * az)
                     idx = (r0 + orow) * L + gc
@@ -102,7 +105,8 @@ def stream_step(p, L, S_in):
 
 
 @pytest.mark.parametrize("L,kw", [(64, dict(D=0.1, Bz=0.1, alpha=0.01)),
-                                  (100, dict(J=-0.7, D=0.05, Bz=0.2, alpha=0.2, spin_length=1.5))])
+                                  (100, dict(J=-0.7, D=0.05, Bz=0.2, alpha=0.2, spin_length=1.5)),
+                                  (130, dict(D=0.1))])   # band of 10 rows: 18 steps padded to 24
 def test_streaming_schedule_reproduces_oracle_step(orc, L, kw):
     p = dict(J=1.0, D=0.0, Bz=0.0, alpha=0.0, dt=0.01, len=1.0)
     for key, v in kw.items():

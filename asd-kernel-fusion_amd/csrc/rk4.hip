@@ -10,12 +10,12 @@
 // and writes the TR x TC interior once: 24 B read + 24 B written per site-step.
 //
 // This file is compiled with -ffp-contract=off and keeps the reference's operation
-// order (field accumulated neighbour by neighbour from 0.0, J multiplied per term,
-// v3d_cross operand order, the k/6 + k/3 + k/3 + k/6 left-to-right sum, correctly
-// rounded divisions), so trajectories match the FP64 CPU restatement bit for bit.
-// Two exact shortcuts apply when the parameters allow: J == 1 drops the multiplies
-// by J (bit for bit), and alpha == 0 drops the damping term, which leaves every value
-// equal (IEEE ==) and can only turn an exact -0 into +0.
+// order (field accumulated neighbour by neighbour, J multiplied per term, v3d_cross
+// operand order, the k/6 + k/3 + k/3 + k/6 left-to-right sum, correctly rounded
+// divisions), so trajectories equal the FP64 CPU restatement's (IEEE ==); bits can
+// differ only in the sign of an exact zero.  Exact shortcuts: the field sums start from
+// -0.0 (the additive identity) where the reference starts from 0.0, J == 1 drops the
+// multiplies by J, and alpha == 0 drops the damping term.
 #include "asd_internal.h"
 
 namespace asd {
@@ -320,11 +320,19 @@ __global__ __launch_bounds__(TR *TC) void rk4_batch_kernel(const BatchTraj *__re
 // at step t it loads row t and evaluates stage 1 at row t-1, stage 2 at row t-2, stage
 // 3 at row t-3 and stage 4 at row t-4, so each stage finds its row+1 / row-1 inputs in
 // the same lane's registers and its col-1 / col+1 inputs in the neighbouring lanes (DPP
-// whole-wave shifts).  No LDS and no barriers; each site is read once and written once
-// (plus the halo: 64/56 columns, (R+8)/R rows).  Rows are kept in 4-slot rings indexed
-// by the step modulo 4, so the 4-way unrolled loop keeps every slot in a register.
-// Every site sees the same expressions as rk4_tile (llg_f, Y = S + k/2.0 or S + k,
-// k/6.0 + k/3.0 + k/3.0 + k/6.0 left to right, len * a / |a|): bit-identical results.
+// whole-wave shifts).  No barriers; LDS holds only each lane's own partial sums.  Each
+// site is read once and written once (plus the halo: 64/56 columns, (R+8)/R rows).
+// Every site gets the values of rk4_tile's expressions (llg_f, Y = S + k/2.0 or S + k,
+// k/6.0 + k/3.0 + k/3.0 + k/6.0 left to right, len * a / |a|), several through exact
+// rewrites that need fewer instructions (halved increments, FMA-corrected quotients,
+// the square root and division without their range wrappers); the results equal the
+// reference's (IEEE ==) and differ in bits at most in the sign of an exact zero.
+//
+// The kernel is bound by FP64 VALU issue, not by HBM: about 236 VALU instructions per
+// row step of a wave (48 of them the DPP lane shifts), 4 cycles each, for 56 output sites.
+// Moving the lane shifts to the LDS crossbar (ds_bpermute) measured slower (the LDS
+// pipe saturates), and so did a persistent grid of one equal share per resident wave
+// (its outer segment loop costs the fourth wave per SIMD).
 constexpr int kStreamHalo = 4, kStreamCols = 64 - 2 * kStreamHalo;
 
 // bound_ctrl: the edge lane (a halo lane) receives 0, so no old value is needed

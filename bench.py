@@ -287,7 +287,7 @@ def workload_rk4sweep(args):
         row = {"L": L, "replicas": nrep, "sites": n, "state_MB": round(n * 24 / 1e6, 1)}
         for tile in (0, 8, 16, 32, 64):   # 0: the automatic choice
             ctx.rk4_set_tile(tile)
-            ctx.rk4_run(p, L, S, 2)
+            ctx.rk4_run(p, L, S, steps)   # untimed: the clocks settle
             torch.cuda.synchronize()
             ctx.rk4_run(p, L, S, steps)
             ms = ctx.stage_times()["rk4"]
@@ -312,7 +312,7 @@ def workload_rk4large(args):
     p = asd_amd.params(J=1.0, D=0.1, dt=0.01)
     L, nrep, steps = 1024, 64, 30
     S = torch.tensor(asd_amd.init_spins(5, L * L), device="cuda").repeat(nrep, 1)
-    ctx.rk4_run(p, L, S, 2)
+    ctx.rk4_run(p, L, S, steps)   # untimed: the clocks settle (the first run is ~10% slower)
     torch.cuda.synchronize()
     ctx.rk4_run(p, L, S, steps)
     ms = ctx.stage_times()["rk4"]
@@ -535,7 +535,7 @@ def main():
     if not args.no_rk4_large:
         Lb, nb, steps_b = 1024, 64, 30
         Sb = torch.tensor(asd_amd.init_spins(5, Lb * Lb), device="cuda").repeat(nb, 1)
-        ctx.rk4_run(prm, Lb, Sb, 2)
+        ctx.rk4_run(prm, Lb, Sb, steps_b)   # untimed: the clocks settle (the first run is ~10% slower)
         torch.cuda.synchronize()
         ctx.rk4_run(prm, Lb, Sb, steps_b)
         ms_b = ctx.stage_times()["rk4"]

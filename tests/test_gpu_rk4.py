@@ -169,3 +169,22 @@ def test_automatic_choice_takes_streaming_for_many_replicas(gctx, orc):
     finally:
         gctx.rk4_set_tile(0)
     assert np.array_equal(out[0], out[1]) and np.array_equal(out[0], out[2])
+
+
+@pytest.mark.gpu
+def test_spin_length_outside_the_kernels_range_is_rejected(gctx, orc):
+    """The streaming kernel's renormalisation assumes |S|^2 within 2^(+-900) of 1; the
+    parameter check keeps spin_length in [1e-100, 1e100] and reports anything else."""
+    torch = need_gpu()
+    import asd_amd
+    L = 64
+    s = torch.tensor(orc.init_spins(3, L * L), device="cuda")
+    for bad in (1e101, 1e-101, 0.0, -1.0):
+        with pytest.raises(asd_amd.AsdError, match="spin_length"):
+            gctx.rk4_run(asd_amd.params(dt=0.01, spin_length=bad), L, s, 1, 1, 0, None)
+    # the ends of the range run (J scaled so the precession per step stays O(dt))
+    for ok in (1e-100, 1e100):
+        s2 = s * ok
+        gctx.rk4_run(asd_amd.params(J=1.0 / ok, dt=0.01, spin_length=ok), L, s2, 2, 1, 0, None)
+        gctx.synchronize()
+        assert np.allclose(torch.linalg.norm(s2, dim=1).cpu().numpy() / ok, 1.0, atol=1e-12)

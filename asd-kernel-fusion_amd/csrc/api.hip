@@ -278,16 +278,23 @@ int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spi
     long s = 0;
     int cur = 0;
 
-    // graph: one chunk = `every` steps (2*every if every is odd, so the ping-pong parity
-    // repeats), first node writes snapshot *counter, a tiny node bumps the counter
+    // launches of two fused steps where the kernel choice allows it and no snapshot falls
+    // between the two steps of a launch (snapshots are of a launch's input state)
+    const int fs = rk4_fusable_steps(c, L, nrep) == 2 && (!d_snaps || snap_every % 2 == 0) ? 2 : 1;
+
+    // graph: one chunk = `every` steps, doubled until it holds an even number of launches
+    // (so the ping-pong parity repeats); the first launch writes snapshot *counter, a tiny
+    // node bumps the counter
     if (use_graph && d_snaps && nsnap_in_loop >= 2) {
-        const int chunk = snap_every % 2 ? 2 * snap_every : snap_every;
+        int chunk = snap_every;
+        while ((chunk / fs) % 2) chunk *= 2;
         const long nchunks = std::min<long>(nsteps / chunk, nsnap_in_loop / (chunk / snap_every));
         if (nchunks >= 1) {
             RK4Graph &g = c->graph;
             const bool same = g.exec && g.L == L && g.nrep == nrep && g.every == snap_every && g.chunk == chunk &&
-                              g.a == bufs[0] && g.b == bufs[1] && g.snaps == d_snaps &&
-                              std::memcmp(&g.p, &p, sizeof p) == 0 && g.snap_stride == snap_stride;
+                              g.tile == c->rk4_tile && g.fuse == fs && g.a == bufs[0] && g.b == bufs[1] &&
+                              g.snaps == d_snaps && std::memcmp(&g.p, &p, sizeof p) == 0 &&
+                              g.snap_stride == snap_stride;
             if (!same) {
                 if (g.exec) (void)hipGraphExecDestroy(g.exec);
                 g = RK4Graph();
@@ -298,10 +305,11 @@ int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spi
                 hipStream_t saved = c->stream;
                 c->stream = cap;
                 int r = ASD_OK;
-                for (int k = 0; k < chunk && r == ASD_OK; k++) {
+                for (int k = 0; k < chunk && r == ASD_OK; k += fs) {
                     const bool snap_here = k % snap_every == 0;
-                    r = rk4_launch_step(c, p, L, nrep, bufs[k & 1], bufs[(k + 1) & 1], snap_here ? d_snaps : nullptr,
-                                        c->counter.as<int>(), snap_stride, snap_here ? 1 : 0);
+                    const int j = k / fs;  // launch index: its input buffer alternates
+                    r = rk4_launch_step(c, p, L, nrep, bufs[j & 1], bufs[(j + 1) & 1], snap_here ? d_snaps : nullptr,
+                                        c->counter.as<int>(), snap_stride, snap_here ? 1 : 0, fs);
                 }
                 c->stream = saved;
                 hipError_t e1 = hipStreamEndCapture(cap, &graph);
@@ -311,21 +319,24 @@ int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spi
                 hipError_t e2 = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(graph);
                 ASD_HIP(e2);
-                g.L = L; g.nrep = nrep; g.every = snap_every; g.chunk = chunk;
+                g.L = L; g.nrep = nrep; g.every = snap_every; g.chunk = chunk; g.tile = c->rk4_tile; g.fuse = fs;
                 g.a = bufs[0]; g.b = bufs[1]; g.snaps = d_snaps; g.p = p; g.snap_stride = snap_stride;
             }
             hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, c->counter.as<int>(), 0);
             for (long ch = 0; ch < nchunks; ch++) ASD_HIP(hipGraphLaunch(g.exec, st));
             s = nchunks * chunk;
-            cur = 0;  // chunk is even: state back in bufs[0]
+            cur = 0;  // an even number of launches per chunk: state back in bufs[0]
             // the counter now equals s / every; continue eagerly with explicit slots
         }
     }
-    for (; s < nsteps; s++) {
+    while (s < nsteps) {
         double *snap = nullptr;
         if (d_snaps && s % snap_every == 0 && s / snap_every < nsnap) snap = d_snaps + (s / snap_every) * snap_stride;
-        ASD_TRY(rk4_launch_step(c, p, L, nrep, bufs[cur], bufs[cur ^ 1], snap, nullptr, snap_stride, 0));
+        const bool mid_snap = d_snaps && (s + 1) % snap_every == 0 && (s + 1) / snap_every < nsnap;
+        const int k = fs == 2 && s + 2 <= nsteps && !mid_snap ? 2 : 1;
+        ASD_TRY(rk4_launch_step(c, p, L, nrep, bufs[cur], bufs[cur ^ 1], snap, nullptr, snap_stride, 0, k));
         cur ^= 1;
+        s += k;
     }
     if (d_snaps && nsteps % snap_every == 0 && nsteps / snap_every < nsnap)
         ASD_HIP(hipMemcpyAsync(d_snaps + (nsteps / snap_every) * snap_stride, bufs[cur], sizeof(double) * n,

@@ -71,7 +71,7 @@ struct BatchGraph {
 struct RK4Graph {
     hipGraphExec_t exec = nullptr;
     // key
-    int L = 0, nrep = 0, every = 0, chunk = 0;
+    int L = 0, nrep = 0, every = 0, chunk = 0, tile = 0, fuse = 1;
     const void *a = nullptr, *b = nullptr, *snaps = nullptr;
     LLGParams p{};
     long snap_stride = 0;
@@ -116,9 +116,14 @@ int mark(asd_ctx *ctx, int k);
 // internal entry points shared between translation units
 int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items, int nitems, int parity,
                      const int *snap_counter, int write_snap, int bump_counter);
+// one launch of `steps` RK4 steps (1, or 2 where rk4_fusable_steps allows it); the
+// snapshot, if any, is of the input state
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in,
                     double *out, double *snap_out, const int *snap_counter, long snap_stride,
-                    int bump_counter);
+                    int bump_counter, int steps = 1);
+// steps one rk4_launch_step may fuse for this lattice: 2 where the automatic choice is the
+// small-lattice 8 x 8 tiles (one step is latency-bound there), else 1
+int rk4_fusable_steps(const asd_ctx *ctx, int L, int nrep);
 int rk4_compat_stages(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *spin,
                       double *cur, double *k1, double *k2, double *k3, double *k4);
 int rk4_stages_store(asd_ctx *ctx, const LLGParams &p, int L, const double *S, double *Y2, double *Y3, double *Y4,

@@ -156,28 +156,34 @@ __device__ __forceinline__ int wrap(int x, int L)
     return x < 0 ? x + L : x;
 }
 
-// One full RK4 step (+ renormalisation) of one TR x TC output tile, one interior
-// site per thread.  LDS holds the tile + halo twice, structure-of-arrays: S and the
-// latest increment K.  The stage input Y_s = S + c k_{s-1} (c = 1/2, 1/2, 1) is formed
-// when read -- the same expression the reference stores, so the same rounding -- which
-// keeps LDS at two arrays (77 KB for 32 x 32: two workgroups per CU overlap each
-// other's load latency and barriers).  Each thread keeps its sites' new increments in
-// registers until every read of the previous K is done.
-template <int TR, int TC> struct TileLDS {
-    static constexpr int R = 4, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
+// NS full RK4 steps (+ renormalisation) of one TR x TC output tile.  LDS holds the tile
+// + a radius-4 NS halo twice, structure-of-arrays: S and the latest increment K.  The
+// stage input Y_s = S + c k_{s-1} (c = 1/2, 1/2, 1) is formed when read -- the same
+// expression the reference stores, so the same rounding -- which keeps LDS at two arrays
+// (77 KB for 32 x 32 in one step: two workgroups per CU overlap each other's load
+// latency and barriers).  Step m of NS produces the new state of the region within
+// radius 4 (NS - 1 - m) of the tile, back into S for the next step, and the last step
+// the tile itself; every site goes through the same operations as in a one-step launch,
+// so fusing steps changes no bit.  Fusing pays where one step is latency-bound: two
+// steps load and store once.  Each thread keeps its sites' new increments in registers
+// until every read of the previous K is done.
+template <int TR, int TC, int NS = 1> struct TileLDS {
+    static constexpr int R = 4 * NS, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
     double S[3][NR], K[3][NR];
 };
 
-template <int TR, int TC, bool UJ = false, bool UD = false, int NT = TR * TC>
-__device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p, int L, int tile_r, int tile_c,
+template <int TR, int TC, bool UJ = false, bool UD = false, int NT = TR * TC, int NS = 1>
+__device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParams &p, int L, int tile_r, int tile_c,
                                          const double *__restrict__ in, double *__restrict__ out,
                                          double *__restrict__ snap)
 {
     // NT threads; threads t < TR * TC own output site t of the tile, all of them share the
-    // stage regions and the tile load
-    constexpr int R = 4, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
+    // stage regions, the intermediate states and the tile load
+    constexpr int R = 4 * NS, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
     static_assert(NT >= TR * TC, "one output site per thread at most");
-    constexpr int MAXE = ((RR - 2) * (RC - 2) + NT - 1) / NT;  // region sites per thread, stage 1
+    constexpr int MAXE = ((RR - 2) * (RC - 2) + NT - 1) / NT;  // region sites per thread, first stage
+    // sites per thread whose state a step produces (the widest: step 0)
+    constexpr int MAXO = ((TR + 8 * (NS - 1)) * (TC + 8 * (NS - 1)) + NT - 1) / NT;
     double(&sS)[3][NR] = sm.S;
     double(&sK)[3][NR] = sm.K;
     const int r0 = tile_r * TR - R, c0 = tile_c * TC - R;
@@ -206,73 +212,96 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC> &sm, const LLGParams &p
     const int own = (ir + R) * RC + (ic + R);
     __syncthreads();
 
-    double ax, ay, az;  // cur = S + k1/6 + k2/3 + k3/3 + k4/6, accumulated left to right
     if (snap && owner) {
         double *o = snap + 3 * ((long)grow * L + gcol);
         o[0] = sS[0][own]; o[1] = sS[1][own]; o[2] = sS[2][own];
     }
 
-    // unrolled: the region width w of each stage is a compile-time constant, so the
-    // e / w, e % w index arithmetic needs no runtime division
+    // unrolled: region widths are compile-time constants, so the e / w, e % w index
+    // arithmetic needs no runtime division
 #pragma unroll
-    for (int stage = 1; stage <= 4; stage++) {
-        // increments on the region within radius R - stage of the interior
-        const int lo = stage, hr = RR - stage, hc = RC - stage;
-        const int w = hc - lo, nsite = (hr - lo) * w;
-        // Y_stage at LDS index i (llg_sqt_fusion.c:111, :121, :131)
-        auto Y = [&](int d, int i) -> double {
-            if (stage == 1) return sS[d][i];
-            if (stage == 4) return sS[d][i] + sK[d][i];
-            return sS[d][i] + sK[d][i] / 2.0;
-        };
-        double kr[MAXE][3];
+    for (int m = 0; m < NS; m++) {
+        // this step's new states: the region within radius RO of the tile (RO = 0 in the
+        // last step: output site o is then thread o's own site)
+        const int RO = 4 * (NS - 1 - m), OW = TC + 2 * RO, NO = (TR + 2 * RO) * OW;
+        double A[MAXO][3];  // cur = S + k1/6 + k2/3 + k3/3 + k4/6, accumulated left to right
 #pragma unroll
-        for (int j = 0; j < MAXE; j++) {
-            const int e = t + j * NT;
-            if (e < nsite) {
-                const int idx = (lo + e / w) * RC + lo + e % w;
-                double n[4][3];
-                const int nb[4] = {idx + RC, idx - RC, idx - 1, idx + 1};
+        for (int stage = 1; stage <= 4; stage++) {
+            // increments on the region within radius R - 4 m - stage of the interior
+            const int lo = 4 * m + stage, hr = RR - lo, hc = RC - lo;
+            const int w = hc - lo, nsite = (hr - lo) * w;
+            // Y_stage at LDS index i (llg_sqt_fusion.c:111, :121, :131)
+            auto Y = [&](int d, int i) -> double {
+                if (stage == 1) return sS[d][i];
+                if (stage == 4) return sS[d][i] + sK[d][i];
+                return sS[d][i] + sK[d][i] / 2.0;
+            };
+            double kr[MAXE][3];
 #pragma unroll
-                for (int q = 0; q < 4; q++) {
-                    n[q][0] = Y(0, nb[q]); n[q][1] = Y(1, nb[q]); n[q][2] = Y(2, nb[q]);
+            for (int j = 0; j < MAXE; j++) {
+                const int e = t + j * NT;
+                if (e < nsite) {
+                    const int idx = (lo + e / w) * RC + lo + e % w;
+                    double n[4][3];
+                    const int nb[4] = {idx + RC, idx - RC, idx - 1, idx + 1};
+#pragma unroll
+                    for (int q = 0; q < 4; q++) {
+                        n[q][0] = Y(0, nb[q]); n[q][1] = Y(1, nb[q]); n[q][2] = Y(2, nb[q]);
+                    }
+                    double fx, fy, fz;
+                    llg_f<UJ, UD>(p, Y(0, idx), Y(1, idx), Y(2, idx), n[0], n[1], n[2], n[3], fx, fy, fz);
+                    kr[j][0] = fx * p.dt; kr[j][1] = fy * p.dt; kr[j][2] = fz * p.dt;
                 }
-                double fx, fy, fz;
-                llg_f<UJ, UD>(p, Y(0, idx), Y(1, idx), Y(2, idx), n[0], n[1], n[2], n[3], fx, fy, fz);
-                kr[j][0] = fx * p.dt; kr[j][1] = fy * p.dt; kr[j][2] = fz * p.dt;
             }
-        }
-        __syncthreads();  // every read of the previous increment is done
+            __syncthreads();  // every read of the previous increment is done
 #pragma unroll
-        for (int j = 0; j < MAXE; j++) {
-            const int e = t + j * NT;
-            if (e < nsite) {
-                const int idx = (lo + e / w) * RC + lo + e % w;
-                sK[0][idx] = kr[j][0]; sK[1][idx] = kr[j][1]; sK[2][idx] = kr[j][2];
+            for (int j = 0; j < MAXE; j++) {
+                const int e = t + j * NT;
+                if (e < nsite) {
+                    const int idx = (lo + e / w) * RC + lo + e % w;
+                    sK[0][idx] = kr[j][0]; sK[1][idx] = kr[j][1]; sK[2][idx] = kr[j][2];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < MAXO; j++) {
+                const int o = t + j * NT;
+                if (o >= NO) continue;
+                const int idx = (R - RO + o / OW) * RC + (R - RO) + o % OW;
+                const double kx = sK[0][idx], ky = sK[1][idx], kz = sK[2][idx];
+                double &ax = A[j][0], &ay = A[j][1], &az = A[j][2];
+                if (stage == 1) {
+                    ax = add_div(sS[0][idx], kx, p.six, kSixth);
+                    ay = add_div(sS[1][idx], ky, p.six, kSixth);
+                    az = add_div(sS[2][idx], kz, p.six, kSixth);
+                } else if (stage == 4) {
+                    ax = add_div(ax, kx, p.six, kSixth); ay = add_div(ay, ky, p.six, kSixth); az = add_div(az, kz, p.six, kSixth);
+                } else {
+                    ax = add_div(ax, kx, p.three, kThird); ay = add_div(ay, ky, p.three, kThird); az = add_div(az, kz, p.three, kThird);
+                }
             }
         }
-        __syncthreads();
-        if (!tile_site) continue;
-        const double kx = sK[0][own], ky = sK[1][own], kz = sK[2][own];
-        if (stage == 1) {
-            ax = add_div(sS[0][own], kx, p.six, kSixth);
-            ay = add_div(sS[1][own], ky, p.six, kSixth);
-            az = add_div(sS[2][own], kz, p.six, kSixth);
-        } else if (stage == 4) {
-            ax = add_div(ax, kx, p.six, kSixth); ay = add_div(ay, ky, p.six, kSixth); az = add_div(az, kz, p.six, kSixth);
-        } else {
-            ax = add_div(ax, kx, p.three, kThird); ay = add_div(ay, ky, p.three, kThird); az = add_div(az, kz, p.three, kThird);
+        if (m + 1 < NS) {
+            // the next step's input: every read of S in this step came before the last
+            // stage's first barrier
+#pragma unroll
+            for (int j = 0; j < MAXO; j++) {
+                const int o = t + j * NT;
+                if (o >= NO) continue;
+                const int idx = (R - RO + o / OW) * RC + (R - RO) + o % OW;
+                renorm(p.len, A[j][0], A[j][1], A[j][2], sS[0][idx], sS[1][idx], sS[2][idx]);
+            }
+            __syncthreads();
+        } else if (owner) {
+            double *o = out + 3 * ((long)grow * L + gcol);
+            renorm(p.len, A[0][0], A[0][1], A[0][2], o[0], o[1], o[2]);
         }
-    }
-    if (owner) {
-        double *o = out + 3 * ((long)grow * L + gcol);
-        renorm(p.len, ax, ay, az, o[0], o[1], o[2]);
     }
 }
 
 // single lattice (nrep replicas along blockIdx.z); snapshot slot from *snap_counter
 // when given (graph replay), else slot 0 of snap
-template <int TR, int TC, bool UJ, bool UD, int NT = TR * TC>
+template <int TR, int TC, bool UJ, bool UD, int NT = TR * TC, int NS = 1>
 __global__ __launch_bounds__(NT) void rk4_step_kernel(LLGParams p, int L,
                                                           const double *__restrict__ in,
                                                           double *__restrict__ out,
@@ -280,11 +309,11 @@ __global__ __launch_bounds__(NT) void rk4_step_kernel(LLGParams p, int L,
                                                           const int *__restrict__ snap_counter,
                                                           long snap_stride)
 {
-    __shared__ TileLDS<TR, TC> sm;
+    __shared__ TileLDS<TR, TC, NS> sm;
     const long base = (long)blockIdx.z * L * L * 3;
     double *sn = nullptr;
     if (snap) sn = snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride + base;
-    rk4_tile<TR, TC, UJ, UD, NT>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
+    rk4_tile<TR, TC, UJ, UD, NT, NS>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
 }
 
 // ------------------------------------------------------------------ batching
@@ -622,20 +651,46 @@ int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items
     return ASD_OK;
 }
 
-template <int TR, int TC, int NT>
+template <int TR, int TC, int NT, int NS = 1>
 static void launch_tiles(asd_ctx *ctx, const LLGParams &p, bool uj, bool ud, int L, int nrep, const double *in,
                          double *out, double *snap_out, const int *snap_counter, long snap_stride)
 {
     const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);
-    auto k = uj && ud ? rk4_step_kernel<TR, TC, true, true, NT>
-                      : (uj ? rk4_step_kernel<TR, TC, true, false, NT>
-                            : (ud ? rk4_step_kernel<TR, TC, false, true, NT> : rk4_step_kernel<TR, TC, false, false, NT>));
+    auto k = uj && ud ? rk4_step_kernel<TR, TC, true, true, NT, NS>
+                      : (uj ? rk4_step_kernel<TR, TC, true, false, NT, NS>
+                            : (ud ? rk4_step_kernel<TR, TC, false, true, NT, NS>
+                                  : rk4_step_kernel<TR, TC, false, false, NT, NS>));
     hipLaunchKernelGGL(k, grid, dim3(NT), 0, ctx->stream, p, L, in, out, snap_out, snap_counter, snap_stride);
 }
 
-int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in, double *out,
-                    double *snap_out, const int *snap_counter, long snap_stride, int bump_counter)
+// the automatic choice among the kernels below: streaming strips once there are at least
+// four waves per CU of strips (L >= 64), else LDS tiles by occupancy
+static bool rk4_streams(const asd_ctx *ctx, int L, int nrep)
 {
+    const int R = std::min(120, L - 8);
+    const int nband = L >= 64 ? (L + R - 1) / R : 0, nstrip = (L + kStreamCols - 1) / kStreamCols;
+    const long waves = (long)nrep * nband * nstrip;
+    return L >= 64 && (ctx->rk4_tile == 64 || (ctx->rk4_tile == 0 && waves >= 4L * ctx->num_cu));
+}
+static bool rk4_small_tiles(const asd_ctx *ctx, int L, int nrep)
+{
+    const long n16 = (long)nrep * ((L + 15) / 16) * ((L + 15) / 16);
+    return ctx->rk4_tile == 8 || (ctx->rk4_tile == 0 && n16 < std::max(ctx->num_cu, 1));
+}
+
+// automatic choice only: measured 4.31 -> 4.04 us per step at L = 128 (one lattice), while
+// forced 8 x 8 tiles on a large lattice lose (L = 1024: 62 -> 90 us, the fused halo costs
+// more than the launch it saves)
+int rk4_fusable_steps(const asd_ctx *ctx, int L, int nrep)
+{
+    return ctx->rk4_tile == 0 && !rk4_streams(ctx, L, nrep) && rk4_small_tiles(ctx, L, nrep) ? 2 : 1;
+}
+
+int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in, double *out,
+                    double *snap_out, const int *snap_counter, long snap_stride, int bump_counter, int steps)
+{
+    ASD_CHECK(steps == 1 || (steps == 2 && rk4_fusable_steps(ctx, L, nrep) == 2), "cannot fuse %d steps here",
+              steps);
     // LDS tiles: 16 x 16 (256 threads, 27.6 KB LDS, 5 workgroups per CU; 2.25 sites
     // loaded and 5.7 f-evaluations per output site) measured faster at every size than
     // 32 x 32 (1024 threads, 76.8 KB, 1.56 loads and 4.8 f-evaluations per site), which
@@ -650,8 +705,7 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
     const int R = std::min(120, L - 8);
     const int nband = L >= 64 ? (L + R - 1) / R : 0, nstrip = (L + kStreamCols - 1) / kStreamCols;
     const long waves = (long)nrep * nband * nstrip;
-    const bool stream = L >= 64 && (ctx->rk4_tile == 64 || (ctx->rk4_tile == 0 && waves >= 4L * ctx->num_cu));
-    if (stream) {
+    if (rk4_streams(ctx, L, nrep)) {
         auto k = uj && ud ? rk4_stream_kernel<true, true>
                           : (uj ? rk4_stream_kernel<true, false>
                                 : (ud ? rk4_stream_kernel<false, true> : rk4_stream_kernel<false, false>));
@@ -667,7 +721,9 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
         // four threads per site below four workgroups per CU, then 16 x 16 with one
         const long n16 = (long)nrep * ((L + 15) / 16) * ((L + 15) / 16);
         const int cus = std::max(ctx->num_cu, 1);
-        if (ctx->rk4_tile == 8 || (ctx->rk4_tile == 0 && n16 < cus))
+        if (steps == 2)
+            launch_tiles<8, 8, 256, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+        else if (rk4_small_tiles(ctx, L, nrep))
             launch_tiles<8, 8, 256>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
         else if (ctx->rk4_tile == 0 && n16 < 4L * cus)
             launch_tiles<16, 16, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);

@@ -188,3 +188,32 @@ def test_spin_length_outside_the_kernels_range_is_rejected(gctx, orc):
         gctx.rk4_run(asd_amd.params(J=1.0 / ok, dt=0.01, spin_length=ok), L, s2, 2, 1, 0, None)
         gctx.synchronize()
         assert np.allclose(torch.linalg.norm(s2, dim=1).cpu().numpy() / ok, 1.0, atol=1e-12)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,nrep,kw", [(64, 1, dict(D=0.1, Bz=0.05)), (40, 3, dict(J=-0.7, D=0.05, alpha=0.2)),
+                                       (128, 1, dict(D=0.1))])
+def test_fused_two_step_launches_equal_single_steps(gctx, orc, L, nrep, kw):
+    """On small lattices a launch fuses two RK4 steps (snapshot interval even); with an
+    odd interval every launch is one step.  Both give the same trajectory and the same
+    snapshots at common times, bit for bit, graph-replayed and eager, and the oracle's."""
+    torch = need_gpu()
+    import asd_amd
+    S0 = np.stack([orc.init_spins(21 + r, L * L) for r in range(nrep)])
+    p = asd_amd.params(dt=0.01, **kw)
+    n = 103                                                 # odd: the last launch is one step
+    res = {}
+    for every, graph in ((10, True), (10, False), (5, True), (5, False)):
+        nsnap = n // every + 1
+        s = torch.tensor(S0, device="cuda")
+        snaps = torch.full((nsnap,) + S0.shape, np.nan, dtype=torch.float64, device="cuda")
+        gctx.rk4_run(p, L, s, n, every, nsnap, snaps, use_graph=graph)
+        gctx.synchronize()
+        res[(every, graph)] = (s.cpu().numpy(), snaps.cpu().numpy())
+    ref = res[(5, False)]
+    for key, (s, sn) in res.items():
+        assert np.array_equal(s, ref[0]), key
+        step = key[0] // 5
+        assert np.array_equal(sn, ref[1][::step][: len(sn)]), key
+    S_or, _ = orc.run(orc.params(dt=0.01, **kw), L, S0, n, 0, 0)
+    assert np.array_equal(ref[0], S_or)

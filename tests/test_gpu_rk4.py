@@ -217,3 +217,45 @@ def test_fused_two_step_launches_equal_single_steps(gctx, orc, L, nrep, kw):
         assert np.array_equal(sn, ref[1][::step][: len(sn)]), key
     S_or, _ = orc.run(orc.params(dt=0.01, **kw), L, S0, n, 0, 0)
     assert np.array_equal(ref[0], S_or)
+
+
+@pytest.mark.gpu
+def test_plain_L128_1000_steps_equal_oracle(gctx, orc):
+    """The benchmark lattice (L = 128, J = 1, D = 0.1, seed 1; two-step fused launches,
+    graph-replayed with snapshots every 10) over 1000 steps: trajectory and every snapshot
+    equal to the oracle's."""
+    torch = need_gpu()
+    import asd_amd
+    L, n, every = 128, 1000, 10
+    S0 = orc.init_spins(1, L * L)
+    s = torch.tensor(S0, device="cuda")
+    snaps = torch.empty((n // every, L * L, 3), dtype=torch.float64, device="cuda")
+    gctx.rk4_run(asd_amd.params(J=1.0, D=0.1, dt=0.01), L, s, n, every, n // every, snaps)
+    gctx.synchronize()
+    S_ref, sn_ref = orc.run(orc.params(J=1.0, D=0.1, dt=0.01), L, S0, n, every, n // every)
+    assert np.array_equal(s.cpu().numpy(), S_ref)
+    assert np.array_equal(snaps.cpu().numpy(), sn_ref)
+
+
+@pytest.mark.gpu
+def test_streaming_equals_tiles_on_the_large_batch(gctx, orc):
+    """L = 1024 x 8 replicas (the automatic choice streams): 10 steps equal to the 16 x 16
+    tile kernel's and to the oracle's on the first replica."""
+    torch = need_gpu()
+    import asd_amd
+    L, nrep, n = 1024, 8, 10
+    S0 = torch.tensor(np.stack([orc.init_spins(60 + r, L * L) for r in range(nrep)]), device="cuda")
+    p = asd_amd.params(J=1.0, D=0.1, Bz=0.02, dt=0.01)
+    out = []
+    try:
+        for tile in (0, 16):
+            gctx.rk4_set_tile(tile)
+            s = S0.clone()
+            gctx.rk4_run(p, L, s, n, 10, 0, None, use_graph=False)
+            gctx.synchronize()
+            out.append(s.cpu().numpy())
+    finally:
+        gctx.rk4_set_tile(0)
+    assert np.array_equal(out[0], out[1])
+    S_ref, _ = orc.run(orc.params(J=1.0, D=0.1, Bz=0.02, dt=0.01), L, S0[0].cpu().numpy(), n, 0, 0)
+    assert np.array_equal(out[0][0], S_ref)

@@ -663,11 +663,24 @@ static void launch_tiles(asd_ctx *ctx, const LLGParams &p, bool uj, bool ud, int
     hipLaunchKernelGGL(k, grid, dim3(NT), 0, ctx->stream, p, L, in, out, snap_out, snap_counter, snap_stride);
 }
 
+// rows per band of the streaming kernel: 104 (measured at L = 1024 with 64 / 32 / 8
+// replicas against 56-248: 781 / 401 / 132 us per step, 120 rows 800 / 428 / 144), or 56
+// where 104-row bands would not give 16 waves per CU (8 replicas: 121 us); more, shorter
+// bands balance the CUs better at the cost of 8 halo rows per band
+static int stream_band_rows(const asd_ctx *ctx, int L, int nrep)
+{
+    if (L < 64) return 0;  // no streaming below the 64-column wave
+    const int nstrip = (L + kStreamCols - 1) / kStreamCols;
+    int R = std::min(104, L - 8);
+    if ((long)nrep * ((L + R - 1) / R) * nstrip < 16L * std::max(ctx->num_cu, 1)) R = std::min(56, L - 8);
+    return R;
+}
+
 // the automatic choice among the kernels below: streaming strips once there are at least
 // four waves per CU of strips (L >= 64), else LDS tiles by occupancy
 static bool rk4_streams(const asd_ctx *ctx, int L, int nrep)
 {
-    const int R = std::min(120, L - 8);
+    const int R = stream_band_rows(ctx, L, nrep);
     const int nband = L >= 64 ? (L + R - 1) / R : 0, nstrip = (L + kStreamCols - 1) / kStreamCols;
     const long waves = (long)nrep * nband * nstrip;
     return L >= 64 && (ctx->rk4_tile == 64 || (ctx->rk4_tile == 0 && waves >= 4L * ctx->num_cu));
@@ -702,7 +715,7 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
     // streaming strips: forced (64), or automatic once there are at least four waves per
     // CU (each wave's rows form one serial chain, so small lattices are faster in tiles;
     // measured: L = 1024 x 8 replicas 205 us vs 275 us per step, x 1 replica 114 vs 41)
-    const int R = std::min(120, L - 8);
+    const int R = stream_band_rows(ctx, L, nrep);
     const int nband = L >= 64 ? (L + R - 1) / R : 0, nstrip = (L + kStreamCols - 1) / kStreamCols;
     const long waves = (long)nrep * nband * nstrip;
     if (rk4_streams(ctx, L, nrep)) {

@@ -113,8 +113,9 @@ def test_tile_edge_does_not_change_results(gctx, orc):
 
 
 STREAM_CASES = [
-    # L, nrep, params: strips of 56 columns and bands of min(120, L - 8) rows, ragged at
-    # both ends; L = 48 is below the 64-column wave and takes the tile kernel
+    # L, nrep, params: strips of 56 columns and bands of min(56, L - 8) rows (these small
+    # batches; large ones take 104-row bands), ragged at both ends; L = 48 is below the
+    # 64-column wave and takes the tile kernel
     (64, 2, dict(D=0.1, Bz=0.1, alpha=0.01)),
     (100, 1, dict(D=0.1, Bz=0.05)),
     (130, 3, dict(J=-0.7, D=0.05, Bz=0.2, alpha=0.2, spin_length=1.5)),
@@ -239,11 +240,12 @@ def test_plain_L128_1000_steps_equal_oracle(gctx, orc):
 
 @pytest.mark.gpu
 def test_streaming_equals_tiles_on_the_large_batch(gctx, orc):
-    """L = 1024 x 8 replicas (the automatic choice streams): 10 steps equal to the 16 x 16
+    """L = 1024 x 24 replicas (the automatic choice streams, in 104-row bands: the small
+    batches of the other streaming tests take 56-row bands): 10 steps equal to the 16 x 16
     tile kernel's and to the oracle's on the first replica."""
     torch = need_gpu()
     import asd_amd
-    L, nrep, n = 1024, 8, 10
+    L, nrep, n = 1024, 24, 10
     S0 = torch.tensor(np.stack([orc.init_spins(60 + r, L * L) for r in range(nrep)]), device="cuda")
     p = asd_amd.params(J=1.0, D=0.1, Bz=0.02, dt=0.01)
     out = []

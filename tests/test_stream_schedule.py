@@ -4,7 +4,7 @@ of row t-s at step t, keeps the loaded rows in an 8-slot ring indexed by the row
 mod 4, pads the steps to a multiple of 8 (padding steps store nothing; loads past the
 last step repeat its row), takes column neighbours from the adjacent lanes (whole-wave
 DPP shifts; the edge lane reads 0) and covers the lattice with 56-column strips and
-bands of min(120, L - 8) rows.  This emulates exactly that schedule with one numpy vector per wave (64 lanes),
+bands of min(R, L - 8) rows (R = 104, or 56 for small batches).  This emulates exactly that schedule with one numpy vector per wave (64 lanes),
 in IEEE double with no fused operations, and checks one step against the oracle bit for
 bit, plus that every site is written once and every load stays inside the lattice.
 It pins the schedule; the GPU tests pin the kernel itself (test_gpu_rk4.py).
@@ -47,10 +47,10 @@ def _k(p, c, up, dn):
     return [x * p["dt"] for x in f]
 
 
-def stream_step(p, L, S_in):
+def stream_step(p, L, S_in, band_rows):
     out = np.full((L * L, 3), np.nan)
     writes = np.zeros(L * L, int)
-    R = min(120, L - 8)
+    R = min(band_rows, L - 8)
     lane = np.arange(64)
     for band in range((L + R - 1) // R):
         for strip in range((L + COLS - 1) // COLS):
@@ -104,15 +104,16 @@ def stream_step(p, L, S_in):
     return out, writes
 
 
+@pytest.mark.parametrize("band_rows", [104, 56])
 @pytest.mark.parametrize("L,kw", [(64, dict(D=0.1, Bz=0.1, alpha=0.01)),
                                   (100, dict(J=-0.7, D=0.05, Bz=0.2, alpha=0.2, spin_length=1.5)),
-                                  (130, dict(D=0.1))])   # band of 10 rows: 18 steps padded to 24
-def test_streaming_schedule_reproduces_oracle_step(orc, L, kw):
+                                  (130, dict(D=0.1))])   # ragged last bands: 26 / 18 rows
+def test_streaming_schedule_reproduces_oracle_step(orc, L, kw, band_rows):
     p = dict(J=1.0, D=0.0, Bz=0.0, alpha=0.0, dt=0.01, len=1.0)
     for key, v in kw.items():
         p["len" if key == "spin_length" else key] = v
     S0 = orc.init_spins(11, L * L) * p["len"]
     ref, _ = orc.run(orc.params(dt=0.01, **kw), L, S0, 1, 0, 0)
-    got, writes = stream_step(p, L, S0)
+    got, writes = stream_step(p, L, S0, band_rows)
     assert (writes == 1).all()
     assert np.array_equal(got, ref)

@@ -529,6 +529,29 @@ def main():
                               "floor": "relative to max_q |S(q,tau)| of each lag"}
         log(f"  fp64 pass: {st64}")
 
+    # calibration: the vendor library's plain fp16 GEMM (torch.matmul -> hipBLASLt) at one lag's
+    # shape (Ns x Ns x Kpad, fp16 output, no epilogue) against the split kernel's MFMA work rate
+    # (three products per useful flop, over the padded K)
+    if args.precision in ("fp16x2", "bf16x3"):
+        kpad = (3 * T0 + 63) // 64 * 64
+        xa = torch.randn(ns, kpad, device="cuda", dtype=torch.float16)
+        xb = torch.randn(kpad, ns, device="cuda", dtype=torch.float16)
+        for _ in range(3):
+            xc = xa @ xb
+        torch.cuda.synchronize()
+        nmm = 20
+        t_mm = time.perf_counter()
+        for _ in range(nmm):
+            xc = xa @ xb
+        torch.cuda.synchronize()
+        lib_tf = 2.0 * ns * ns * kpad * nmm / (time.perf_counter() - t_mm) / 1e12
+        ours = 3.0 * gemm_tflops * kpad / (3 * T0)
+        res["library_calibration"] = {"hipblaslt_fp16_gemm_tflops": round(lib_tf, 1), "shape": f"{ns}x{ns}x{kpad}",
+                                      "split_kernel_mfma_tflops": round(ours, 1),
+                                      "ratio": round(ours / lib_tf, 3)}
+        del xa, xb, xc
+        log(f"  library calibration: {res['library_calibration']}")
+
     # the same RK4 step on a lattice batch far larger than the caches (L = 1024, 64
     # replicas: 1.6 GB of state), the HBM-bound regime; the automatic kernel choice
     # takes the register-streaming strips here

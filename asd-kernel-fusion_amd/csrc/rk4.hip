@@ -361,7 +361,8 @@ __global__ __launch_bounds__(TR *TC) void rk4_batch_kernel(const BatchTraj *__re
 // row step of a wave (48 of them the DPP lane shifts), 4 cycles each, for 56 output sites.
 // Moving the lane shifts to the LDS crossbar (ds_bpermute) measured slower (the LDS
 // pipe saturates), and so did a persistent grid of one equal share per resident wave
-// (its outer segment loop costs the fourth wave per SIMD).
+// (its outer segment loop costs the fourth wave per SIMD) and two columns per lane
+// (fewer instructions per site, but two waves per SIMD and a shallower prefetch).
 constexpr int kStreamHalo = 4, kStreamCols = 64 - 2 * kStreamHalo;
 
 // bound_ctrl: the edge lane (a halo lane) receives 0, so no old value is needed

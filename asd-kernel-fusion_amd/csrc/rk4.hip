@@ -44,7 +44,8 @@ LLGParams make_llg(const asd_params &p)
 // UNIT_J (J == 1): n * J is n exactly, so the multiplies are dropped.
 // UNDAMPED (alpha == 0): (p + 0 q) * 1 is p, so the damping term is dropped; the value
 // is the same (IEEE ==), only an exact -0 in p may come out as +0 the other way.
-template <bool UNIT_J = false, bool UNDAMPED = false>
+// NO_BZ (B_z == 0): the add of B_z is dropped (hz + 0.0 is hz, up to the sign of a zero).
+template <bool UNIT_J = false, bool UNDAMPED = false, bool NO_BZ = false>
 __device__ __forceinline__ void llg_f(const LLGParams &p, double sx, double sy, double sz,
                                       const double *n0, const double *n1, const double *n2,
                                       const double *n3, double &fx, double &fy, double &fz)
@@ -66,7 +67,7 @@ __device__ __forceinline__ void llg_f(const LLGParams &p, double sx, double sy, 
         hx += n3[0] * p.J; hy += n3[1] * p.J; hz += n3[2] * p.J;
     }
     hz += 2.0 * p.D * sz;
-    hz += p.Bz;
+    if constexpr (!NO_BZ) hz += p.Bz;
     const double px = hy * sz - hz * sy;
     const double py = hz * sx - hx * sz;
     const double pz = hx * sy - hy * sx;
@@ -123,6 +124,8 @@ __device__ __forceinline__ void renorm(double len, double ax, double ay, double 
 // of which is then exactly the compiler's IEEE division.  Valid while |a|^2 and every
 // nonzero len a_i stay above 2^-900 and below 2^900 (the launcher's parameter range
 // keeps spin_length there).
+// UNIT_LEN (spin_length == 1): the multiply 1 * a (exact) is dropped.
+template <bool UNIT_LEN = false>
 __device__ __forceinline__ void renorm_fast(double len, double ax, double ay, double az, double &ox, double &oy,
                                             double &oz)
 {
@@ -142,7 +145,7 @@ __device__ __forceinline__ void renorm_fast(double len, double ax, double ay, do
     e = __builtin_fma(-nrm, y, 1.0);
     y = __builtin_fma(y, e, y);
     auto one = [&](double a) {
-        const double n = len * a, q = n * y;
+        const double n = UNIT_LEN ? a : len * a, q = n * y;
         const double r = __builtin_fma(-nrm, q, n);
         return __builtin_fma(r, y, q);
     };
@@ -381,7 +384,7 @@ __device__ __forceinline__ double lane_from_right(double v)  // lane l receives 
 
 // k = h f(c) (h = dt, or dt/2 for the halved increment) with row+1 = up, row-1 = dn and
 // the column neighbours from the adjacent lanes
-template <bool UJ, bool UD>
+template <bool UJ, bool UD, bool PL = false>
 __device__ __forceinline__ void stream_k(const LLGParams &p, const double (&c)[3], const double (&up)[3],
                                          const double (&dn)[3], double h, double (&k)[3])
 {
@@ -392,13 +395,14 @@ __device__ __forceinline__ void stream_k(const LLGParams &p, const double (&c)[3
         rt[d] = lane_from_right(c[d]);
     }
     double fx, fy, fz;
-    llg_f<UJ, UD>(p, c[0], c[1], c[2], up, dn, lf, rt, fx, fy, fz);
+    llg_f<UJ, UD, PL>(p, c[0], c[1], c[2], up, dn, lf, rt, fx, fy, fz);
     k[0] = fx * h; k[1] = fy * h; k[2] = fz * h;
 }
 
 // grid: 4 waves per block, one work item (replica, band, strip) per wave; R % 4 == 0,
 // 8 <= R <= L - 8, L >= 64 (checked by the launcher)
-template <bool UJ, bool UD>
+// PL (B_z == 0 and spin_length == 1, the plain workloads): both exact shortcuts above.
+template <bool UJ, bool UD, bool PL>
 __global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int nrep, int R, int nband,
                                                          int nstrip, const double *__restrict__ in,
                                                          double *__restrict__ out, double *__restrict__ snap,
@@ -505,34 +509,34 @@ __global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int
             // Y = S + k/2 is S + kh and k/6, k/3 are the correctly rounded kh/3, kh/1.5.
             double k[3];
             // stage 1 at row t-1: Y1 = S
-            stream_k<UJ, UD>(p, B[b1], B[b0], B[b2], p.half_dt, k);
+            stream_k<UJ, UD, PL>(p, B[b1], B[b0], B[b2], p.half_dt, k);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 A[i1][d][lane] = add_div(B[b1][d], k[d], p.three, kThird);  // + k1/6
                 Y2[i1][d] = B[b1][d] + k[d];                                // S + k1/2
             }
             // stage 2 at row t-2
-            stream_k<UJ, UD>(p, Y2[i2], Y2[i1], Y2[i3], p.half_dt, k);
+            stream_k<UJ, UD, PL>(p, Y2[i2], Y2[i1], Y2[i3], p.half_dt, k);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 A[i2][d][lane] = add_div(A[i2][d][lane], k[d], p.three_halves, kTwoThirds);  // + k2/3
                 Y3[i2][d] = B[b2][d] + k[d];                                                  // S + k2/2
             }
             // stage 3 at row t-3
-            stream_k<UJ, UD>(p, Y3[i3], Y3[i2], Y3[i4], p.dt, k);
+            stream_k<UJ, UD, PL>(p, Y3[i3], Y3[i2], Y3[i4], p.dt, k);
 #pragma unroll
             for (int d = 0; d < 3; d++) {
                 A[i3][d][lane] = add_div(A[i3][d][lane], k[d], p.three, kThird);
                 Y4[i3][d] = B[b3][d] + k[d];
             }
             // stage 4 at row t-4, then the new state of that row
-            stream_k<UJ, UD>(p, Y4[i4], Y4[i3], Y4[i5], p.dt, k);
+            stream_k<UJ, UD, PL>(p, Y4[i4], Y4[i3], Y4[i5], p.dt, k);
             const int orow = t - 2 * kStreamHalo;  // band-relative output row
 #pragma unroll
             for (int d = 0; d < 3; d++) A[i4][d][lane] = add_div(A[i4][d][lane], k[d], p.six, kSixth);
             if (ocol && orow >= 0 && orow < rows) {
                 double *o = at_w(dst + (r0 + orow) * row_stride);
-                renorm_fast(p.len, A[i4][0][lane], A[i4][1][lane], A[i4][2][lane], o[0], o[1], o[2]);
+                renorm_fast<PL>(p.len, A[i4][0][lane], A[i4][1][lane], A[i4][2][lane], o[0], o[1], o[2]);
             }
         }
     }
@@ -720,9 +724,10 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
     const int nband = L >= 64 ? (L + R - 1) / R : 0, nstrip = (L + kStreamCols - 1) / kStreamCols;
     const long waves = (long)nrep * nband * nstrip;
     if (rk4_streams(ctx, L, nrep)) {
-        auto k = uj && ud ? rk4_stream_kernel<true, true>
-                          : (uj ? rk4_stream_kernel<true, false>
-                                : (ud ? rk4_stream_kernel<false, true> : rk4_stream_kernel<false, false>));
+        const bool pl = p.Bz == 0.0 && p.len == 1.0;
+        auto k = uj && ud ? (pl ? rk4_stream_kernel<true, true, true> : rk4_stream_kernel<true, true, false>)
+                          : (uj ? rk4_stream_kernel<true, false, false>
+                                : (ud ? rk4_stream_kernel<false, true, false> : rk4_stream_kernel<false, false, false>));
         hipLaunchKernelGGL(k, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, p, L, nrep, R, nband,
                            nstrip, in, out, snap_out, snap_counter, snap_stride);
     } else if (big) {

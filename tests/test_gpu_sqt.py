@@ -258,3 +258,21 @@ def test_unfused_path_rejects_sizes_it_cannot_hold(gctx):
     big = torch.zeros((3, 1024 * 1024, 3), dtype=torch.float64, device="cuda")
     with pytest.raises(RuntimeError, match="Ns\\^2"):
         gctx.sqt_unfused(1024, big, 2, 1, [(0, 0)])                   # Ns^2 = 1.1e12 doubles
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,T0,nlags", [(9, 12, 4), (33, 10, 3)])
+def test_odd_lattice_full_grid(gctx, orc, L, T0, nlags):
+    """Odd L (the displacement bins wrap by L, the twiddle table has L entries; L = 33 gives
+    ragged tiles on both sides): FP64 against the oracle's GEMM form, FP16X2 within 1e-5."""
+    torch = need_gpu()
+    say(f"sqt odd L={L} T0={T0} nlags={nlags} full grid")
+    snaps = trajectory(orc, L, T0 + nlags - 1, D=0.1, Bz=0.05)
+    q = full_grid(L)
+    ref = orc.sqt_gemm(L, snaps, T0, nlags, q)
+    d = torch.tensor(snaps, device="cuda")
+    e64 = max_rel(gctx.sqt(L, d, T0, nlags, q, "fp64"), ref)
+    mx, rms = err_stats(gctx.sqt(L, d, T0, nlags, q, "fp16x2"), ref)
+    say(f"  fp64 {e64:.3e}; fp16x2 max {mx:.3e} rms {rms:.3e}")
+    assert e64 <= 1e-10
+    assert mx <= 1e-5

@@ -261,3 +261,24 @@ def test_streaming_equals_tiles_on_the_large_batch(gctx, orc):
     assert np.array_equal(out[0], out[1])
     S_ref, _ = orc.run(orc.params(J=1.0, D=0.1, Bz=0.02, dt=0.01), L, S0[0].cpu().numpy(), n, 0, 0)
     assert np.array_equal(out[0][0], S_ref)
+
+
+@pytest.mark.gpu
+def test_zero_and_one_step_and_unused_snapshot_slots(gctx, orc):
+    """nsteps = 0 leaves the state and writes snapshot 0 only; nsteps = 1 with every = 1
+    writes the initial and the stepped state; slots beyond the run stay untouched."""
+    torch = need_gpu()
+    import asd_amd
+    L = 32
+    S0 = orc.init_spins(4, L * L)
+    p = asd_amd.params(D=0.1, dt=0.01)
+    for n, every in ((0, 1), (1, 1), (25, 10)):
+        s = torch.tensor(S0, device="cuda")
+        snaps = torch.full((5, L * L, 3), np.nan, dtype=torch.float64, device="cuda")
+        gctx.rk4_run(p, L, s, n, every, 5, snaps)
+        gctx.synchronize()
+        S_ref, sn_ref = orc.run(orc.params(D=0.1, dt=0.01), L, S0, n, every, n // every + 1)
+        used = n // every + 1
+        assert np.array_equal(s.cpu().numpy(), S_ref), (n, every)
+        assert np.array_equal(snaps[:used].cpu().numpy(), sn_ref), (n, every)
+        assert np.isnan(snaps[used:].cpu().numpy()).all(), (n, every)

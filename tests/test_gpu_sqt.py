@@ -276,3 +276,33 @@ def test_odd_lattice_full_grid(gctx, orc, L, T0, nlags):
     say(f"  fp64 {e64:.3e}; fp16x2 max {mx:.3e} rms {rms:.3e}")
     assert e64 <= 1e-10
     assert mx <= 1e-5
+
+
+@pytest.mark.gpu
+def test_single_time_origin_gives_zero(gctx, orc):
+    """With one time origin the connected term cancels exactly: the window average of
+    F_q(tau) conj F_q(0) is the product of the window means, so S(q, tau) = 0 for every q and
+    lag (FP64 to rounding; FP16X2 to its product error)."""
+    torch = need_gpu()
+    L, nlags = 16, 5
+    snaps = torch.tensor(trajectory(orc, L, nlags, D=0.1), device="cuda")
+    q = full_grid(L)
+    s64 = gctx.sqt(L, snaps, 1, nlags, q, "fp64")
+    s16 = gctx.sqt(L, snaps, 1, nlags, q, "fp16x2")
+    say(f"  T0 = 1: max |S| fp64 {np.abs(s64).max():.2e}, fp16x2 {np.abs(s16).max():.2e}")
+    assert np.abs(s64).max() <= 1e-13
+    assert np.abs(s16).max() <= 1e-5
+
+
+@pytest.mark.gpu
+def test_invalid_sizes_are_rejected(gctx, orc):
+    torch = need_gpu()
+    import asd_amd
+    L = 16
+    snaps = torch.tensor(trajectory(orc, L, 6), device="cuda")
+    q = full_grid(L)
+    for T0, nlags in ((0, 3), (3, 0), (4, 4)):          # the last needs 7 snapshots, 6 given
+        with pytest.raises(asd_amd.AsdError):
+            gctx.sqt(L, snaps, T0, nlags, q, "fp64")
+    with pytest.raises(asd_amd.AsdError):
+        gctx.sqt(L, snaps, 3, 2, [(0, L)], "fp64")       # q index out of [0, L)

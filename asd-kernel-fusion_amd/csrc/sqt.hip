@@ -25,6 +25,7 @@
 #include <cmath>
 #include <map>
 #include <mutex>
+#include <type_traits>
 
 #include "asd_internal.h"
 
@@ -164,6 +165,7 @@ struct GemmArgs {
     long ld;         // k-major: row stride in elements; i-major: row stride (k) in elements
     long copy_stride;  // i-major A: elements between the 8 shifted copies
     int L, Ns, Kpad, nt, nlags;
+    int K;           // k extent without the zero padding (3 T0)
     int group;       // tile-rows per group in the work order
     double *bins;    // [nlags][Ns]
 };
@@ -664,72 +666,86 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
                 for (int e = 0; e < 16; e++) acc2[a][b][e] = 0.0f;
     }
 
-    const int nk = args.Kpad / BK;
+    // k steps over the unpadded K: where the last step's second 16-deep half lies wholly
+    // in the zero padding (K % 32 in 1..16), that step does its first half only.  The
+    // products skipped are exact zeros, so the sums are unchanged.
+    const int nk = (args.K + BK - 1) / BK;
+    const bool half_tail = args.K % BK != 0 && args.K % BK <= 16;
+    const int nk_full = half_tail ? nk - 1 : nk;
+
+    // one k step in ring slot h: wait for this wave's loads of the slot (the loads of the
+    // NS - 2 later stages may stay in flight) and for its LDS reads of the previous step,
+    // barrier, refill the slot the previous step used, then the MFMAs of KH halves
+    auto step = [&](int k, int h, auto khc) {
+        constexpr int KH = decltype(khc)::value;
+        __builtin_amdgcn_s_waitcnt(wait_vmcnt(PER_STAGE * (NS - 2)));
+        __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        issue(min(k + NS - 1, nk - 1), (h + NS - 1) % NS);
+        const uint16_t *sb = lds + h * STAGE;
+        // the reads of the second 16-deep half overlap the MFMAs of the first
+        v_t ah[KH][WM], al[KH][WM], bh[KH][WN], bl[KH][WN];
 #pragma unroll
-    for (int st = 0; st < NS - 1; st++) issue(min(st, nk - 1), st);
-    for (int kt = 0; kt < nk; kt += NS) {
+        for (int kk = 0; kk < KH; kk++) {
+            const int qo = 8 * ((2 * kk + rhalf) ^ rx);
 #pragma unroll
-        for (int h = 0; h < NS; h++) {
-            const int k = kt + h;
-            if (k < nk) {
-                __builtin_amdgcn_s_waitcnt(wait_vmcnt(PER_STAGE * (NS - 2)));
-                __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
-                __builtin_amdgcn_s_barrier();
-                asm volatile("" ::: "memory");
-                issue(min(k + NS - 1, nk - 1), (h + NS - 1) % NS);
-                const uint16_t *sb = lds + h * STAGE;
-                // the reads of the second 16-deep half overlap the MFMAs of the first
-                constexpr int KH = BK / 16;
-                v_t ah[KH][WM], al[KH][WM], bh[KH][WN], bl[KH][WN];
+            for (int m = 0; m < WM; m++) {
+                const int r = wi * WM * 32 + m * 32 + rrow;
+                ah[kk][m] = *reinterpret_cast<const v_t *>(sb + r * BK + qo);
+                al[kk][m] = *reinterpret_cast<const v_t *>(sb + OFF_AL + r * BK + qo);
+            }
 #pragma unroll
-                for (int kk = 0; kk < KH; kk++) {
-                    const int qo = 8 * ((2 * kk + rhalf) ^ rx);
-#pragma unroll
-                    for (int m = 0; m < WM; m++) {
-                        const int r = wi * WM * 32 + m * 32 + rrow;
-                        ah[kk][m] = *reinterpret_cast<const v_t *>(sb + r * BK + qo);
-                        al[kk][m] = *reinterpret_cast<const v_t *>(sb + OFF_AL + r * BK + qo);
-                    }
-#pragma unroll
-                    for (int n = 0; n < WN; n++) {
-                        const int r = wj * WN * 32 + n * 32 + rrow;
-                        bh[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BH + r * BK + qo);
-                        bl[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BL + r * BK + qo);
-                    }
-                }
-#pragma unroll
-                for (int kk = 0; kk < KH; kk++)
-#pragma unroll
-                    for (int m = 0; m < WM; m++)
-#pragma unroll
-                        for (int n = 0; n < WN; n++) {
-                            acc[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bh[kk][n], acc[m][n]);
-                            if constexpr (DUAL) {
-                                acc2[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bl[kk][n], acc2[m][n]);
-                                acc2[m][n] = SplitMfma<PREC>::mfma(al[kk][m], bh[kk][n], acc2[m][n]);
-                            } else {
-                                acc[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bl[kk][n], acc[m][n]);
-                                acc[m][n] = SplitMfma<PREC>::mfma(al[kk][m], bh[kk][n], acc[m][n]);
-                            }
-                        }
-                // issue order: the first half's R reads, then one MFMA and one read of
-                // the second half in turn, then the remaining MFMAs (at most R + 1 reads
-                // in flight, so the LDS counter can wait for exactly the ones needed)
-                {
-                    constexpr int R = 2 * (WM + WN), M = 3 * WM * WN;
-                    static_assert(M >= R, "interleave needs at least R MFMAs per half");
-                    __builtin_amdgcn_sched_group_barrier(0x100, R, 0);
-#pragma unroll
-                    for (int i = 0; i < R; i++) {
-                        __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                        __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
-                    }
-                    __builtin_amdgcn_sched_group_barrier(0x008, 2 * M - R, 0);
-                }
-                asm volatile("" ::: "memory");
+            for (int n = 0; n < WN; n++) {
+                const int r = wj * WN * 32 + n * 32 + rrow;
+                bh[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BH + r * BK + qo);
+                bl[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BL + r * BK + qo);
             }
         }
+#pragma unroll
+        for (int kk = 0; kk < KH; kk++)
+#pragma unroll
+            for (int m = 0; m < WM; m++)
+#pragma unroll
+                for (int n = 0; n < WN; n++) {
+                    acc[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bh[kk][n], acc[m][n]);
+                    if constexpr (DUAL) {
+                        acc2[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bl[kk][n], acc2[m][n]);
+                        acc2[m][n] = SplitMfma<PREC>::mfma(al[kk][m], bh[kk][n], acc2[m][n]);
+                    } else {
+                        acc[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bl[kk][n], acc[m][n]);
+                        acc[m][n] = SplitMfma<PREC>::mfma(al[kk][m], bh[kk][n], acc[m][n]);
+                    }
+                }
+        // issue order: the first half's R reads, then one MFMA and one read of the second
+        // half in turn, then the remaining MFMAs (at most R + 1 reads in flight, so the LDS
+        // counter can wait for exactly the ones needed)
+        {
+            constexpr int R = 2 * (WM + WN), M = 3 * WM * WN;
+            static_assert(M >= R, "interleave needs at least R MFMAs per half");
+            __builtin_amdgcn_sched_group_barrier(0x100, R, 0);
+            if constexpr (KH == 2) {
+#pragma unroll
+                for (int i = 0; i < R; i++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                }
+                __builtin_amdgcn_sched_group_barrier(0x008, 2 * M - R, 0);
+            } else {
+                __builtin_amdgcn_sched_group_barrier(0x008, M, 0);
+            }
+        }
+        asm volatile("" ::: "memory");
+    };
+
+#pragma unroll
+    for (int st = 0; st < NS - 1; st++) issue(min(st, nk - 1), st);
+    for (int kt = 0; kt < nk_full; kt += NS) {
+#pragma unroll
+        for (int h = 0; h < NS; h++)
+            if (kt + h < nk_full) step(kt + h, h, std::integral_constant<int, BK / 16>{});
     }
+    if (half_tail) step(nk - 1, (nk - 1) % NS, std::integral_constant<int, 1>{});
     // every load in flight lands before the bins reuse the LDS
     __builtin_amdgcn_s_waitcnt(wait_vmcnt(0));
     __syncthreads();
@@ -1094,6 +1110,7 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
     ga.Ns = Ns;
     ga.nt = nt_tiles;
     ga.nlags = nlags;
+    ga.K = (int)K;
     ga.bins = ctx->bins.as<double>();
     ga.group = kKmajorCfgs[prec <= ASD_FP32 ? cfg : 0].group;
 

@@ -170,11 +170,11 @@ def test_large_work_list_all_lags_computed(gctx, orc, prec, tol):
     assert err.max() <= tol
 
 
-# Oracle bounds at these small samples (9 or fewer time origins), relative to
+# Oracle bounds at these small samples (5 to 22 time origins), relative to
 # max |S| over all (q, tau).  BF16X3 keeps about 17 significant bits per operand, so
-# with few origins to average over its error reaches ~3e-5; a wrong fragment layout
-# or a dropped tile gives errors of order 1.
-SMALL_SAMPLE_TOL = {"fp64": 1e-10, "fp32": 1e-5, "bf16x3": 1e-4, "fp16x2": 1e-5}
+# with few origins to average over its error reaches 3e-5 to 1.2e-4 (5 origins); a
+# wrong fragment layout or a dropped tile gives errors of order 1.
+SMALL_SAMPLE_TOL = {"fp64": 1e-10, "fp32": 1e-5, "bf16x3": 2e-4, "fp16x2": 1e-5}
 
 
 @pytest.mark.gpu
@@ -183,13 +183,16 @@ def test_every_kernel_config_matches_oracle(gctx, orc, prec):
     """Every tile / wave layout the autotuner may pick gives the oracle's S(q,tau):
     lattices whose site count is not a multiple of any tile edge (400, 1296) and
     one smaller than a tile (144), so partial tiles and masked rows are covered.
+    K = 3 T0 covers one whole 32-deep k step (18, 21, 27), a last step whose second
+    16-deep half lies in the zero padding after zero, one and two whole steps (15, 48,
+    66; the direct-to-LDS kernels do only its first half).
     Each output element is accumulated over k in the same order by every layout,
     so the layouts agree with each other up to the order of the FP64 bin sums."""
     torch = need_gpu()
     import asd_amd
     n = asd_amd.lib().asd_sqt_num_configs(asd_amd.PRECISIONS[prec])
     try:
-        for L, T0, nlags in ((20, 9, 5), (36, 6, 4), (12, 7, 3)):
+        for L, T0, nlags in ((20, 9, 5), (36, 6, 4), (12, 7, 3), (20, 5, 4), (36, 16, 3), (12, 22, 3)):
             snaps = trajectory(orc, L, T0 + nlags - 1, D=0.1, Bz=0.05)
             q = gxmg(L)
             ref = orc.sqt_gemm(L, snaps, T0, nlags, q)

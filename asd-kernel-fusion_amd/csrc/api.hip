@@ -131,10 +131,10 @@ int asd_create(asd_ctx **out, int device)
     ASD_HIP(hipGetDeviceCount(&n));
     ASD_CHECK(device >= 0 && device < n, "device %d out of range (%d devices)", device, n);
     ASD_HIP(hipSetDevice(device));
-    asd_ctx *c = new asd_ctx();
-    c->device = device;
     hipDeviceProp_t prop;
     ASD_HIP(hipGetDeviceProperties(&prop, device));
+    asd_ctx *c = new asd_ctx();
+    c->device = device;
     c->num_cu = prop.multiProcessorCount;
     *out = c;
     return ASD_OK;
@@ -149,13 +149,8 @@ int asd_destroy(asd_ctx *c)
     if (c->bgraph.exec) (void)hipGraphExecDestroy(c->bgraph.exec);
     for (hipEvent_t &e : c->ev)
         if (e) (void)hipEventDestroy(e);
-    DevBuf *bufs[] = {&c->state_b, &c->counter, &c->opA, &c->opB, &c->opA2, &c->opB2, &c->bins, &c->fq,
-                      &c->twiddle, &c->qdev, &c->outdev, &c->stats, &c->c0, &c->c1, &c->c2, &c->c3, &c->c4,
-                      &c->c5, &c->c6, &c->work, &c->tabs, &c->batch_b, &c->adj0, &c->adj1, &c->adj2, &c->adj3, &c->adj4,
-                      &c->adj5, &c->ref0, &c->ref1, &c->ref2, &c->ref3, &c->ref4, &c->ref5};
     blas_destroy(c);
-    for (DevBuf *b : bufs) b->release();
-    delete c;
+    delete c;  // every DevBuf member frees its memory
     return ASD_OK;
 }
 
@@ -423,6 +418,7 @@ int asd_sqt_replicas(asd_ctx *c, int L, int nrep, const double *d_now, const dou
     ASD_CHECK(c && d_now && d_zero && out, "NULL argument");
     ASD_TRY(check_L(L));
     ASD_CHECK(nrep >= 1, "nrep must be >= 1");
+    ASD_CHECK(prec >= ASD_FP64 && prec <= ASD_FP16X2, "unknown precision");
     ASD_TRY(check_q(L, h_q, nq));
     ASD_HIP(hipSetDevice(c->device));
     ASD_TRY(c->outdev.ensure(sizeof(double) * 2 * (long)nq));

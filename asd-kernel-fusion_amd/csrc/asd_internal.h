@@ -33,10 +33,14 @@ int set_error(int code, const char *fmt, ...);
         if (!(cond)) return ::asd::set_error(ASD_EINVAL, __VA_ARGS__);                 \
     } while (0)
 
-// a device buffer that grows on demand and is reused across calls
+// a device buffer that grows on demand and is reused across calls; freed with its owner
 struct DevBuf {
     void *ptr = nullptr;
     size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
     int ensure(size_t want);
     void release();
     template <class T> T *as() const { return static_cast<T *>(ptr); }
@@ -96,7 +100,7 @@ struct asd_ctx {
     // autotuner choice per precision (-1 = automatic); tuned per (L, K)
     int forced_cfg[4] = {-1, -1, -1, -1};
     std::map<std::pair<int, int>, int> tuned[4];
-    // RK4 tile edge: 0 = automatic, 16 or 32
+    // RK4 tile edge: 0 = automatic; 8, 16 or 32 (LDS tiles) or 64 (register streaming)
     int rk4_tile = 0;
     // 1: S(q,tau) always takes the per-q path (the adjoint needs its F_q(t) sums)
     int sqt_generic_only = 0;

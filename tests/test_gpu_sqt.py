@@ -309,3 +309,8 @@ def test_invalid_sizes_are_rejected(gctx, orc):
             gctx.sqt(L, snaps, T0, nlags, q, "fp64")
     with pytest.raises(asd_amd.AsdError):
         gctx.sqt(L, snaps, 3, 2, [(0, L)], "fp64")       # q index out of [0, L)
+    for prec in (-1, 4):                                 # no such precision
+        with pytest.raises(asd_amd.AsdError):
+            gctx.sqt(L, snaps, 3, 2, q, prec)
+        with pytest.raises(asd_amd.AsdError):
+            gctx.sqt_replicas(L, snaps[0], snaps[1], q, prec)

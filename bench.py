@@ -530,11 +530,12 @@ def main():
         log(f"  fp64 pass: {st64}")
 
     # calibration: the vendor library's plain fp16 GEMM (torch.matmul -> hipBLASLt) at one lag's
-    # shape (Ns x Ns x K, fp16 output, no epilogue) against the split kernel's MFMA work rate
-    # (three products per useful flop, over the K its MFMAs cover: K rounded up to the
+    # shape with K padded to a multiple of 64 (the library's faster shape: 16384 x 16384 x 1200
+    # runs ~15% below x 1216), fp16 output, no epilogue, against the split kernel's MFMA work
+    # rate (three products per useful flop, over the K its MFMAs cover: K rounded up to the
     # MFMA depth of 16)
     if args.precision in ("fp16x2", "bf16x3"):
-        kpad = 3 * T0
+        kpad = (3 * T0 + 63) // 64 * 64
         kexec = (3 * T0 + 15) // 16 * 16
         xa = torch.randn(ns, kpad, device="cuda", dtype=torch.float16)
         xb = torch.randn(kpad, ns, device="cuda", dtype=torch.float16)

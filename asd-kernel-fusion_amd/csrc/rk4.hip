@@ -16,6 +16,8 @@
 // differ only in the sign of an exact zero.  Exact shortcuts: the field sums start from
 // -0.0 (the additive identity) where the reference starts from 0.0, J == 1 drops the
 // multiplies by J, and alpha == 0 drops the damping term.
+#include <type_traits>
+
 #include "asd_internal.h"
 
 namespace asd {
@@ -302,6 +304,121 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
     }
 }
 
+// The same NS steps of one tile where every site of the widest stage region (radius
+// R - 1 around the tile) has a thread of its own (NT >= (RR - 2) (RC - 2): the
+// latency-bound small-lattice shapes).  A thread keeps its site's partial sum
+// S + k1/6 + ... in registers across the four stages, so a stage is: read the neighbours'
+// stage inputs, write this site's increment into the other of two LDS increment arrays,
+// update the own sum, one barrier (rk4_tile needs two, and a second pass over the output
+// region reading the increments back).  Stages 1 and 2 store the halved increment
+// kh = f (dt/2), which is k/2 exactly (RN(f dt)/2 = RN(f dt/2) while k is above the
+// subnormal range), so Y = S + k/2 is S + kh and k1/6, k2/3 are the correctly rounded
+// kh/3, kh/1.5; the renormalisation is renorm_fast (|S|^2 within 2^+-900 of 1, kept by
+// the parameter check).  Every site therefore gets the values rk4_tile gives it.
+template <int TR, int TC, int NS> struct TileLDS2 {
+    static constexpr int R = 4 * NS, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
+    double S[3][NR], K[2][3][NR];
+};
+
+template <int TR, int TC, bool UJ, bool UD, int NT, int NS>
+__device__ __forceinline__ void rk4_tile_fixed(TileLDS2<TR, TC, NS> &sm, const LLGParams &p, int L, int tile_r,
+                                               int tile_c, const double *__restrict__ in, double *__restrict__ out,
+                                               double *__restrict__ snap)
+{
+    constexpr int R = 4 * NS, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
+    constexpr int W = RC - 2, NW = (RR - 2) * W;  // the widest stage region
+    static_assert(NW <= NT, "one region site per thread");
+    double(&sS)[3][NR] = sm.S;
+    const int r0 = tile_r * TR - R, c0 = tile_c * TC - R;
+    const int t = threadIdx.x;
+
+    const bool near = L >= RR;
+    for (int idx = t; idx < NR; idx += NT) {
+        int gr = r0 + idx / RC, gc = c0 + idx % RC;
+        if (near) {
+            gr += gr < 0 ? L : (gr >= L ? -L : 0);
+            gc += gc < 0 ? L : (gc >= L ? -L : 0);
+        } else {
+            gr = wrap(gr, L);
+            gc = wrap(gc, L);
+        }
+        const double *s = in + 3 * ((long)gr * L + gc);
+        sS[0][idx] = s[0]; sS[1][idx] = s[1]; sS[2][idx] = s[2];
+    }
+    __syncthreads();
+    if (snap && t < TR * TC) {  // coalesced: thread t writes tile site t
+        const int ir = t / TC, ic = t % TC, grow = tile_r * TR + ir, gcol = tile_c * TC + ic;
+        if (grow < L && gcol < L) {
+            const int own = (ir + R) * RC + (ic + R);
+            double *o = snap + 3 * ((long)grow * L + gcol);
+            o[0] = sS[0][own]; o[1] = sS[1][own]; o[2] = sS[2][own];
+        }
+    }
+
+    // this thread's site: row er, column ec of the LDS tile, at depth (distance from the
+    // tile's edge) `depth`; the region of stage s of step m is depth >= 4 m + s
+    const bool has = t < NW;
+    const int er = has ? 1 + t / W : 0, ec = has ? 1 + t % W : 0;
+    const int idx = er * RC + ec;
+    const int depth = has ? min(min(er, RR - 1 - er), min(ec, RC - 1 - ec)) : -1;
+    const int grow = tile_r * TR + er - R, gcol = tile_c * TC + ec - R;
+    const bool owner = depth >= R && grow < L && gcol < L;
+
+#pragma unroll
+    for (int m = 0; m < NS; m++) {
+        // sites whose new state this step produces: the stage-4 region (depth >= 4 m + 4)
+        const bool outreg = depth >= 4 * m + 4;
+        double ax = 0.0, ay = 0.0, az = 0.0;  // S + k1/6 + k2/3 + k3/3 + k4/6, left to right
+#pragma unroll
+        for (int stage = 1; stage <= 4; stage++) {
+            // stage s reads the increment array stage s - 1 wrote and writes the other
+            const double(&Kin)[3][NR] = sm.K[stage & 1];
+            double(&Kout)[3][NR] = sm.K[(stage - 1) & 1];
+            if (depth >= 4 * m + stage) {
+                auto Y = [&](int d, int i) -> double { return stage == 1 ? sS[d][i] : sS[d][i] + Kin[d][i]; };
+                double n[4][3];
+                const int nb[4] = {idx + RC, idx - RC, idx - 1, idx + 1};
+#pragma unroll
+                for (int q = 0; q < 4; q++) {
+                    n[q][0] = Y(0, nb[q]); n[q][1] = Y(1, nb[q]); n[q][2] = Y(2, nb[q]);
+                }
+                double fx, fy, fz;
+                llg_f<UJ, UD>(p, Y(0, idx), Y(1, idx), Y(2, idx), n[0], n[1], n[2], n[3], fx, fy, fz);
+                const double h = stage <= 2 ? p.half_dt : p.dt;
+                const double kx = fx * h, ky = fy * h, kz = fz * h;
+                if (stage < 4) {
+                    Kout[0][idx] = kx; Kout[1][idx] = ky; Kout[2][idx] = kz;
+                }
+                if (outreg) {
+                    if (stage == 1) {  // kh/3 = k1/6
+                        ax = add_div(sS[0][idx], kx, p.three, kThird);
+                        ay = add_div(sS[1][idx], ky, p.three, kThird);
+                        az = add_div(sS[2][idx], kz, p.three, kThird);
+                    } else if (stage == 2) {  // kh/1.5 = k2/3
+                        ax = add_div(ax, kx, p.three_halves, kTwoThirds);
+                        ay = add_div(ay, ky, p.three_halves, kTwoThirds);
+                        az = add_div(az, kz, p.three_halves, kTwoThirds);
+                    } else if (stage == 3) {
+                        ax = add_div(ax, kx, p.three, kThird); ay = add_div(ay, ky, p.three, kThird); az = add_div(az, kz, p.three, kThird);
+                    } else {
+                        ax = add_div(ax, kx, p.six, kSixth); ay = add_div(ay, ky, p.six, kSixth); az = add_div(az, kz, p.six, kSixth);
+                    }
+                }
+            }
+            // the last stage of the last step reads nothing from LDS afterwards
+            if (stage < 4 || m + 1 < NS) __syncthreads();
+        }
+        if (m + 1 < NS) {
+            // the next step's input (every read of S in this step came before the barrier)
+            if (outreg) renorm_fast(p.len, ax, ay, az, sS[0][idx], sS[1][idx], sS[2][idx]);
+            __syncthreads();
+        } else if (owner) {
+            double *o = out + 3 * ((long)grow * L + gcol);
+            renorm_fast(p.len, ax, ay, az, o[0], o[1], o[2]);
+        }
+    }
+}
+
 // single lattice (nrep replicas along blockIdx.z); snapshot slot from *snap_counter
 // when given (graph replay), else slot 0 of snap
 template <int TR, int TC, bool UJ, bool UD, int NT = TR * TC, int NS = 1>
@@ -312,11 +429,16 @@ __global__ __launch_bounds__(NT) void rk4_step_kernel(LLGParams p, int L,
                                                           const int *__restrict__ snap_counter,
                                                           long snap_stride)
 {
-    __shared__ TileLDS<TR, TC, NS> sm;
+    // one thread per site of the widest stage region where the block has enough threads
+    constexpr bool kFixed = (TR + 8 * NS - 2) * (TC + 8 * NS - 2) <= NT;
+    __shared__ std::conditional_t<kFixed, TileLDS2<TR, TC, NS>, TileLDS<TR, TC, NS>> sm;
     const long base = (long)blockIdx.z * L * L * 3;
     double *sn = nullptr;
     if (snap) sn = snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride + base;
-    rk4_tile<TR, TC, UJ, UD, NT, NS>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
+    if constexpr (kFixed)
+        rk4_tile_fixed<TR, TC, UJ, UD, NT, NS>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
+    else
+        rk4_tile<TR, TC, UJ, UD, NT, NS>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
 }
 
 // ------------------------------------------------------------------ batching

@@ -16,8 +16,6 @@
 // differ only in the sign of an exact zero.  Exact shortcuts: the field sums start from
 // -0.0 (the additive identity) where the reference starts from 0.0, J == 1 drops the
 // multiplies by J, and alpha == 0 drops the damping term.
-#include <type_traits>
-
 #include "asd_internal.h"
 
 namespace asd {
@@ -161,20 +159,27 @@ __device__ __forceinline__ int wrap(int x, int L)
     return x < 0 ? x + L : x;
 }
 
-// NS full RK4 steps (+ renormalisation) of one TR x TC output tile.  LDS holds the tile
-// + a radius-4 NS halo twice, structure-of-arrays: S and the latest increment K.  The
-// stage input Y_s = S + c k_{s-1} (c = 1/2, 1/2, 1) is formed when read -- the same
-// expression the reference stores, so the same rounding -- which keeps LDS at two arrays
-// (77 KB for 32 x 32 in one step: two workgroups per CU overlap each other's load
-// latency and barriers).  Step m of NS produces the new state of the region within
-// radius 4 (NS - 1 - m) of the tile, back into S for the next step, and the last step
-// the tile itself; every site goes through the same operations as in a one-step launch,
-// so fusing steps changes no bit.  Fusing pays where one step is latency-bound: two
-// steps load and store once.  Each thread keeps its sites' new increments in registers
-// until every read of the previous K is done.
+// NS full RK4 steps (+ renormalisation) of one TR x TC output tile.  The tile and a
+// radius-4 NS halo are loaded once; every RK stage widens the dependency by one site, so
+// stage s of step m is evaluated on the sites at depth >= 4 m + s (depth: distance from
+// the edge of the loaded tile), and step m produces new states for depth >= 4 m + 4 (the
+// last step: the tile itself).  Thread t owns the sites t, t + NT, .. of the widest stage
+// region (depth >= 1) for all stages and steps, and keeps their states, stage inputs and
+// partial sums S + k1/6 + k2/3 + k3/3 + k4/6 (accumulated left to right) in registers.
+// LDS holds two arrays of stage inputs: the owner of a site forms its next stage input
+// Y = S + kh (or S + k3) once and stores it into the array the current stage does not
+// read, so a stage reads the four neighbours' Y (12 doubles a site; forming Y from S and
+// the increment on read would take 30), writes one Y and takes one barrier.  The states
+// start in the second array (the stage-1 inputs), which stage 2 then overwrites.
+// Stages 1 and 2 use the halved increment kh = f (dt/2), which is k/2 exactly
+// (RN(f dt)/2 = RN(f dt/2) while k is above the subnormal range), so Y = S + k/2 is
+// S + kh and k1/6, k2/3 are the correctly rounded kh/3, kh/1.5; the renormalisation is
+// renorm_fast (|S|^2 within 2^+-900 of 1, kept by the parameter check).  Every site goes
+// through the same operations whatever the tile shape and NS, so fusing steps changes no
+// bit.  Fusing pays where one step is latency-bound: two steps load and store once.
 template <int TR, int TC, int NS = 1> struct TileLDS {
     static constexpr int R = 4 * NS, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
-    double S[3][NR], K[3][NR];
+    double Y[2][3][NR];
 };
 
 template <int TR, int TC, bool UJ = false, bool UD = false, int NT = TR * TC, int NS = 1>
@@ -182,156 +187,16 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
                                          const double *__restrict__ in, double *__restrict__ out,
                                          double *__restrict__ snap)
 {
-    // NT threads; threads t < TR * TC own output site t of the tile, all of them share the
-    // stage regions, the intermediate states and the tile load
     constexpr int R = 4 * NS, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
-    static_assert(NT >= TR * TC, "one output site per thread at most");
-    constexpr int MAXE = ((RR - 2) * (RC - 2) + NT - 1) / NT;  // region sites per thread, first stage
-    // sites per thread whose state a step produces (the widest: step 0)
-    constexpr int MAXO = ((TR + 8 * (NS - 1)) * (TC + 8 * (NS - 1)) + NT - 1) / NT;
-    double(&sS)[3][NR] = sm.S;
-    double(&sK)[3][NR] = sm.K;
+    constexpr int W = RC - 2, NW = (RR - 2) * W;  // the widest stage region
+    constexpr int NE = (NW + NT - 1) / NT;        // its sites per thread
+    static_assert(NT >= TR * TC, "the snapshot writer takes one tile site per thread");
+    double(&sS)[3][NR] = sm.Y[1];
     const int r0 = tile_r * TR - R, c0 = tile_c * TC - R;
     const int t = threadIdx.x;
 
     // tile coordinates lie in [-R, L + RR); when L >= RR one correction wraps them,
     // which avoids a runtime integer division per loaded site
-    const bool near = L >= RR;
-    for (int idx = t; idx < NR; idx += NT) {
-        int gr = r0 + idx / RC, gc = c0 + idx % RC;
-        if (near) {
-            gr += gr < 0 ? L : (gr >= L ? -L : 0);
-            gc += gc < 0 ? L : (gc >= L ? -L : 0);
-        } else {
-            gr = wrap(gr, L);
-            gc = wrap(gc, L);
-        }
-        const double *s = in + 3 * ((long)gr * L + gc);
-        sS[0][idx] = s[0]; sS[1][idx] = s[1]; sS[2][idx] = s[2];
-    }
-    // this thread's interior site
-    const bool tile_site = t < TR * TC;
-    const int ir = tile_site ? t / TC : 0, ic = tile_site ? t % TC : 0;
-    const int grow = tile_r * TR + ir, gcol = tile_c * TC + ic;
-    const bool owner = tile_site && grow < L && gcol < L;
-    const int own = (ir + R) * RC + (ic + R);
-    __syncthreads();
-
-    if (snap && owner) {
-        double *o = snap + 3 * ((long)grow * L + gcol);
-        o[0] = sS[0][own]; o[1] = sS[1][own]; o[2] = sS[2][own];
-    }
-
-    // unrolled: region widths are compile-time constants, so the e / w, e % w index
-    // arithmetic needs no runtime division
-#pragma unroll
-    for (int m = 0; m < NS; m++) {
-        // this step's new states: the region within radius RO of the tile (RO = 0 in the
-        // last step: output site o is then thread o's own site)
-        const int RO = 4 * (NS - 1 - m), OW = TC + 2 * RO, NO = (TR + 2 * RO) * OW;
-        double A[MAXO][3];  // cur = S + k1/6 + k2/3 + k3/3 + k4/6, accumulated left to right
-#pragma unroll
-        for (int stage = 1; stage <= 4; stage++) {
-            // increments on the region within radius R - 4 m - stage of the interior
-            const int lo = 4 * m + stage, hr = RR - lo, hc = RC - lo;
-            const int w = hc - lo, nsite = (hr - lo) * w;
-            // Y_stage at LDS index i (llg_sqt_fusion.c:111, :121, :131)
-            auto Y = [&](int d, int i) -> double {
-                if (stage == 1) return sS[d][i];
-                if (stage == 4) return sS[d][i] + sK[d][i];
-                return sS[d][i] + sK[d][i] / 2.0;
-            };
-            double kr[MAXE][3];
-#pragma unroll
-            for (int j = 0; j < MAXE; j++) {
-                const int e = t + j * NT;
-                if (e < nsite) {
-                    const int idx = (lo + e / w) * RC + lo + e % w;
-                    double n[4][3];
-                    const int nb[4] = {idx + RC, idx - RC, idx - 1, idx + 1};
-#pragma unroll
-                    for (int q = 0; q < 4; q++) {
-                        n[q][0] = Y(0, nb[q]); n[q][1] = Y(1, nb[q]); n[q][2] = Y(2, nb[q]);
-                    }
-                    double fx, fy, fz;
-                    llg_f<UJ, UD>(p, Y(0, idx), Y(1, idx), Y(2, idx), n[0], n[1], n[2], n[3], fx, fy, fz);
-                    kr[j][0] = fx * p.dt; kr[j][1] = fy * p.dt; kr[j][2] = fz * p.dt;
-                }
-            }
-            __syncthreads();  // every read of the previous increment is done
-#pragma unroll
-            for (int j = 0; j < MAXE; j++) {
-                const int e = t + j * NT;
-                if (e < nsite) {
-                    const int idx = (lo + e / w) * RC + lo + e % w;
-                    sK[0][idx] = kr[j][0]; sK[1][idx] = kr[j][1]; sK[2][idx] = kr[j][2];
-                }
-            }
-            __syncthreads();
-#pragma unroll
-            for (int j = 0; j < MAXO; j++) {
-                const int o = t + j * NT;
-                if (o >= NO) continue;
-                const int idx = (R - RO + o / OW) * RC + (R - RO) + o % OW;
-                const double kx = sK[0][idx], ky = sK[1][idx], kz = sK[2][idx];
-                double &ax = A[j][0], &ay = A[j][1], &az = A[j][2];
-                if (stage == 1) {
-                    ax = add_div(sS[0][idx], kx, p.six, kSixth);
-                    ay = add_div(sS[1][idx], ky, p.six, kSixth);
-                    az = add_div(sS[2][idx], kz, p.six, kSixth);
-                } else if (stage == 4) {
-                    ax = add_div(ax, kx, p.six, kSixth); ay = add_div(ay, ky, p.six, kSixth); az = add_div(az, kz, p.six, kSixth);
-                } else {
-                    ax = add_div(ax, kx, p.three, kThird); ay = add_div(ay, ky, p.three, kThird); az = add_div(az, kz, p.three, kThird);
-                }
-            }
-        }
-        if (m + 1 < NS) {
-            // the next step's input: every read of S in this step came before the last
-            // stage's first barrier
-#pragma unroll
-            for (int j = 0; j < MAXO; j++) {
-                const int o = t + j * NT;
-                if (o >= NO) continue;
-                const int idx = (R - RO + o / OW) * RC + (R - RO) + o % OW;
-                renorm(p.len, A[j][0], A[j][1], A[j][2], sS[0][idx], sS[1][idx], sS[2][idx]);
-            }
-            __syncthreads();
-        } else if (owner) {
-            double *o = out + 3 * ((long)grow * L + gcol);
-            renorm(p.len, A[0][0], A[0][1], A[0][2], o[0], o[1], o[2]);
-        }
-    }
-}
-
-// The same NS steps of one tile where every site of the widest stage region (radius
-// R - 1 around the tile) has a thread of its own (NT >= (RR - 2) (RC - 2): the
-// latency-bound small-lattice shapes).  A thread keeps its site's partial sum
-// S + k1/6 + ... in registers across the four stages, so a stage is: read the neighbours'
-// stage inputs, write this site's increment into the other of two LDS increment arrays,
-// update the own sum, one barrier (rk4_tile needs two, and a second pass over the output
-// region reading the increments back).  Stages 1 and 2 store the halved increment
-// kh = f (dt/2), which is k/2 exactly (RN(f dt)/2 = RN(f dt/2) while k is above the
-// subnormal range), so Y = S + k/2 is S + kh and k1/6, k2/3 are the correctly rounded
-// kh/3, kh/1.5; the renormalisation is renorm_fast (|S|^2 within 2^+-900 of 1, kept by
-// the parameter check).  Every site therefore gets the values rk4_tile gives it.
-template <int TR, int TC, int NS> struct TileLDS2 {
-    static constexpr int R = 4 * NS, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
-    double S[3][NR], K[2][3][NR];
-};
-
-template <int TR, int TC, bool UJ, bool UD, int NT, int NS>
-__device__ __forceinline__ void rk4_tile_fixed(TileLDS2<TR, TC, NS> &sm, const LLGParams &p, int L, int tile_r,
-                                               int tile_c, const double *__restrict__ in, double *__restrict__ out,
-                                               double *__restrict__ snap)
-{
-    constexpr int R = 4 * NS, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
-    constexpr int W = RC - 2, NW = (RR - 2) * W;  // the widest stage region
-    static_assert(NW <= NT, "one region site per thread");
-    double(&sS)[3][NR] = sm.S;
-    const int r0 = tile_r * TR - R, c0 = tile_c * TC - R;
-    const int t = threadIdx.x;
-
     const bool near = L >= RR;
     for (int idx = t; idx < NR; idx += NT) {
         int gr = r0 + idx / RC, gc = c0 + idx % RC;
@@ -355,53 +220,66 @@ __device__ __forceinline__ void rk4_tile_fixed(TileLDS2<TR, TC, NS> &sm, const L
         }
     }
 
-    // this thread's site: row er, column ec of the LDS tile, at depth (distance from the
-    // tile's edge) `depth`; the region of stage s of step m is depth >= 4 m + s
-    const bool has = t < NW;
-    const int er = has ? 1 + t / W : 0, ec = has ? 1 + t % W : 0;
-    const int idx = er * RC + ec;
-    const int depth = has ? min(min(er, RR - 1 - er), min(ec, RC - 1 - ec)) : -1;
-    const int grow = tile_r * TR + er - R, gcol = tile_c * TC + ec - R;
-    const bool owner = depth >= R && grow < L && gcol < L;
+    // this thread's sites: LDS index, depth, state
+    int idx[NE], depth[NE];
+    double S[NE][3];
+#pragma unroll
+    for (int j = 0; j < NE; j++) {
+        const int e = t + j * NT;
+        const bool has = e < NW;
+        const int er = has ? 1 + e / W : 1, ec = has ? 1 + e % W : 1;
+        idx[j] = er * RC + ec;
+        depth[j] = has ? min(min(er, RR - 1 - er), min(ec, RC - 1 - ec)) : -1;
+        S[j][0] = sS[0][idx[j]]; S[j][1] = sS[1][idx[j]]; S[j][2] = sS[2][idx[j]];
+    }
 
 #pragma unroll
     for (int m = 0; m < NS; m++) {
-        // sites whose new state this step produces: the stage-4 region (depth >= 4 m + 4)
-        const bool outreg = depth >= 4 * m + 4;
-        double ax = 0.0, ay = 0.0, az = 0.0;  // S + k1/6 + k2/3 + k3/3 + k4/6, left to right
+        double A[NE][3], y[NE][3];  // partial sums, stage inputs
+#pragma unroll
+        for (int j = 0; j < NE; j++)
+#pragma unroll
+            for (int d = 0; d < 3; d++) {
+                A[j][d] = 0.0;
+                y[j][d] = S[j][d];
+            }
 #pragma unroll
         for (int stage = 1; stage <= 4; stage++) {
-            // stage s reads the increment array stage s - 1 wrote and writes the other
-            const double(&Kin)[3][NR] = sm.K[stage & 1];
-            double(&Kout)[3][NR] = sm.K[(stage - 1) & 1];
-            if (depth >= 4 * m + stage) {
-                auto Y = [&](int d, int i) -> double { return stage == 1 ? sS[d][i] : sS[d][i] + Kin[d][i]; };
+            // stage 1 reads the states (array 1), stage s > 1 the inputs stage s - 1 wrote
+            const double(&Yin)[3][NR] = sm.Y[stage & 1];
+            double(&Yout)[3][NR] = sm.Y[(stage - 1) & 1];
+#pragma unroll
+            for (int j = 0; j < NE; j++) {
+                if (depth[j] < 4 * m + stage) continue;
+                const int i = idx[j];
                 double n[4][3];
-                const int nb[4] = {idx + RC, idx - RC, idx - 1, idx + 1};
+                const int nb[4] = {i + RC, i - RC, i - 1, i + 1};  // row+1, row-1, col-1, col+1
 #pragma unroll
                 for (int q = 0; q < 4; q++) {
-                    n[q][0] = Y(0, nb[q]); n[q][1] = Y(1, nb[q]); n[q][2] = Y(2, nb[q]);
+                    n[q][0] = Yin[0][nb[q]]; n[q][1] = Yin[1][nb[q]]; n[q][2] = Yin[2][nb[q]];
                 }
                 double fx, fy, fz;
-                llg_f<UJ, UD>(p, Y(0, idx), Y(1, idx), Y(2, idx), n[0], n[1], n[2], n[3], fx, fy, fz);
+                llg_f<UJ, UD>(p, y[j][0], y[j][1], y[j][2], n[0], n[1], n[2], n[3], fx, fy, fz);
                 const double h = stage <= 2 ? p.half_dt : p.dt;
-                const double kx = fx * h, ky = fy * h, kz = fz * h;
-                if (stage < 4) {
-                    Kout[0][idx] = kx; Kout[1][idx] = ky; Kout[2][idx] = kz;
+                const double k[3] = {fx * h, fy * h, fz * h};
+                if (stage < 4) {  // the next stage's input: S + kh (stages 1, 2) or S + k3
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        y[j][d] = S[j][d] + k[d];
+                        Yout[d][i] = y[j][d];
+                    }
                 }
-                if (outreg) {
-                    if (stage == 1) {  // kh/3 = k1/6
-                        ax = add_div(sS[0][idx], kx, p.three, kThird);
-                        ay = add_div(sS[1][idx], ky, p.three, kThird);
-                        az = add_div(sS[2][idx], kz, p.three, kThird);
-                    } else if (stage == 2) {  // kh/1.5 = k2/3
-                        ax = add_div(ax, kx, p.three_halves, kTwoThirds);
-                        ay = add_div(ay, ky, p.three_halves, kTwoThirds);
-                        az = add_div(az, kz, p.three_halves, kTwoThirds);
-                    } else if (stage == 3) {
-                        ax = add_div(ax, kx, p.three, kThird); ay = add_div(ay, ky, p.three, kThird); az = add_div(az, kz, p.three, kThird);
-                    } else {
-                        ax = add_div(ax, kx, p.six, kSixth); ay = add_div(ay, ky, p.six, kSixth); az = add_div(az, kz, p.six, kSixth);
+                if (depth[j] >= 4 * m + 4) {
+#pragma unroll
+                    for (int d = 0; d < 3; d++) {
+                        if (stage == 1)
+                            A[j][d] = add_div(S[j][d], k[d], p.three, kThird);  // kh/3 = k1/6
+                        else if (stage == 2)
+                            A[j][d] = add_div(A[j][d], k[d], p.three_halves, kTwoThirds);  // kh/1.5 = k2/3
+                        else if (stage == 3)
+                            A[j][d] = add_div(A[j][d], k[d], p.three, kThird);
+                        else
+                            A[j][d] = add_div(A[j][d], k[d], p.six, kSixth);
                     }
                 }
             }
@@ -409,12 +287,24 @@ __device__ __forceinline__ void rk4_tile_fixed(TileLDS2<TR, TC, NS> &sm, const L
             if (stage < 4 || m + 1 < NS) __syncthreads();
         }
         if (m + 1 < NS) {
-            // the next step's input (every read of S in this step came before the barrier)
-            if (outreg) renorm_fast(p.len, ax, ay, az, sS[0][idx], sS[1][idx], sS[2][idx]);
+            // the next step's states, into array 1 (last read by stage 3)
+#pragma unroll
+            for (int j = 0; j < NE; j++) {
+                if (depth[j] < 4 * m + 4) continue;
+                renorm_fast(p.len, A[j][0], A[j][1], A[j][2], S[j][0], S[j][1], S[j][2]);
+                sS[0][idx[j]] = S[j][0]; sS[1][idx[j]] = S[j][1]; sS[2][idx[j]] = S[j][2];
+            }
             __syncthreads();
-        } else if (owner) {
-            double *o = out + 3 * ((long)grow * L + gcol);
-            renorm_fast(p.len, ax, ay, az, o[0], o[1], o[2]);
+        } else {
+#pragma unroll
+            for (int j = 0; j < NE; j++) {
+                if (depth[j] < R) continue;
+                const int e = t + j * NT, er = 1 + e / W, ec = 1 + e % W;
+                const int grow = tile_r * TR + er - R, gcol = tile_c * TC + ec - R;
+                if (grow >= L || gcol >= L) continue;
+                double *o = out + 3 * ((long)grow * L + gcol);
+                renorm_fast(p.len, A[j][0], A[j][1], A[j][2], o[0], o[1], o[2]);
+            }
         }
     }
 }
@@ -429,16 +319,11 @@ __global__ __launch_bounds__(NT) void rk4_step_kernel(LLGParams p, int L,
                                                           const int *__restrict__ snap_counter,
                                                           long snap_stride)
 {
-    // one thread per site of the widest stage region where the block has enough threads
-    constexpr bool kFixed = (TR + 8 * NS - 2) * (TC + 8 * NS - 2) <= NT;
-    __shared__ std::conditional_t<kFixed, TileLDS2<TR, TC, NS>, TileLDS<TR, TC, NS>> sm;
+    __shared__ TileLDS<TR, TC, NS> sm;
     const long base = (long)blockIdx.z * L * L * 3;
     double *sn = nullptr;
     if (snap) sn = snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride + base;
-    if constexpr (kFixed)
-        rk4_tile_fixed<TR, TC, UJ, UD, NT, NS>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
-    else
-        rk4_tile<TR, TC, UJ, UD, NT, NS>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
+    rk4_tile<TR, TC, UJ, UD, NT, NS>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
 }
 
 // ------------------------------------------------------------------ batching

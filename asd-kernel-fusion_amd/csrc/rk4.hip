@@ -703,9 +703,9 @@ static bool rk4_small_tiles(const asd_ctx *ctx, int L, int nrep)
     return ctx->rk4_tile == 8 || (ctx->rk4_tile == 0 && n16 < std::max(ctx->num_cu, 1));
 }
 
-// automatic choice only: measured 4.31 -> 4.04 us per step at L = 128 (one lattice), while
-// forced 8 x 8 tiles on a large lattice lose (L = 1024: 62 -> 90 us, the fused halo costs
-// more than the launch it saves)
+// automatic choice only (L = 128, one lattice: 3.0 us per step against 3.6 for single
+// steps); fused steps are not offered for forced 8 x 8 tiles, which on a large lattice lose
+// (the fused halo costs more than the launch it saves)
 int rk4_fusable_steps(const asd_ctx *ctx, int L, int nrep)
 {
     return ctx->rk4_tile == 0 && !rk4_streams(ctx, L, nrep) && rk4_small_tiles(ctx, L, nrep) ? 2 : 1;
@@ -716,11 +716,9 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
 {
     ASD_CHECK(steps == 1 || (steps == 2 && rk4_fusable_steps(ctx, L, nrep) == 2), "cannot fuse %d steps here",
               steps);
-    // LDS tiles: 16 x 16 (256 threads, 27.6 KB LDS, 5 workgroups per CU; 2.25 sites
-    // loaded and 5.7 f-evaluations per output site) measured faster at every size than
-    // 32 x 32 (1024 threads, 76.8 KB, 1.56 loads and 4.8 f-evaluations per site), which
-    // the 128-register cap of 1024-thread workgroups limits to one workgroup per CU.
-    // 32 x 32 stays selectable (asd_rk4_set_tile).
+    // LDS tiles: 16 x 16 (256 threads, 27.6 KB LDS; 2.25 sites loaded and 5.7
+    // f-evaluations per output site) or 32 x 32 (1024 threads, 76.8 KB, two workgroups per
+    // CU; 1.56 loads and 4.8 f-evaluations per site); the automatic choice below
     const bool big = ctx->rk4_tile == 32;
     // J == 1 and alpha == 0 (the plain workloads) take the specialised arithmetic
     const bool uj = p.J == 1.0, ud = p.alpha == 0.0;
@@ -740,20 +738,28 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
     } else if (big) {
         launch_tiles<32, 32, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
     } else {
-        // tiles by occupancy (measured at L = 128 / 256 / 1024, one lattice: 8 x 8 with 256
-        // threads 4.8 us vs 6.7 us for 16 x 16; 16 x 16 with 1024 threads 6.3 vs 7.2 us;
-        // 16 x 16 with 256 threads 38 us, the others 39-62 us): 8 x 8 tiles (four threads per
-        // output site) while 16 x 16 tiles would not give every CU a workgroup, 16 x 16 with
-        // four threads per site below four workgroups per CU, then 16 x 16 with one
+        // tiles by occupancy, n16 / n32 tiles of 16 x 16 / 32 x 32 in the launch (per step,
+        // one lattice unless stated): 8 x 8 tiles (four threads per output site) while 16 x 16
+        // tiles would not give every CU a workgroup (L = 128: 3.6 us, 16 x 16 with 256
+        // threads 5.3); 32 x 32 from half a tile to two tiles per CU (L = 384 / 448 / 512 /
+        // 640: 7.6 / 8.2 / 8.0 / 14.2 us against 8.4 / 11.4 / 10.4 / 15.5 for 16 x 16; 96 x 16,
+        // 192 x 4, 128 x 16, 256 x 4 replicas 17-28% faster; at n32 = CUs / 2, 128 x 8
+        // replicas, 16 x 16 with 1024 threads 6.4 us against 7.4); 16 x 16 with four threads
+        // per site below four workgroups per CU (L = 256: 4.7 us against 5.8 and 6.9), then
+        // 16 x 16 with one (L = 768: 18.4 us against 19.5; L = 1024: 29 us against 27 and
+        // 47 for streaming strips)
         const long n16 = (long)nrep * ((L + 15) / 16) * ((L + 15) / 16);
+        const long n32 = (long)nrep * ((L + 31) / 32) * ((L + 31) / 32);
         const int cus = std::max(ctx->num_cu, 1);
         // two fused steps: 8 x 8 tiles with 512 threads, one site of the widest stage region
-        // (22 x 22) per thread (L = 128, 8000 steps: 32.3 ms against 35.0 with 256 threads; 4 x 8
-        // tiles 36.6 ms)
+        // (22 x 22) per thread (L = 128, 8000 steps with snapshots: 25.9 ms; 3.0 us per step
+        // against 3.6 for single steps)
         if (steps == 2)
             launch_tiles<8, 8, 512, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
         else if (rk4_small_tiles(ctx, L, nrep))
             launch_tiles<8, 8, 256>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+        else if (ctx->rk4_tile == 0 && 2 * n32 > cus && n32 <= 2L * cus)
+            launch_tiles<32, 32, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
         else if (ctx->rk4_tile == 0 && n16 < 4L * cus)
             launch_tiles<16, 16, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
         else

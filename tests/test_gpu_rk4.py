@@ -151,6 +151,27 @@ def test_streaming_kernel_bit_exact(gctx, orc, L, nrep, kw):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("L,nrep", [(100, 12), (128, 8)])
+def test_automatic_tile_shapes_bit_exact(gctx, orc, L, nrep):
+    """Replica batches whose automatic choice is 32 x 32 tiles (L = 100 x 12: 192 tiles of
+    32 x 32, ragged: 100 = 3 x 32 + 4) and 16 x 16 tiles with four threads per site
+    (L = 128 x 8) give the oracle's trajectory and snapshots bit for bit."""
+    torch = need_gpu()
+    import asd_amd
+    kw = dict(J=0.9, D=0.1, Bz=0.05, alpha=0.03)
+    say(f"rk4 automatic tiles L={L} nrep={nrep} {kw}")
+    S0 = np.stack([orc.init_spins(21 + r, L * L) for r in range(nrep)])
+    nsteps, every, nsnap = 30, 10, 3
+    S_ref, snaps_ref = orc.run(orc.params(dt=0.01, **kw), L, S0, nsteps, every, nsnap)
+    spins = torch.tensor(S0, device="cuda")
+    snaps = torch.zeros((nsnap,) + S0.shape, dtype=torch.float64, device="cuda")
+    gctx.rk4_run(asd_amd.params(dt=0.01, **kw), L, spins, nsteps, every, nsnap, snaps)
+    gctx.synchronize()
+    assert np.array_equal(spins.cpu().numpy(), S_ref)
+    assert np.array_equal(snaps.cpu().numpy(), snaps_ref)
+
+
+@pytest.mark.gpu
 def test_automatic_choice_takes_streaming_for_many_replicas(gctx, orc):
     """At L = 128 x 180 replicas (1080 waves) the automatic choice runs the streaming
     strips; the result equals both forced kernels bit for bit."""

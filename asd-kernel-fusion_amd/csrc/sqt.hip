@@ -621,31 +621,38 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
     const uint16_t *Al = static_cast<const uint16_t *>(args.A2) + acopy;
     const uint16_t *Bh = static_cast<const uint16_t *>(args.B) + (long)j0 * ld;
     const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (long)j0 * ld;
-    // per-lane source offsets of this wave's chunks (chunk c = wave * C + u: rows 16 c ..)
-    long srcA[CA], srcB[CB];
-#pragma unroll
-    for (int u = 0; u < CA; u++) srcA[u] = (long)(16 * (wave * CA + u) + lrow) * ld + 8 * lq;
-#pragma unroll
-    for (int u = 0; u < CB; u++) srcB[u] = (long)(16 * (wave * CB + u) + lrow) * ld + 8 * lq;
+    // The loads are buffer loads into LDS (MUBUF), not global ones: the compiler then counts
+    // them in vmcnt only, and can wait for single LDS fragment reads (behind a pending
+    // global_load_lds it waits for every LDS read in flight).  One resource per operand
+    // array with the tile's first row as its base (the host keeps BM rows of the operand
+    // below 2 GiB); a chunk's source is that base + a wave-uniform row / k offset (SGPR) +
+    // this lane's byte offset (row lrow, quarter lq: the same in every chunk; VGPR).
+    const __amdgpu_buffer_rsrc_t rAh = __builtin_amdgcn_make_buffer_rsrc((void *)Ah, 0, 0x7fffffff, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rAl = __builtin_amdgcn_make_buffer_rsrc((void *)Al, 0, 0x7fffffff, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rBh = __builtin_amdgcn_make_buffer_rsrc((void *)Bh, 0, 0x7fffffff, 0x00020000);
+    const __amdgpu_buffer_rsrc_t rBl = __builtin_amdgcn_make_buffer_rsrc((void *)Bl, 0, 0x7fffffff, 0x00020000);
+    const int lane_off = (int)(2 * (lrow * ld + 8 * lq));
 
     auto issue = [&](int kt, int st) {
-        const long k0 = (long)kt * BK;
+        const int k0 = 2 * kt * BK;  // bytes
         uint16_t *base = lds + st * STAGE;
 #pragma unroll
         for (int u = 0; u < CA; u++) {
             const int c = (wave * CA + u) * 512;
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(Ah + srcA[u] + k0),
-                                             (__attribute__((address_space(3))) void *)(base + c), 16, 0, 0);
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(Al + srcA[u] + k0),
-                                             (__attribute__((address_space(3))) void *)(base + OFF_AL + c), 16, 0, 0);
+            const int so = (int)(2 * 16 * (wave * CA + u) * ld) + k0;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rAh, (__attribute__((address_space(3))) void *)(base + c), 16,
+                                                     lane_off, so, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rAl, (__attribute__((address_space(3))) void *)(base + OFF_AL + c),
+                                                     16, lane_off, so, 0, 0);
         }
 #pragma unroll
         for (int u = 0; u < CB; u++) {
             const int c = (wave * CB + u) * 512;
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(Bh + srcB[u] + k0),
-                                             (__attribute__((address_space(3))) void *)(base + OFF_BH + c), 16, 0, 0);
-            __builtin_amdgcn_global_load_lds((const __attribute__((address_space(1))) void *)(Bl + srcB[u] + k0),
-                                             (__attribute__((address_space(3))) void *)(base + OFF_BL + c), 16, 0, 0);
+            const int so = (int)(2 * 16 * (wave * CB + u) * ld) + k0;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rBh, (__attribute__((address_space(3))) void *)(base + OFF_BH + c),
+                                                     16, lane_off, so, 0, 0);
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rBl, (__attribute__((address_space(3))) void *)(base + OFF_BL + c),
+                                                     16, lane_off, so, 0, 0);
         }
     };
 
@@ -684,56 +691,66 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
         asm volatile("" ::: "memory");
         issue(min(k + NS - 1, nk - 1), (h + NS - 1) % NS);
         const uint16_t *sb = lds + h * STAGE;
-        // the reads of the second 16-deep half overlap the MFMAs of the first
+        // reads in the order the MFMAs use them: per 16-deep half A hi, B hi (the hh
+        // products), B lo (hl), A lo (lh)
         v_t ah[KH][WM], al[KH][WM], bh[KH][WN], bl[KH][WN];
 #pragma unroll
         for (int kk = 0; kk < KH; kk++) {
             const int qo = 8 * ((2 * kk + rhalf) ^ rx);
 #pragma unroll
-            for (int m = 0; m < WM; m++) {
-                const int r = wi * WM * 32 + m * 32 + rrow;
-                ah[kk][m] = *reinterpret_cast<const v_t *>(sb + r * BK + qo);
-                al[kk][m] = *reinterpret_cast<const v_t *>(sb + OFF_AL + r * BK + qo);
-            }
+            for (int m = 0; m < WM; m++)
+                ah[kk][m] = *reinterpret_cast<const v_t *>(sb + (wi * WM * 32 + m * 32 + rrow) * BK + qo);
 #pragma unroll
-            for (int n = 0; n < WN; n++) {
-                const int r = wj * WN * 32 + n * 32 + rrow;
-                bh[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BH + r * BK + qo);
-                bl[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BL + r * BK + qo);
-            }
+            for (int n = 0; n < WN; n++)
+                bh[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BH + (wj * WN * 32 + n * 32 + rrow) * BK + qo);
+#pragma unroll
+            for (int n = 0; n < WN; n++)
+                bl[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BL + (wj * WN * 32 + n * 32 + rrow) * BK + qo);
+#pragma unroll
+            for (int m = 0; m < WM; m++)
+                al[kk][m] = *reinterpret_cast<const v_t *>(sb + OFF_AL + (wi * WM * 32 + m * 32 + rrow) * BK + qo);
         }
+        // per half: every hh product, then every hl, then every lh (each accumulator still
+        // takes hh, hl, lh of a half in that order), so the first MFMAs wait only for the
+        // first A hi and B hi reads
 #pragma unroll
-        for (int kk = 0; kk < KH; kk++)
+        for (int kk = 0; kk < KH; kk++) {
+#pragma unroll
+            for (int m = 0; m < WM; m++)
+#pragma unroll
+                for (int n = 0; n < WN; n++) acc[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bh[kk][n], acc[m][n]);
 #pragma unroll
             for (int m = 0; m < WM; m++)
 #pragma unroll
                 for (int n = 0; n < WN; n++) {
-                    acc[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bh[kk][n], acc[m][n]);
-                    if constexpr (DUAL) {
+                    if constexpr (DUAL)
                         acc2[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bl[kk][n], acc2[m][n]);
-                        acc2[m][n] = SplitMfma<PREC>::mfma(al[kk][m], bh[kk][n], acc2[m][n]);
-                    } else {
+                    else
                         acc[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bl[kk][n], acc[m][n]);
-                        acc[m][n] = SplitMfma<PREC>::mfma(al[kk][m], bh[kk][n], acc[m][n]);
-                    }
                 }
-        // issue order: the first half's R reads, then one MFMA and one read of the second
-        // half in turn, then the remaining MFMAs (at most R + 1 reads in flight, so the LDS
-        // counter can wait for exactly the ones needed)
-        {
-            constexpr int R = 2 * (WM + WN), M = 3 * WM * WN;
-            static_assert(M >= R, "interleave needs at least R MFMAs per half");
-            __builtin_amdgcn_sched_group_barrier(0x100, R, 0);
-            if constexpr (KH == 2) {
 #pragma unroll
-                for (int i = 0; i < R; i++) {
-                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
-                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+            for (int m = 0; m < WM; m++)
+#pragma unroll
+                for (int n = 0; n < WN; n++) {
+                    if constexpr (DUAL)
+                        acc2[m][n] = SplitMfma<PREC>::mfma(al[kk][m], bh[kk][n], acc2[m][n]);
+                    else
+                        acc[m][n] = SplitMfma<PREC>::mfma(al[kk][m], bh[kk][n], acc[m][n]);
                 }
-                __builtin_amdgcn_sched_group_barrier(0x008, 2 * M - R, 0);
-            } else {
-                __builtin_amdgcn_sched_group_barrier(0x008, M, 0);
+        }
+        // issue order: the first WM + WN reads (A hi, B hi), then one MFMA and one read in
+        // turn until every read is issued, then the remaining MFMAs; the LDS counter waits
+        // for single reads (the loads into LDS are buffer loads)
+        {
+            constexpr int R0 = WM + WN, RT = 2 * KH * (WM + WN), MT = 3 * KH * WM * WN;
+            static_assert(MT >= RT - R0, "interleave needs an MFMA per read");
+            __builtin_amdgcn_sched_group_barrier(0x100, R0, 0);
+#pragma unroll
+            for (int i = 0; i < RT - R0; i++) {
+                __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
             }
+            if constexpr (MT > RT - R0) __builtin_amdgcn_sched_group_barrier(0x008, MT - (RT - R0), 0);
         }
         asm volatile("" ::: "memory");
     };
@@ -1145,6 +1162,8 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
     } else {
         const long Kpad = round_up(K, 64);  // an even number of 32-wide k steps
         const long ldk = round_up(3L * (nlags - 1) + Kpad + 8, 8);  // halves per i-row
+        // the direct-to-LDS kernels address a tile's operand rows through 32-bit byte offsets
+        ASD_CHECK(2L * 256 * ldk + 2L * Kpad < (1L << 31), "too many lags for the split GEMM (%d)", nlags);
         const long cs = ldk * ldi;                                   // one shifted copy
         ASD_TRY(ctx->opA.ensure(2 * 8 * cs));
         ASD_TRY(ctx->opA2.ensure(2 * 8 * cs));

@@ -15,7 +15,10 @@
 // divisions), so trajectories equal the FP64 CPU restatement's (IEEE ==); bits can
 // differ only in the sign of an exact zero.  Exact shortcuts: the field sums start from
 // -0.0 (the additive identity) where the reference starts from 0.0, J == 1 drops the
-// multiplies by J, and alpha == 0 drops the damping term.
+// multiplies by J, alpha == 0 drops the damping term, stages 1 and 2 use the halved
+// increment f dt/2 (k/2 exactly), the quotients k/6, k/3 and the renormalisation take
+// FMA-corrected reciprocals, and the square root and division drop their range wrappers
+// (|S|^2 stays within 2^+-900 of 1).
 #include "asd_internal.h"
 
 namespace asd {
@@ -361,11 +364,11 @@ __global__ __launch_bounds__(TR *TC) void rk4_batch_kernel(const BatchTraj *__re
 // the same lane's registers and its col-1 / col+1 inputs in the neighbouring lanes (DPP
 // whole-wave shifts).  No barriers; LDS holds only each lane's own partial sums.  Each
 // site is read once and written once (plus the halo: 64/56 columns, (R+8)/R rows).
-// Every site gets the values of rk4_tile's expressions (llg_f, Y = S + k/2.0 or S + k,
-// k/6.0 + k/3.0 + k/3.0 + k/6.0 left to right, len * a / |a|), several through exact
-// rewrites that need fewer instructions (halved increments, FMA-corrected quotients,
-// the square root and division without their range wrappers); the results equal the
-// reference's (IEEE ==) and differ in bits at most in the sign of an exact zero.
+// Every site gets the values of the reference's expressions (llg_f, Y = S + k/2.0 or
+// S + k, k/6.0 + k/3.0 + k/3.0 + k/6.0 left to right, len * a / |a|) through the same
+// exact rewrites as rk4_tile (halved increments, FMA-corrected quotients, the square
+// root and division without their range wrappers); the results equal the reference's
+// (IEEE ==) and differ in bits at most in the sign of an exact zero.
 //
 // The kernel is bound by FP64 VALU issue, not by HBM: about 236 VALU instructions per
 // row step of a wave (48 of them the DPP lane shifts), 4 cycles each, for 56 output sites.

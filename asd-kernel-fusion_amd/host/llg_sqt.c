@@ -16,7 +16,10 @@
  * -S steps with a snapshot every -E steps, then S(q,tau) for -G lags over -T
  * origins on the Gamma-X-M-Gamma path (or the full grid with -F) in precision -p,
  * and writes S(q,tau) as raw little-endian doubles [lag][q][re,im] to -O plus a
- * JSON header next to it (<file>.json).
+ * JSON header next to it (<file>.json).  -W file writes the snapshots the same way
+ * (doubles [snapshot][site][x,y,z], site = row * L + col, plus <file>.json); -R file
+ * reads such a file instead of integrating (its size must match -n, -S and -E), so
+ * S(q,tau) can be recomputed from stored trajectories, on another host or precision.
  */
 #define _POSIX_C_SOURCE 200809L
 #include <stdint.h>
@@ -71,9 +74,9 @@ int main(int argc, char **argv)
     int every = 10, T0 = 400, nlags = 400;
     double D = 0.0, Bz = 0.0, alpha = 0.0, len = 1.0;
     uint64_t seed = 0;
-    const char *out_timing = "mi355x_fusion_sqt.txt", *out_sqt = NULL;
+    const char *out_timing = "mi355x_fusion_sqt.txt", *out_sqt = NULL, *out_snaps = NULL, *in_snaps = NULL;
     int ch;
-    while ((ch = getopt(argc, argv, "n:m:f:e:t:x:y:QPFS:E:T:G:p:J:D:B:a:l:s:o:O:h")) != -1) {
+    while ((ch = getopt(argc, argv, "n:m:f:e:t:x:y:QPFS:E:T:G:p:J:D:B:a:l:s:o:O:W:R:h")) != -1) {
         switch (ch) {
         case 'n': N = atoi(optarg); break;
         case 'm': MCsweep = atoi(optarg); break;
@@ -98,10 +101,13 @@ int main(int argc, char **argv)
         case 's': seed = strtoull(optarg, NULL, 10); break;
         case 'o': out_timing = optarg; break;
         case 'O': out_sqt = optarg; break;
+        case 'W': out_snaps = optarg; break;
+        case 'R': in_snaps = optarg; break;
         default:
             fprintf(stderr,
                     "usage: %s [-n L -m MCsweep -f Nfile -e dt -t trd -x vx -y vy -Q]  (reference mode)\n"
-                    "       %s -P [-n L -S steps -E every -T T0 -G lags -p fp64|fp32|bf16x3|fp16x2 -F]\n"
+                    "       %s -P [-n L -S steps -E every -T T0 -G lags -p fp64|fp32|bf16x3|fp16x2 -F\n"
+                    "              -W snapshot_file | -R snapshot_file]\n"
                     "       common: -J J -D D -B Bz -a alpha -l spin_length -s seed -o timing_file -O sqt_file\n",
                     argv[0], argv[0]);
             return 2;
@@ -181,15 +187,57 @@ int main(int argc, char **argv)
         CHECK(asd_malloc(ctx, sizeof(double) * 3 * ns * nsnap, (void **)&d_snaps));
         CHECK(asd_memcpy(ctx, d_spins, h, sizeof(double) * 3 * ns, 0));
         double *out = malloc(sizeof(double) * 2 * nq * (size_t)nlags);
+        const size_t snap_doubles = 3 * (size_t)ns * nsnap;
         double t0 = now_s();
-        CHECK(asd_rk4_run(ctx, &p, N, 1, d_spins, steps, every, nsnap, d_snaps, 1));
+        if (in_snaps) {
+            /* stored snapshots instead of the integration */
+            FILE *fi = fopen(in_snaps, "rb");
+            if (!fi) { perror(in_snaps); return 1; }
+            double *hs = malloc(sizeof(double) * snap_doubles);
+            const size_t got = fread(hs, sizeof(double), snap_doubles, fi);
+            const int extra = fgetc(fi) != EOF;
+            fclose(fi);
+            if (got != snap_doubles || extra) {
+                fprintf(stderr, "%s: expected %zu doubles (L=%d, %d snapshots)\n", in_snaps, snap_doubles, N, nsnap);
+                return 1;
+            }
+            CHECK(asd_memcpy(ctx, d_snaps, hs, sizeof(double) * snap_doubles, 0));
+            free(hs);
+        } else {
+            CHECK(asd_rk4_run(ctx, &p, N, 1, d_spins, steps, every, nsnap, d_snaps, 1));
+        }
         CHECK(asd_synchronize(ctx));
         double t1 = now_s();
+        if (out_snaps) {
+            double *hs = malloc(sizeof(double) * snap_doubles);
+            CHECK(asd_memcpy(ctx, hs, d_snaps, sizeof(double) * snap_doubles, 1));
+            FILE *fo = fopen(out_snaps, "wb");
+            if (!fo) { perror(out_snaps); return 1; }
+            if (fwrite(hs, sizeof(double), snap_doubles, fo) != snap_doubles) { perror(out_snaps); return 1; }
+            fclose(fo);
+            free(hs);
+            char hdr[4096];
+            snprintf(hdr, sizeof hdr, "%s.json", out_snaps);
+            FILE *fj = fopen(hdr, "w");
+            if (!fj) { perror(hdr); return 1; }
+            fprintf(fj,
+                    "{\"format\": \"float64 little-endian [snapshot][site][x,y,z], site = row * L + col\", "
+                    "\"L\": %d, \"snapshots\": %d, \"snap_every\": %d, \"steps\": %ld, \"dt\": %.17g, "
+                    "\"J\": %.17g, \"D\": %.17g, \"Bz\": %.17g, \"alpha\": %.17g, \"spin_length\": %.17g, "
+                    "\"seed\": %llu, \"source\": \"%s\"}\n",
+                    N, nsnap, every, steps, dt, J1, D, Bz, alpha, len, (unsigned long long)seed,
+                    in_snaps ? "read from file" : "asd_rk4_run");
+            fclose(fj);
+        }
+        const double t1w = now_s();
         CHECK(asd_sqt(ctx, N, d_snaps, nsnap, T0, nlags, q, nq, (asd_precision)prec, out, 0));
         double t2 = now_s();
-        printf("rk4 %.4f s (%.3e spin-steps/s)   S(q,tau) %.4f s (%.2f TFLOP/s GEMM-equivalent, %s)\n", t1 - t0,
-               steps * (double)ns / (t1 - t0), t2 - t1, nlags * 2.0 * ns * ns * 3.0 * T0 / (t2 - t1) / 1e12,
-               prec_name(prec));
+        if (in_snaps)
+            printf("snapshots read %.4f s   ", t1 - t0);
+        else
+            printf("rk4 %.4f s (%.3e spin-steps/s)   ", t1 - t0, steps * (double)ns / (t1 - t0));
+        printf("S(q,tau) %.4f s (%.2f TFLOP/s GEMM-equivalent, %s)\n", t2 - t1w,
+               nlags * 2.0 * ns * ns * 3.0 * T0 / (t2 - t1w) / 1e12, prec_name(prec));
         if (out_sqt) {
             FILE *fo = fopen(out_sqt, "wb");
             if (!fo) { perror(out_sqt); return 1; }

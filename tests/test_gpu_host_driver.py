@@ -43,3 +43,35 @@ def test_pipeline_mode_matches_api(gctx, tmp_path):
     gctx.rk4_run(asd_amd.params(D=0.1), L, S, steps, every, nsnap, snaps)
     ref = gctx.sqt(L, snaps, T0, G, np.array(hdr["q"], dtype=np.int32), "fp64")
     assert np.abs(got[..., 0] + 1j * got[..., 1] - ref).max() <= 1e-13 * np.abs(ref).max()
+
+
+@pytest.mark.gpu
+def test_pipeline_snapshot_file_round_trip(orc, tmp_path):
+    """-W writes the trajectory's snapshots (equal to the oracle's, bit for bit); -R reads
+    them back instead of integrating and gives the same S(q,tau); a file of the wrong size
+    is rejected."""
+    need_gpu()
+    L, steps, every, T0, G = 16, 200, 10, 12, 6
+    common = ["-P", "-n", str(L), "-S", str(steps), "-E", str(every), "-T", str(T0), "-G", str(G), "-e", "0.01",
+              "-D", "0.1", "-B", "0.05", "-s", "7", "-p", "fp64"]
+    r = subprocess.run([BIN, *common, "-W", "snaps.bin", "-O", "a.bin"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    hdr = json.load(open(tmp_path / "snaps.bin.json"))
+    nsnap = steps // every
+    assert (hdr["L"], hdr["snapshots"], hdr["snap_every"]) == (L, nsnap, every)
+    snaps = np.fromfile(tmp_path / "snaps.bin", dtype="<f8").reshape(nsnap, L * L, 3)
+    S0 = orc.init_spins(7, L * L)
+    _, snaps_ref = orc.run(orc.params(dt=0.01, D=0.1, Bz=0.05), L, S0[None], steps, every, nsnap)
+    assert np.array_equal(snaps, snaps_ref.reshape(nsnap, L * L, 3))
+
+    r = subprocess.run([BIN, *common, "-R", "snaps.bin", "-O", "b.bin"], cwd=tmp_path, capture_output=True,
+                       text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    a = np.fromfile(tmp_path / "a.bin", dtype="<f8")
+    b = np.fromfile(tmp_path / "b.bin", dtype="<f8")
+    assert a.shape == b.shape and np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
+
+    (tmp_path / "short.bin").write_bytes((tmp_path / "snaps.bin").read_bytes()[:-8])
+    r = subprocess.run([BIN, *common, "-R", "short.bin"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode != 0 and "expected" in r.stderr

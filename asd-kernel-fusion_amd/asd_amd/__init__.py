@@ -314,7 +314,18 @@ def runge_kutta(p: Params, L: int, spin: np.ndarray):
 
 
 def fused_dgemm(m, n, k, alpha, a, lda, b, ldb, c, ldc, fuse_c, sums):
-    """gemm_kernel.h:355 semantics; c and sums (numpy float64) updated in place."""
+    """gemm_kernel.h:355 semantics; c and sums (numpy float64) updated in place.
+    a: m x k column-major (lda >= m), b: k x n (ldb >= k), c and fuse_c: the blocked
+    4-real/4-imaginary layout with ldc >= 2 ceil4(m) per column, sums: at least 2."""
+    need = {"a": (a, lda * k), "b": (b, ldb * n), "c": (c, ldc * n), "fuse_c": (fuse_c, ldc * n),
+            "sums": (sums, 2)}
+    if m < 0 or n < 0 or k < 0 or lda < max(m, 1) or ldb < max(k, 1) or ldc < 2 * ((m + 3) // 4 * 4):
+        raise ValueError(f"bad sizes m={m} n={n} k={k} lda={lda} ldb={ldb} ldc={ldc}")
+    for name, (arr, size) in need.items():
+        if not (isinstance(arr, np.ndarray) and arr.dtype == np.float64 and arr.flags.c_contiguous):
+            raise ValueError(f"{name} must be a C-contiguous float64 numpy array")
+        if arr.size < size:
+            raise ValueError(f"{name} holds {arr.size} doubles, needs {size}")
     ci = lambda v: C.byref(C.c_int(v))  # noqa: E731
     _check(lib().asd_fused_dgemm(ci(m), ci(n), ci(k), C.byref(C.c_double(alpha)), _np_ptr(a), ci(lda),
                                  _np_ptr(b), ci(ldb), _np_ptr(c), ci(ldc), _np_ptr(fuse_c), _np_ptr(sums)))

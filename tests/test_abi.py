@@ -60,3 +60,27 @@ def test_error_path_without_device_work():
     with pytest.raises(asd_amd.AsdError):
         asd_amd.qpath_gxmg(7)      # odd L rejected on the host
 
+
+
+def test_fused_dgemm_wrapper_rejects_bad_arrays():
+    """The host-pointer fused_dgemm hands raw pointers to C, so the wrapper checks dtype,
+    layout and sizes first (before the library is touched: no GPU needed)."""
+    import asd_amd
+    m = n = k = 8
+    ldc = 2 * m
+    ok = dict(a=np.zeros(m * k), b=np.zeros(k * n), c=np.zeros(ldc * n), fuse_c=np.zeros(ldc * n), sums=np.zeros(2))
+
+    def call(**kw):
+        a = {**ok, **kw}
+        asd_amd.fused_dgemm(m, n, k, 1.0, a["a"], m, a["b"], k, a["c"], ldc, a["fuse_c"], a["sums"])
+
+    bad = [dict(a=np.zeros(m * k, dtype=np.float32)),           # dtype
+           dict(b=np.zeros((n, 2 * k))[:, ::2]),                # not contiguous
+           dict(c=np.zeros(ldc * n - 1)),                       # too small
+           dict(sums=np.zeros(1)),
+           dict(fuse_c=list(range(ldc * n)))]                   # not an ndarray
+    for kw in bad:
+        with pytest.raises(ValueError):
+            call(**kw)
+    with pytest.raises(ValueError):   # ldc below 2 ceil4(m)
+        asd_amd.fused_dgemm(m, n, k, 1.0, ok["a"], m, ok["b"], k, ok["c"], ldc - 2, ok["fuse_c"], ok["sums"])

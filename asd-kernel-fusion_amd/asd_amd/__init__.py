@@ -199,6 +199,16 @@ class Context:
         """Uneven-size trajectories in one persistent launch per step.  spins_list[t]:
         cuda float64 [L_t*L_t, 3] (in/out); snaps_list[t]: [nsnap, L_t*L_t, 3] or None."""
         n = len(Ls)
+        if len(plist) != n or len(spins_list) != n or (snaps_list is not None and len(snaps_list) != n):
+            raise ValueError("one parameter set, state and snapshot buffer per trajectory")
+        t = self._torch
+        for i, (L, s) in enumerate(zip(Ls, spins_list)):
+            if not (s.is_cuda and s.dtype == t.float64 and s.is_contiguous()) or s.numel() != 3 * L * L:
+                raise ValueError(f"spins_list[{i}] must be a contiguous cuda float64 tensor of [{L * L}, 3]")
+            if snaps_list is not None:
+                x = snaps_list[i]
+                if not (x.is_cuda and x.dtype == t.float64 and x.is_contiguous()) or x.numel() < nsnap * 3 * L * L:
+                    raise ValueError(f"snaps_list[{i}] must hold [{nsnap}, {L * L}, 3] float64 on the GPU")
         Ls_c = (C.c_int * n)(*Ls)
         P = (Params * n)(*plist)
         sp = (C.c_void_p * n)(*[s.data_ptr() for s in spins_list])
@@ -222,11 +232,17 @@ class Context:
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         _check(lib().asd_sqt_set_config(self.h, prec, cfg))
 
+    def _check_state(self, S0, L: int):
+        t = self._torch
+        if not (S0.is_cuda and S0.dtype == t.float64 and S0.is_contiguous()) or S0.numel() != 3 * L * L:
+            raise ValueError(f"S0 must be a contiguous cuda float64 tensor of [{L * L}, 3]")
+
     # ------------------------------------------------------------ adjoint
     def sqt_vjp(self, p: Params, L: int, S0, nsteps: int, every: int, T0: int, nlags: int, q, g=None,
                 checkpoint_every: int = 50):
         """Returns (S(q,tau) complex [nlags, nq], grad[4] or None) for upstream g
         (float64 [nlags, nq, 2], dL/dRe S and dL/dIm S)."""
+        self._check_state(S0, L)
         q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
         s = np.empty((nlags, len(q), 2))
         grad = np.zeros(4)
@@ -241,8 +257,11 @@ class Context:
 
     def sqt_loss_grad(self, p: Params, L: int, S0, nsteps: int, every: int, T0: int, nlags: int, q, target,
                       checkpoint_every: int = 50):
+        self._check_state(S0, L)
         q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
         t = np.ascontiguousarray(np.stack([np.real(target), np.imag(target)], axis=-1), dtype=np.float64)
+        if t.size != 2 * nlags * len(q):
+            raise ValueError(f"target must be complex [{nlags}, {len(q)}] (got {np.shape(target)})")
         loss = C.c_double()
         grad = np.zeros(4)
         _check(lib().asd_sqt_loss_grad(self.h, C.byref(p), L, C.c_void_p(S0.data_ptr()), nsteps, every, T0, nlags,
@@ -259,6 +278,8 @@ class Context:
         q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
         assert snaps.dtype == t.float64 and snaps.is_cuda and snaps.is_contiguous()
         nsnap = snaps.shape[0]
+        if snaps.numel() != nsnap * 3 * L * L:
+            raise ValueError(f"snaps must be [nsnap, {L * L}, 3] (got {tuple(snaps.shape)})")
         if out is not None:
             assert out.is_cuda and out.dtype == t.float64 and out.numel() == 2 * nlags * len(q)
             _check(lib().asd_sqt(self.h, L, C.c_void_p(snaps.data_ptr()), nsnap, T0, nlags, _np_ptr(q), len(q),
@@ -273,6 +294,8 @@ class Context:
         t = self._torch
         q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
         assert snaps.dtype == t.float64 and snaps.is_cuda and snaps.is_contiguous()
+        if snaps.numel() != snaps.shape[0] * 3 * L * L:
+            raise ValueError(f"snaps must be [nsnap, {L * L}, 3] (got {tuple(snaps.shape)})")
         res = np.empty((nlags, len(q), 2))
         _check(fn(self.h, L, C.c_void_p(snaps.data_ptr()), snaps.shape[0], T0, nlags, _np_ptr(q), len(q),
                   _np_ptr(res), 0))
@@ -290,6 +313,10 @@ class Context:
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
         nrep = now.numel() // (3 * L * L)
+        for name, x in (("now", now), ("zero", zero)):
+            if not (x.is_cuda and x.dtype == self._torch.float64 and x.is_contiguous()) or x.numel() != nrep * 3 * L * L \
+                    or nrep < 1:
+                raise ValueError(f"{name} must be a contiguous cuda float64 tensor of [nrep, {L * L}, 3]")
         res = np.empty((len(q), 2))
         _check(lib().asd_sqt_replicas(self.h, L, nrep, C.c_void_p(now.data_ptr()), C.c_void_p(zero.data_ptr()),
                                       _np_ptr(q), len(q), prec, _np_ptr(res)))

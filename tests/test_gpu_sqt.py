@@ -309,6 +309,8 @@ def test_invalid_sizes_are_rejected(gctx, orc):
             gctx.sqt(L, snaps, T0, nlags, q, "fp64")
     with pytest.raises(asd_amd.AsdError):
         gctx.sqt(L, snaps, 3, 2, [(0, L)], "fp64")       # q index out of [0, L)
+    with pytest.raises(ValueError):                      # snapshots of another lattice size
+        gctx.sqt(L, snaps[:, : L * L // 2].contiguous(), 3, 2, q, "fp64")
     for prec in (-1, 4):                                 # no such precision
         with pytest.raises(asd_amd.AsdError):
             gctx.sqt(L, snaps, 3, 2, q, prec)

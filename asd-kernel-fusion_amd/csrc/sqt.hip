@@ -566,7 +566,7 @@ constexpr int wait_vmcnt(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >
 constexpr int kWaitLgkm0 = 15 | (7 << 4) | (3 << 14);
 
 // The split-precision GEMM with operand tiles moved by direct global -> LDS loads
-// (global_load_lds_dwordx4: 64 lanes x 16 B into 1 KiB of LDS at a wave-uniform base)
+// (buffer_load_dwordx4 ... lds: 64 lanes x 16 B into 1 KiB of LDS at a wave-uniform base)
 // through a ring of NS stages, so no staging registers and no LDS stores are needed
 // and every chunk is requested NS - 1 steps before it is used.  A stage holds
 // [A hi][A lo][B hi][B lo], rows of 32 halves (64 B) with no padding; the 16-byte

@@ -303,3 +303,35 @@ def test_zero_and_one_step_and_unused_snapshot_slots(gctx, orc):
         assert np.array_equal(s.cpu().numpy(), S_ref), (n, every)
         assert np.array_equal(snaps[:used].cpu().numpy(), sn_ref), (n, every)
         assert np.isnan(snaps[used:].cpu().numpy()).all(), (n, every)
+
+
+SWEEP_L = [9, 17, 23, 31, 33, 47, 65, 100, 129]
+SWEEP_PARAMS = [dict(D=0.1), dict(J=-0.8, D=0.05, Bz=0.2, alpha=0.1, spin_length=1.3)]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L", SWEEP_L)
+def test_every_tile_shape_bit_exact_sweep(gctx, orc, L):
+    """Every kernel choice (automatic, forced 8 / 16 / 32 tiles, streaming strips from
+    L = 64) on ragged and odd lattices, including ones smaller than a loaded tile (the
+    wrapped-coordinate load), gives the oracle's trajectory and snapshots bit for bit,
+    undamped with J = 1 and damped with J != 1, B_z and spin_length != 1."""
+    torch = need_gpu()
+    import asd_amd
+    nsteps, every, nsnap = 24, 6, 4
+    tiles = (0, 8, 16, 32) + ((64,) if L >= 64 else ())
+    try:
+        for k, kw in enumerate(SWEEP_PARAMS):
+            S0 = orc.init_spins(60 + L + k, L * L)[None] * kw.get("spin_length", 1.0)
+            S_ref, snaps_ref = orc.run(orc.params(dt=0.01, **kw), L, S0, nsteps, every, nsnap)
+            for tile in tiles:
+                say(f"rk4 sweep L={L} tile={tile} {kw}")
+                gctx.rk4_set_tile(tile)
+                spins = torch.tensor(S0, device="cuda")
+                snaps = torch.zeros((nsnap,) + S0.shape, dtype=torch.float64, device="cuda")
+                gctx.rk4_run(asd_amd.params(dt=0.01, **kw), L, spins, nsteps, every, nsnap, snaps)
+                gctx.synchronize()
+                assert np.array_equal(spins.cpu().numpy(), S_ref), (L, tile, kw)
+                assert np.array_equal(snaps.cpu().numpy(), snaps_ref), (L, tile, kw)
+    finally:
+        gctx.rk4_set_tile(0)

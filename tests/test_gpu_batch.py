@@ -1,7 +1,7 @@
 """Batched uneven-size trajectories (asd_batch_rk4_run): one persistent launch per
 step over a (trajectory, tile) work list, graph-captured step loop.  Each
-trajectory must equal its unbatched run bit for bit (same tile code) and the
-oracle within 1e-10."""
+trajectory must equal its unbatched run and the oracle bit for bit (same exact
+tile arithmetic)."""
 import numpy as np
 import pytest
 
@@ -57,3 +57,25 @@ def test_batch_without_snapshots_repeat(gctx, orc):
         for t, L in enumerate(Ls):
             ref, _ = orc.run(orc.params(D=D, Bz=0.1 * t, dt=0.01), L, S0[t], 60)
             assert np.abs(spins[t].cpu().numpy() - ref).max() <= 1e-10
+
+
+@pytest.mark.gpu
+def test_batch_ragged_sizes_bit_exact(gctx, orc):
+    """Odd and ragged lattices (some smaller than the 24 x 24 loaded tile) in one batch,
+    undamped J = 1 trajectories (the specialised arithmetic) next to damped J != 1 ones:
+    every trajectory and snapshot equals the oracle's bit for bit."""
+    torch = need_gpu()
+    import asd_amd
+    Ls = [9, 17, 33, 47, 100, 20]
+    kws = [dict(D=0.1), dict(J=0.7, D=0.05, Bz=0.1, alpha=0.05), dict(D=0.2, Bz=0.02),
+           dict(J=1.2, alpha=0.2), dict(D=0.1), dict(J=-0.5, D=0.1, alpha=0.01)]
+    nsteps, every, nsnap = 30, 10, 3
+    S0 = [orc.init_spins(80 + t, L * L) for t, L in enumerate(Ls)]
+    spins = [torch.tensor(s, device="cuda") for s in S0]
+    snaps = [torch.zeros((nsnap, L * L, 3), dtype=torch.float64, device="cuda") for L in Ls]
+    gctx.batch_rk4_run(Ls, [asd_amd.params(dt=0.01, **kw) for kw in kws], spins, nsteps, every, nsnap, snaps)
+    gctx.synchronize()
+    for t, (L, kw) in enumerate(zip(Ls, kws)):
+        ref, ref_snaps = orc.run(orc.params(dt=0.01, **kw), L, S0[t], nsteps, every, nsnap)
+        assert np.array_equal(spins[t].cpu().numpy(), ref), (t, L, kw)
+        assert np.array_equal(snaps[t].cpu().numpy(), ref_snaps), (t, L, kw)

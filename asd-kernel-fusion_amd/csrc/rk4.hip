@@ -706,12 +706,27 @@ static bool rk4_small_tiles(const asd_ctx *ctx, int L, int nrep)
     return ctx->rk4_tile == 8 || (ctx->rk4_tile == 0 && n16 < std::max(ctx->num_cu, 1));
 }
 
-// automatic choice only (L = 128, one lattice: 3.0 us per step against 3.6 for single
-// steps); fused steps are not offered for forced 8 x 8 tiles, which on a large lattice lose
-// (the fused halo costs more than the launch it saves)
+// the automatic choice's 16 x 16 tiles with 1024 threads (four threads per site: below four
+// workgroups per CU, where 32 x 32 tiles are not taken)
+static bool rk4_mid_tiles(const asd_ctx *ctx, int L, int nrep)
+{
+    const long n16 = (long)nrep * ((L + 15) / 16) * ((L + 15) / 16);
+    const long n32 = (long)nrep * ((L + 31) / 32) * ((L + 31) / 32);
+    const int cus = std::max(ctx->num_cu, 1);
+    return ctx->rk4_tile == 0 && !rk4_small_tiles(ctx, L, nrep) && !(2 * n32 > cus && n32 <= 2L * cus) &&
+           n16 < 4L * cus;
+}
+
+// automatic choice only, on the 8 x 8 and the 16 x 16 / 1024-thread paths (per 2000 steps
+// with snapshots: L = 128 6.0 against 7.2 ms for single steps; L = 256 / 320 / 352 8.75 /
+// 14.1 / 14.4 against 10.4 / 14.8 / 15.3 ms); fused steps are not offered for forced tiles,
+// which on a large lattice lose (the fused halo costs more than the launch it saves)
 int rk4_fusable_steps(const asd_ctx *ctx, int L, int nrep)
 {
-    return ctx->rk4_tile == 0 && !rk4_streams(ctx, L, nrep) && rk4_small_tiles(ctx, L, nrep) ? 2 : 1;
+    return ctx->rk4_tile == 0 && !rk4_streams(ctx, L, nrep) &&
+                   (rk4_small_tiles(ctx, L, nrep) || rk4_mid_tiles(ctx, L, nrep))
+               ? 2
+               : 1;
 }
 
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in, double *out,
@@ -754,10 +769,12 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
         const long n16 = (long)nrep * ((L + 15) / 16) * ((L + 15) / 16);
         const long n32 = (long)nrep * ((L + 31) / 32) * ((L + 31) / 32);
         const int cus = std::max(ctx->num_cu, 1);
-        // two fused steps: 8 x 8 tiles with 512 threads, one site of the widest stage region
-        // (22 x 22) per thread (L = 128, 8000 steps with snapshots: 25.9 ms; 3.0 us per step
-        // against 3.6 for single steps)
-        if (steps == 2)
+        // two fused steps: 8 x 8 tiles with 512 threads or 16 x 16 tiles with 1024, one site of
+        // the widest stage region (22 x 22, 30 x 30) per thread (L = 128, 8000 steps with
+        // snapshots: 25.9 ms; 3.0 us per step against 3.6 for single steps)
+        if (steps == 2 && rk4_mid_tiles(ctx, L, nrep))
+            launch_tiles<16, 16, 1024, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+        else if (steps == 2)
             launch_tiles<8, 8, 512, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
         else if (rk4_small_tiles(ctx, L, nrep))
             launch_tiles<8, 8, 256>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);

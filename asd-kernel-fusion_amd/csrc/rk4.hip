@@ -706,25 +706,32 @@ static bool rk4_small_tiles(const asd_ctx *ctx, int L, int nrep)
     return ctx->rk4_tile == 8 || (ctx->rk4_tile == 0 && n16 < std::max(ctx->num_cu, 1));
 }
 
-// the automatic choice's 16 x 16 tiles with 1024 threads (four threads per site: below four
-// workgroups per CU, where 32 x 32 tiles are not taken)
+// the automatic choice's 32 x 32 tiles: from half a tile to two tiles per CU
+static bool rk4_big_tiles(const asd_ctx *ctx, int L, int nrep)
+{
+    const long n32 = (long)nrep * ((L + 31) / 32) * ((L + 31) / 32);
+    const int cus = std::max(ctx->num_cu, 1);
+    return ctx->rk4_tile == 0 && !rk4_small_tiles(ctx, L, nrep) && 2 * n32 > cus && n32 <= 2L * cus;
+}
+// the automatic choice's 16 x 16 tiles with 1024 threads (four threads per site): below four
+// workgroups per CU, where 32 x 32 tiles are not taken
 static bool rk4_mid_tiles(const asd_ctx *ctx, int L, int nrep)
 {
     const long n16 = (long)nrep * ((L + 15) / 16) * ((L + 15) / 16);
-    const long n32 = (long)nrep * ((L + 31) / 32) * ((L + 31) / 32);
-    const int cus = std::max(ctx->num_cu, 1);
-    return ctx->rk4_tile == 0 && !rk4_small_tiles(ctx, L, nrep) && !(2 * n32 > cus && n32 <= 2L * cus) &&
-           n16 < 4L * cus;
+    return ctx->rk4_tile == 0 && !rk4_small_tiles(ctx, L, nrep) && !rk4_big_tiles(ctx, L, nrep) &&
+           n16 < 4L * std::max(ctx->num_cu, 1);
 }
 
-// automatic choice only, on the 8 x 8 and the 16 x 16 / 1024-thread paths (per 2000 steps
-// with snapshots: L = 128 6.0 against 7.2 ms for single steps; L = 256 / 320 / 352 8.75 /
-// 14.1 / 14.4 against 10.4 / 14.8 / 15.3 ms); fused steps are not offered for forced tiles,
-// which on a large lattice lose (the fused halo costs more than the launch it saves)
+// automatic choice only, on the 8 x 8, 32 x 32 and 16 x 16 / 1024-thread paths (with
+// snapshots every 10: L = 128 6.0 against 7.2 ms per 2000 steps; L = 256 / 320 / 352 8.75 /
+// 14.1 / 14.4 against 10.4 / 14.8 / 15.3 ms; 32 x 32 at L = 384 / 448 / 512 / 640 7.8 / 8.25 /
+// 7.9 / 14.15 against 8.26 / 8.8 / 8.6 / 14.65 ms per 1000 steps); not for forced tiles, nor
+// for 16 x 16 tiles with 256 threads, where the fused halo costs more than the launch it
+// saves
 int rk4_fusable_steps(const asd_ctx *ctx, int L, int nrep)
 {
     return ctx->rk4_tile == 0 && !rk4_streams(ctx, L, nrep) &&
-                   (rk4_small_tiles(ctx, L, nrep) || rk4_mid_tiles(ctx, L, nrep))
+                   (rk4_small_tiles(ctx, L, nrep) || rk4_big_tiles(ctx, L, nrep) || rk4_mid_tiles(ctx, L, nrep))
                ? 2
                : 1;
 }
@@ -766,21 +773,20 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
         // per site below four workgroups per CU (L = 256: 4.7 us against 5.8 and 6.9), then
         // 16 x 16 with one (L = 768: 18.4 us against 19.5; L = 1024: 29 us against 27 and
         // 47 for streaming strips)
-        const long n16 = (long)nrep * ((L + 15) / 16) * ((L + 15) / 16);
-        const long n32 = (long)nrep * ((L + 31) / 32) * ((L + 31) / 32);
-        const int cus = std::max(ctx->num_cu, 1);
-        // two fused steps: 8 x 8 tiles with 512 threads or 16 x 16 tiles with 1024, one site of
-        // the widest stage region (22 x 22, 30 x 30) per thread (L = 128, 8000 steps with
-        // snapshots: 25.9 ms; 3.0 us per step against 3.6 for single steps)
-        if (steps == 2 && rk4_mid_tiles(ctx, L, nrep))
+        // two fused steps: 8 x 8 tiles with 512 threads or 16 x 16 tiles with 1024 (one site of
+        // the widest stage region, 22 x 22 or 30 x 30, per thread), or 32 x 32 tiles with 1024
+        // (three sites of the 46 x 46 region per thread)
+        if (steps == 2 && rk4_big_tiles(ctx, L, nrep))
+            launch_tiles<32, 32, 1024, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+        else if (steps == 2 && rk4_mid_tiles(ctx, L, nrep))
             launch_tiles<16, 16, 1024, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
         else if (steps == 2)
             launch_tiles<8, 8, 512, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
         else if (rk4_small_tiles(ctx, L, nrep))
             launch_tiles<8, 8, 256>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
-        else if (ctx->rk4_tile == 0 && 2 * n32 > cus && n32 <= 2L * cus)
+        else if (rk4_big_tiles(ctx, L, nrep))
             launch_tiles<32, 32, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
-        else if (ctx->rk4_tile == 0 && n16 < 4L * cus)
+        else if (rk4_mid_tiles(ctx, L, nrep))
             launch_tiles<16, 16, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
         else
             launch_tiles<16, 16, 256>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);

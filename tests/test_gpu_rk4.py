@@ -215,10 +215,11 @@ def test_spin_length_outside_the_kernels_range_is_rejected(gctx, orc):
 @pytest.mark.gpu
 @pytest.mark.parametrize("L,nrep,kw", [(64, 1, dict(D=0.1, Bz=0.05)), (40, 3, dict(J=-0.7, D=0.05, alpha=0.2)),
                                        (128, 1, dict(D=0.1)), (256, 1, dict(D=0.05, Bz=0.2)),
-                                       (320, 1, dict(J=0.8, D=0.1, alpha=0.05))])
+                                       (320, 1, dict(J=0.8, D=0.1, alpha=0.05)), (448, 1, dict(D=0.1, Bz=0.05))])
 def test_fused_two_step_launches_equal_single_steps(gctx, orc, L, nrep, kw):
-    """On the 8 x 8 and the 16 x 16 / 1024-thread tile paths (L = 256, 320) a launch fuses
-    two RK4 steps (snapshot interval even); with an odd interval every launch is one step.
+    """On the 8 x 8, 16 x 16 / 1024-thread (L = 256, 320) and 32 x 32 (L = 448) tile paths a
+    launch fuses two RK4 steps (snapshot interval even); with an odd interval every launch
+    is one step.
     Both give the same trajectory and the same snapshots at common times, bit for bit,
     graph-replayed and eager, and the oracle's."""
     torch = need_gpu()

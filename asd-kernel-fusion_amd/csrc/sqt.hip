@@ -40,7 +40,7 @@ typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 // members, which arrays of do not get promoted to registers and end up in scratch)
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-constexpr int TILE = 128;  // BM = BN for every configuration
+constexpr int TILE = 128;  // tile edge of the FP64 / FP32 kernels; the split kernels' edges (128, 256) divide 2 TILE
 
 // ------------------------------------------------------------------ packing
 

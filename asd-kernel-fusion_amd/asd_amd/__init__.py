@@ -28,6 +28,7 @@ FP64, FP32, BF16X3, FP16X2 = 0, 1, 2, 3
 PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "fp16x2": FP16X2}
 
 ASD_OK = 0
+ASD_EINVAL, ASD_ENOMEM, ASD_EHIP, ASD_ENOTSUP, ASD_ENOTFINITE = -1, -2, -3, -4, -5
 
 
 class AsdError(RuntimeError):
@@ -69,6 +70,11 @@ _SIGS = {
     "asd_sqt_bruteforce": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int]),
     "asd_batch_rk4_run": (C.c_int, [_vp, C.c_int, _vp, _vp, _vp, C.c_long, C.c_int, C.c_int, _vp]),
     "asd_autotune_sqt": (C.c_int, [_vp, C.c_int, C.c_int, C.c_int, _ip, _dp]),
+    "asd_autotune_batch": (C.c_int, [_vp, C.c_int, _vp, _vp, _ip, _dp]),
+    "asd_batch_num_variants": (C.c_int, []),
+    "asd_batch_variant_name": (C.c_char_p, [C.c_int]),
+    "asd_batch_set_variant": (C.c_int, [_vp, C.c_int]),
+    "asd_batch_variant_for": (C.c_int, [_vp, C.c_int, _vp]),
     "asd_sqt_num_configs": (C.c_int, [C.c_int]),
     "asd_sqt_config_name": (C.c_char_p, [C.c_int, C.c_int]),
     "asd_sqt_set_config": (C.c_int, [_vp, C.c_int, C.c_int]),
@@ -80,6 +86,7 @@ _SIGS = {
     "asd_sqt_vjp": (C.c_int, [_vp, C.POINTER(Params), C.c_int, _vp, C.c_long, C.c_int, C.c_int, C.c_int, _vp,
                               C.c_int, _vp, C.c_int, _dp, _vp]),
     "asd_rk4_set_tile": (C.c_int, [_vp, C.c_int]),
+    "asd_rk4_set_exact": (C.c_int, [_vp, C.c_int]),
     "asd_set_timing": (C.c_int, [_vp, C.c_int]),
     "asd_stage_times": (C.c_int, [_vp, _dp]),
     "asd_euler": (C.c_int, [C.POINTER(Params), C.c_int, C.c_int, _vp, _vp]),
@@ -130,6 +137,15 @@ def qpath_gxmg(L: int) -> np.ndarray:
     return q
 
 
+def _check_f64_cuda(x, name: str, numel: int = None):
+    """Raise ValueError unless x is a contiguous float64 CUDA tensor (of `numel` elements)."""
+    import torch
+    if not isinstance(x, torch.Tensor) or not x.is_cuda or x.dtype != torch.float64 or not x.is_contiguous():
+        raise ValueError(f"{name} must be a contiguous float64 tensor on the GPU")
+    if numel is not None and x.numel() != numel:
+        raise ValueError(f"{name} holds {x.numel()} doubles, expected {numel}")
+
+
 def full_grid(L: int) -> np.ndarray:
     mx, my = np.meshgrid(np.arange(L), np.arange(L), indexing="ij")
     return np.ascontiguousarray(np.stack([mx.ravel(), my.ravel()], axis=1).astype(np.int32))
@@ -167,6 +183,11 @@ class Context:
     def rk4_set_tile(self, tile: int = 0):
         _check(lib().asd_rk4_set_tile(self.h, tile))
 
+    def rk4_set_exact(self, exact: bool = True):
+        """True: the reference's RK4 operation order (IEEE-equal values, the default);
+        False: the tolerance mode (fused multiply-adds, fewer instructions per site)."""
+        _check(lib().asd_rk4_set_exact(self.h, 1 if exact else 0))
+
     def set_timing(self, on: bool = True):
         _check(lib().asd_set_timing(self.h, 1 if on else 0))
 
@@ -179,18 +200,27 @@ class Context:
         return dict(zip(self.STAGES, list(t)))
 
     # -------------------------------------------------------------- RK4
+    def _rk4_args(self, L, spins, snap_every, nsnap, snaps):
+        if L < 2:
+            raise ValueError(f"L must be >= 2 (got {L})")
+        _check_f64_cuda(spins, "spins")
+        nrep = spins.numel() // (3 * L * L)
+        if nrep < 1 or nrep * 3 * L * L != spins.numel():
+            raise ValueError(f"spins must hold nrep x {L * L} x 3 doubles (got {spins.numel()})")
+        sp = None
+        if snaps is not None:
+            if snap_every < 1 or nsnap < 0:
+                raise ValueError("snap_every must be >= 1 and nsnap >= 0 with a snapshot buffer")
+            _check_f64_cuda(snaps, "snaps", nsnap * spins.numel())
+            sp = C.c_void_p(snaps.data_ptr())
+        return nrep, sp
+
     def rk4_run(self, p: Params, L: int, spins, nsteps: int, snap_every: int = 0, nsnap: int = 0, snaps=None,
                 use_graph: bool = True):
         """spins: cuda float64 tensor [nrep, L*L, 3] or [L*L, 3] (in/out);
-        snaps: cuda float64 tensor [nsnap, ...spins.shape] or None."""
-        t = self._torch
-        assert spins.dtype == t.float64 and spins.is_cuda and spins.is_contiguous()
-        nrep = spins.numel() // (3 * L * L)
-        assert nrep * 3 * L * L == spins.numel()
-        sp = None
-        if snaps is not None:
-            assert snaps.dtype == t.float64 and snaps.is_contiguous() and snaps.numel() == nsnap * spins.numel()
-            sp = C.c_void_p(snaps.data_ptr())
+        snaps: cuda float64 tensor [nsnap, ...spins.shape] or None.  Raises AsdError with
+        code ASD_ENOTFINITE when a step leaves the range where it is defined (NaN states)."""
+        nrep, sp = self._rk4_args(L, spins, snap_every, nsnap, snaps)
         _check(lib().asd_rk4_run(self.h, C.byref(p), L, nrep, C.c_void_p(spins.data_ptr()), nsteps, snap_every,
                                  nsnap, sp, 1 if use_graph else 0))
 
@@ -216,6 +246,29 @@ class Context:
         if snaps_list is not None:
             sn = (C.c_void_p * n)(*[s.data_ptr() for s in snaps_list])
         _check(lib().asd_batch_rk4_run(self.h, n, Ls_c, P, sp, nsteps, snap_every, nsnap, sn))
+
+    def autotune_batch(self, Ls, plist):
+        """Times every batch kernel variant for this composition; returns (best, {name: ms})
+        and records the choice for later batch_rk4_run calls with the same multiset of L."""
+        n = len(Ls)
+        if len(plist) != n or n < 1:
+            raise ValueError("one parameter set per trajectory")
+        nv = lib().asd_batch_num_variants()
+        times = (C.c_double * nv)()
+        best = C.c_int()
+        _check(lib().asd_autotune_batch(self.h, n, (C.c_int * n)(*Ls), (Params * n)(*plist), C.byref(best), times))
+        names = [lib().asd_batch_variant_name(i).decode() for i in range(nv)]
+        return best.value, dict(zip(names, list(times)))
+
+    def batch_set_variant(self, variant: int = -1):
+        _check(lib().asd_batch_set_variant(self.h, variant))
+
+    def batch_variant_for(self, Ls) -> int:
+        """The variant batch_rk4_run would take for this composition (-1: not tuned yet)."""
+        r = lib().asd_batch_variant_for(self.h, len(Ls), (C.c_int * len(Ls))(*Ls))
+        if r < -1:
+            _check(r)
+        return r
 
     # ---------------------------------------------------------- autotuning
     def autotune(self, L: int, T0: int, precision="fp16x2"):
@@ -273,15 +326,14 @@ class Context:
     def sqt(self, L: int, snaps, T0: int, nlags: int, q, precision="fp64", out=None):
         """snaps: cuda float64 [nsnap, L*L, 3].  Returns complex128 numpy [nlags, nq]
         (or fills the cuda float64 tensor ``out`` [nlags, nq, 2] and returns it)."""
-        t = self._torch
         prec = PRECISIONS[precision] if isinstance(precision, str) else int(precision)
         q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
-        assert snaps.dtype == t.float64 and snaps.is_cuda and snaps.is_contiguous()
-        nsnap = snaps.shape[0]
+        _check_f64_cuda(snaps, "snaps")
+        nsnap = snaps.shape[0] if snaps.dim() > 0 else 0
         if snaps.numel() != nsnap * 3 * L * L:
             raise ValueError(f"snaps must be [nsnap, {L * L}, 3] (got {tuple(snaps.shape)})")
         if out is not None:
-            assert out.is_cuda and out.dtype == t.float64 and out.numel() == 2 * nlags * len(q)
+            _check_f64_cuda(out, "out", 2 * nlags * len(q))
             _check(lib().asd_sqt(self.h, L, C.c_void_p(snaps.data_ptr()), nsnap, T0, nlags, _np_ptr(q), len(q),
                                  prec, C.c_void_p(out.data_ptr()), 1))
             return out
@@ -291,9 +343,8 @@ class Context:
         return res[..., 0] + 1j * res[..., 1]
 
     def _sqt_other(self, fn, L, snaps, T0, nlags, q):
-        t = self._torch
         q = np.ascontiguousarray(np.asarray(q, dtype=np.int32).reshape(-1, 2))
-        assert snaps.dtype == t.float64 and snaps.is_cuda and snaps.is_contiguous()
+        _check_f64_cuda(snaps, "snaps")
         if snaps.numel() != snaps.shape[0] * 3 * L * L:
             raise ValueError(f"snaps must be [nsnap, {L * L}, 3] (got {tuple(snaps.shape)})")
         res = np.empty((nlags, len(q), 2))

@@ -71,12 +71,13 @@ __global__ void adj_renorm_kernel(long ns, double len, const double *__restrict_
 }
 
 // per-site dL/dh (gh) and local dL/dS (gS) for mu = ca*lamc + cb*carry at stage input Y,
-// plus dt-weighted parameter partials accumulated into grad[4]
+// plus dt-weighted parameter partials added to this block's own slot gacc[block][4] (no
+// atomics: launches are stream-ordered, so the sums repeat bit for bit)
 __global__ __launch_bounds__(256) void adj_jt1_kernel(LLGParams p, int L, const double *__restrict__ Y,
                                                       const double *__restrict__ lamc, double ca,
                                                       const double *__restrict__ carry, double cb,
                                                       double *__restrict__ gh, double *__restrict__ gS,
-                                                      double *__restrict__ grad)
+                                                      double *__restrict__ gacc)
 {
     __shared__ double red[4][256];
     const long ns = (long)L * L;
@@ -120,7 +121,17 @@ __global__ __launch_bounds__(256) void adj_jt1_kernel(LLGParams p, int L, const 
             for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + st];
         __syncthreads();
     }
-    if (threadIdx.x < 4) unsafeAtomicAdd(&grad[threadIdx.x], red[threadIdx.x][0]);
+    if (threadIdx.x < 4) gacc[4L * blockIdx.x + threadIdx.x] += red[threadIdx.x][0];
+}
+
+// grad[k] = sum over blocks b (in order) of gacc[b][k]; one thread per parameter
+__global__ void grad_reduce_kernel(const double *__restrict__ gacc, int nblk, double *__restrict__ grad)
+{
+    const int k = threadIdx.x;
+    if (k >= 4) return;
+    double s = 0.0;
+    for (int b = 0; b < nblk; b++) s += gacc[4L * b + k];
+    grad[k] = s;
 }
 
 // lambda_Y(j) = dt (gS(j) + J sum_{i in nn(j)} gh(i)); lamS += lambda_Y; carry = lambda_Y
@@ -222,9 +233,28 @@ static dim3 g1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 using namespace asd;
 
+static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *d_S0, long nsteps, int every,
+                         int T0, int nlags, const int *h_q, int nq, const double *h_g, int checkpoint_every,
+                         double grad[4], double *h_sqt);
+
+// The reverse sweep linearises the reference's operation order (rk4_stages_store), so the
+// forward and recomputed trajectories take the exact RK4 arithmetic whatever the
+// context's setting.
 extern "C" int asd_sqt_vjp(asd_ctx *c, const asd_params *pp, int L, const double *d_S0, long nsteps, int every,
                            int T0, int nlags, const int *h_q, int nq, const double *h_g, int checkpoint_every,
                            double grad[4], double *h_sqt)
+{
+    ASD_CHECK(c, "NULL argument");
+    const int saved = c->rk4_exact;
+    c->rk4_exact = 1;
+    const int r = sqt_vjp_exact(c, pp, L, d_S0, nsteps, every, T0, nlags, h_q, nq, h_g, checkpoint_every, grad, h_sqt);
+    c->rk4_exact = saved;
+    return r;
+}
+
+static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *d_S0, long nsteps, int every,
+                         int T0, int nlags, const int *h_q, int nq, const double *h_g, int checkpoint_every,
+                         double grad[4], double *h_sqt)
 {
     ASD_CHECK(c && pp && d_S0 && h_q && (grad || !h_g), "NULL argument");
     ASD_CHECK(L >= 2 && L <= 1024, "L out of range");
@@ -242,8 +272,10 @@ extern "C" int asd_sqt_vjp(asd_ctx *c, const asd_params *pp, int L, const double
     hipStream_t st = c->stream;
     const LLGParams p = make_llg(*pp);
 
-    // workspace: snapshots | checkpoints | segment states | 8 stage buffers | 5 adjoint buffers | dX | grad
-    const long words = (long)nsnap * n + ncp * n + (long)C * n + 8 * n + 5 * n + (long)nsnap * n + 8;
+    // workspace: snapshots | checkpoints | segment states | 8 stage buffers | 5 adjoint buffers | dX |
+    // per-block gradient partials | grad
+    const int nblk = (int)g1(ns).x;
+    const long words = (long)nsnap * n + ncp * n + (long)C * n + 8 * n + 5 * n + (long)nsnap * n + 4L * nblk + 4;
     ASD_TRY(c->adj0.ensure(sizeof(double) * words));
     double *snaps = c->adj0.as<double>();
     double *cps = snaps + (long)nsnap * n;
@@ -252,12 +284,15 @@ extern "C" int asd_sqt_vjp(asd_ctx *c, const asd_params *pp, int L, const double
     double *k1 = cc + n, *k2 = k1 + n, *k3 = k2 + n, *k4 = k3 + n;
     double *lam = k4 + n, *lamc = lam + n, *carry = lamc + n, *gh = carry + n, *gS = gh + n;
     double *dX = gS + n;
-    double *dgrad = dX + (long)nsnap * n;
+    double *gacc = dX + (long)nsnap * n;
+    double *dgrad = gacc + 4L * nblk;
 
     // ---- forward with checkpoints (every C steps) and snapshots (every `every` steps)
     ASD_TRY(c->adj1.ensure(sizeof(double) * n));
     double *S = c->adj1.as<double>();
     ASD_HIP(hipMemcpyAsync(S, d_S0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+    if (nsteps == 0)  // the only snapshot is the initial state (nsnap == 1 here)
+        ASD_HIP(hipMemcpyAsync(snaps, d_S0, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
     for (long a = 0, k = 0; a < nsteps || k == 0; a += C, k++) {
         ASD_HIP(hipMemcpyAsync(cps + k * n, S, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
         if (a >= nsteps) break;
@@ -295,7 +330,7 @@ extern "C" int asd_sqt_vjp(asd_ctx *c, const asd_params *pp, int L, const double
     ASD_HIP(hipGetLastError());
 
     // ---- reverse sweep over checkpointed segments
-    ASD_HIP(hipMemsetAsync(dgrad, 0, 4 * sizeof(double), st));
+    ASD_HIP(hipMemsetAsync(gacc, 0, sizeof(double) * 4L * nblk, st));
     ASD_HIP(hipMemsetAsync(lam, 0, sizeof(double) * n, st));
     if (nsteps % every == 0 && nsteps / every < nsnap)
         ASD_HIP(hipMemcpyAsync(lam, dX + (nsteps / every) * n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
@@ -318,7 +353,7 @@ extern "C" int asd_sqt_vjp(asd_ctx *c, const asd_params *pp, int L, const double
             for (int sidx = 3; sidx >= 0; sidx--) {
                 hipLaunchKernelGGL(adj_jt1_kernel, g1(ns), dim3(256), 0, st, p, L, Ys[sidx], (const double *)lamc,
                                    ca[sidx], sidx == 3 ? (const double *)nullptr : (const double *)carry, cb[sidx],
-                                   gh, gS, dgrad);
+                                   gh, gS, gacc);
                 hipLaunchKernelGGL(adj_jt2_kernel, g1(ns), dim3(256), 0, st, p, L, (const double *)gh,
                                    (const double *)gS, lamS, carry);
             }
@@ -328,6 +363,7 @@ extern "C" int asd_sqt_vjp(asd_ctx *c, const asd_params *pp, int L, const double
             ASD_HIP(hipMemcpyAsync(lam, lamS, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
         }
     }
+    hipLaunchKernelGGL(grad_reduce_kernel, dim3(1), dim3(64), 0, st, (const double *)gacc, nblk, dgrad);
     ASD_HIP(hipGetLastError());
     ASD_HIP(hipMemcpyAsync(grad, dgrad, 4 * sizeof(double), hipMemcpyDeviceToHost, st));
     ASD_HIP(hipStreamSynchronize(st));

@@ -76,6 +76,41 @@ static inline uint64_t hash64(uint64_t seed, uint64_t a, uint64_t b)
 
 __global__ void set_int_kernel(int *p, int v) { *p = v; }
 
+// longest step loop captured as one graph (longer snapshot intervals run eagerly)
+constexpr long kMaxGraphSteps = 4096;
+
+// the guard word's verdict: ASD_OK, or ASD_ENOTFINITE naming the first step
+static int guard_verdict(unsigned long long h, bool batch)
+{
+    if (h == kGuardNone) return ASD_OK;
+    const long step = (long)(h >> 20);
+    const int who = (int)(h & 0xFFFFF);
+    return set_error(ASD_ENOTFINITE,
+                     "RK4 state left the range of the step (|c|^2 NaN, infinite or outside [2^-900, 2^900]) "
+                     "in %s %d, first in the launch starting at step %ld",
+                     batch ? "trajectory" : "replica", who, step);
+}
+
+// the guard of a launch at offset k of a captured chunk whose snapshot launches (k % every
+// == 0) bump *ctr after they run: its step is every * *ctr + k % every, less every when it
+// is not a snapshot launch (the counter was bumped once more in its interval)
+static Guard chunk_guard(unsigned long long *first, const int *ctr, int k, int every)
+{
+    Guard g;
+    g.first = first;
+    g.ctr = ctr;
+    g.every = every;
+    g.step = (k % every) - (k % every ? every : 0);
+    return g;
+}
+static Guard eager_guard(unsigned long long *first, long step)
+{
+    Guard g;
+    g.first = first;
+    g.step = step;
+    return g;
+}
+
 int mark(asd_ctx *c, int k)
 {
     if (!c->timing) return ASD_OK;
@@ -239,6 +274,13 @@ int asd_rk4_set_tile(asd_ctx *c, int tile)
     return ASD_OK;
 }
 
+int asd_rk4_set_exact(asd_ctx *c, int exact)
+{
+    ASD_CHECK(c && (exact == 0 || exact == 1), "exact must be 0 or 1");
+    c->rk4_exact = exact;
+    return ASD_OK;  // graphs are keyed on the setting
+}
+
 int asd_synchronize(asd_ctx *c)
 {
     ASD_CHECK(c, "ctx is NULL");
@@ -248,24 +290,17 @@ int asd_synchronize(asd_ctx *c)
 
 // --------------------------------------------------------------------- RK4
 
-int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spins, long nsteps,
-                int snap_every, int nsnap, double *d_snaps, int use_graph)
+// The RK4 step loop of asd_rk4_run: `other` is the second
+// state buffer, `counter` the snapshot counter (graph mode only), `first` the guard word
+// (NULL: no check).  Everything is stream-ordered; nothing is allocated here.
+static int rk4_drive(asd_ctx *c, const LLGParams &p, int L, int nrep, double *d_spins, double *other,
+                     int *counter, unsigned long long *first, long nsteps, int snap_every, int nsnap,
+                     double *d_snaps, bool use_graph)
 {
-    ASD_CHECK(c && d_spins, "ctx/spins NULL");
-    ASD_TRY(check_params(pp));
-    ASD_TRY(check_L(L));
-    ASD_CHECK(nrep >= 1 && nrep <= 65535, "nrep out of range");
-    ASD_CHECK(nsteps >= 0, "nsteps < 0");
-    ASD_CHECK(!d_snaps || (snap_every >= 1 && nsnap >= 0), "snap_every must be >= 1");
-    ASD_HIP(hipSetDevice(c->device));
-    const LLGParams p = make_llg(*pp);
     const long n = (long)nrep * L * L * 3;
     const long snap_stride = n;
-    ASD_TRY(c->state_b.ensure(sizeof(double) * n));
-    ASD_TRY(c->counter.ensure(sizeof(int)));
-    double *bufs[2] = {d_spins, c->state_b.as<double>()};
+    double *bufs[2] = {d_spins, other};
     hipStream_t st = c->stream;
-    ASD_TRY(mark(c, 5));
 
     // snapshots due while stepping: those at s = k * every < nsteps; the one at
     // s == nsteps (if any) is the final state, copied at the end
@@ -277,19 +312,20 @@ int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spi
     // between the two steps of a launch (snapshots are of a launch's input state)
     const int fs = rk4_fusable_steps(c, L, nrep) == 2 && (!d_snaps || snap_every % 2 == 0) ? 2 : 1;
 
-    // graph: one chunk = `every` steps, doubled until it holds an even number of launches
-    // (so the ping-pong parity repeats); the first launch writes snapshot *counter, a tiny
-    // node bumps the counter
-    if (use_graph && d_snaps && nsnap_in_loop >= 2) {
-        int chunk = snap_every;
-        while ((chunk / fs) % 2) chunk *= 2;
+    // graph: one chunk = `every` steps, doubled where needed so that it holds an even number
+    // of launches (the ping-pong parity repeats); the first launch of each snapshot interval
+    // writes snapshot *counter and a tiny node bumps the counter
+    long chunk = snap_every;
+    if ((chunk / fs) % 2) chunk *= 2;
+    if (use_graph && d_snaps && nsnap_in_loop >= 2 && counter && chunk <= kMaxGraphSteps) {
         const long nchunks = std::min<long>(nsteps / chunk, nsnap_in_loop / (chunk / snap_every));
         if (nchunks >= 1) {
             RK4Graph &g = c->graph;
             const bool same = g.exec && g.L == L && g.nrep == nrep && g.every == snap_every && g.chunk == chunk &&
                               g.tile == c->rk4_tile && g.fuse == fs && g.a == bufs[0] && g.b == bufs[1] &&
                               g.snaps == d_snaps && std::memcmp(&g.p, &p, sizeof p) == 0 &&
-                              g.snap_stride == snap_stride;
+                              g.snap_stride == snap_stride && g.guard == (const void *)first &&
+                              g.exact == c->rk4_exact;
             if (!same) {
                 if (g.exec) (void)hipGraphExecDestroy(g.exec);
                 g = RK4Graph();
@@ -303,8 +339,9 @@ int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spi
                 for (int k = 0; k < chunk && r == ASD_OK; k += fs) {
                     const bool snap_here = k % snap_every == 0;
                     const int j = k / fs;  // launch index: its input buffer alternates
+                    const Guard gd = first ? chunk_guard(first, counter, k, snap_every) : Guard();
                     r = rk4_launch_step(c, p, L, nrep, bufs[j & 1], bufs[(j + 1) & 1], snap_here ? d_snaps : nullptr,
-                                        c->counter.as<int>(), snap_stride, snap_here ? 1 : 0, fs);
+                                        counter, snap_stride, snap_here ? 1 : 0, fs, gd);
                 }
                 c->stream = saved;
                 hipError_t e1 = hipStreamEndCapture(cap, &graph);
@@ -314,10 +351,12 @@ int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spi
                 hipError_t e2 = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
                 (void)hipGraphDestroy(graph);
                 ASD_HIP(e2);
-                g.L = L; g.nrep = nrep; g.every = snap_every; g.chunk = chunk; g.tile = c->rk4_tile; g.fuse = fs;
+                g.L = L; g.nrep = nrep; g.every = snap_every; g.chunk = (int)chunk; g.tile = c->rk4_tile; g.fuse = fs;
                 g.a = bufs[0]; g.b = bufs[1]; g.snaps = d_snaps; g.p = p; g.snap_stride = snap_stride;
+                g.guard = first;
+                g.exact = c->rk4_exact;
             }
-            hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, c->counter.as<int>(), 0);
+            hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, counter, 0);
             for (long ch = 0; ch < nchunks; ch++) ASD_HIP(hipGraphLaunch(g.exec, st));
             s = nchunks * chunk;
             cur = 0;  // an even number of launches per chunk: state back in bufs[0]
@@ -329,7 +368,8 @@ int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spi
         if (d_snaps && s % snap_every == 0 && s / snap_every < nsnap) snap = d_snaps + (s / snap_every) * snap_stride;
         const bool mid_snap = d_snaps && (s + 1) % snap_every == 0 && (s + 1) / snap_every < nsnap;
         const int k = fs == 2 && s + 2 <= nsteps && !mid_snap ? 2 : 1;
-        ASD_TRY(rk4_launch_step(c, p, L, nrep, bufs[cur], bufs[cur ^ 1], snap, nullptr, snap_stride, 0, k));
+        ASD_TRY(rk4_launch_step(c, p, L, nrep, bufs[cur], bufs[cur ^ 1], snap, nullptr, snap_stride, 0, k,
+                                first ? eager_guard(first, s) : Guard()));
         cur ^= 1;
         s += k;
     }
@@ -338,8 +378,42 @@ int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spi
                                hipMemcpyDeviceToDevice, st));
     if (cur == 1)
         ASD_HIP(hipMemcpyAsync(d_spins, bufs[1], sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    ASD_TRY(mark(c, 6));
     return ASD_OK;
+}
+
+static int check_rk4_args(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spins, long nsteps,
+                          int snap_every, int nsnap, double *d_snaps)
+{
+    ASD_CHECK(c && d_spins, "ctx/spins NULL");
+    ASD_TRY(check_params(pp));
+    ASD_TRY(check_L(L));
+    ASD_CHECK(nrep >= 1 && nrep <= 65535, "nrep out of range");
+    ASD_CHECK(nsteps >= 0, "nsteps < 0");
+    ASD_CHECK(!d_snaps || (snap_every >= 1 && nsnap >= 0), "snap_every must be >= 1");
+    return ASD_OK;
+}
+
+int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spins, long nsteps,
+                int snap_every, int nsnap, double *d_snaps, int use_graph)
+{
+    ASD_TRY(check_rk4_args(c, pp, L, nrep, d_spins, nsteps, snap_every, nsnap, d_snaps));
+    ASD_HIP(hipSetDevice(c->device));
+    const LLGParams p = make_llg(*pp);
+    const long n = (long)nrep * L * L * 3;
+    ASD_TRY(c->state_b.ensure(sizeof(double) * n));
+    ASD_TRY(c->counter.ensure(sizeof(int)));
+    ASD_TRY(c->guard.ensure(sizeof(unsigned long long)));
+    unsigned long long *first = c->guard.as<unsigned long long>();
+    hipStream_t st = c->stream;
+    ASD_TRY(mark(c, 5));
+    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, first, kGuardNone);
+    ASD_TRY(rk4_drive(c, p, L, nrep, d_spins, c->state_b.as<double>(), c->counter.as<int>(), first, nsteps,
+                      snap_every, nsnap, d_snaps, use_graph != 0));
+    ASD_TRY(mark(c, 6));
+    unsigned long long h = kGuardNone;
+    ASD_HIP(hipMemcpyAsync(&h, first, sizeof h, hipMemcpyDeviceToHost, st));
+    ASD_HIP(hipStreamSynchronize(st));
+    return guard_verdict(h, false);
 }
 
 // ---------------------------------------------------------------- S(q,tau)
@@ -453,63 +527,203 @@ int asd_autotune_sqt(asd_ctx *c, int L, int T0, asd_precision prec, int *best, d
     return sqt_autotune(c, L, T0, prec, best, times_ms);
 }
 
-int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *params, double *const *d_spins,
-                      long nsteps, int snap_every, int nsnap, double *const *d_snaps)
+// work list of (trajectory, tile) items for tiles of edge T, interleaved tile by tile so
+// that every stretch of the list mixes lattice sizes (balanced persistent workgroups)
+static std::vector<int2> batch_items(int ntraj, const int *Ls, int T)
 {
-    ASD_CHECK(c && ntraj >= 1 && Ls && params && d_spins, "bad arguments");
-    ASD_CHECK(nsteps >= 0, "nsteps < 0");
-    ASD_CHECK(!d_snaps || (snap_every >= 1 && nsnap >= 0), "snap_every must be >= 1");
-    ASD_HIP(hipSetDevice(c->device));
-    constexpr int T = 16;  // tile edge of rk4_batch_kernel
-    std::vector<BatchTraj> tab(ntraj);
     std::vector<int2> items;
-    long total = 0;
+    for (int t = 0; t < ntraj; t++) {
+        const int nt = (Ls[t] + T - 1) / T;
+        for (int k = 0; k < nt * nt; k++) items.push_back(make_int2(t, k));
+    }
+    std::stable_sort(items.begin(), items.end(), [](const int2 &x, const int2 &y) { return x.y < y.y; });
+    return items;
+}
+
+// uploads the trajectory table and variant v's work list into the context's buffers
+static int batch_tables(asd_ctx *c, const std::vector<BatchTraj> &tab, const int *Ls, int v, int *nitems)
+{
+    const int ntraj = (int)tab.size();
+    const std::vector<int2> items = batch_items(ntraj, Ls, rk4_batch_variant_tile(v));
+    *nitems = (int)items.size();
+    ASD_TRY(c->tabs.ensure(sizeof(BatchTraj) * ntraj));
+    ASD_TRY(c->work.ensure(sizeof(int2) * items.size()));
+    ASD_HIP(hipMemcpyAsync(c->tabs.ptr, tab.data(), sizeof(BatchTraj) * ntraj, hipMemcpyHostToDevice, c->stream));
+    ASD_HIP(hipMemcpyAsync(c->work.ptr, items.data(), sizeof(int2) * items.size(), hipMemcpyHostToDevice, c->stream));
+    ASD_HIP(hipStreamSynchronize(c->stream));  // the host vectors go out of scope
+    return ASD_OK;
+}
+
+static int check_batch(int ntraj, const int *Ls, const asd_params *params, long *total)
+{
+    ASD_CHECK(ntraj >= 1 && ntraj <= (1 << 20) && Ls && params, "bad arguments");
+    *total = 0;
     for (int t = 0; t < ntraj; t++) {
         ASD_TRY(check_L(Ls[t]));
         ASD_TRY(check_params(&params[t]));
-        ASD_CHECK(d_spins[t], "d_spins[%d] is NULL", t);
-        total += 3L * Ls[t] * Ls[t];
+        *total += 3L * Ls[t] * Ls[t];
     }
+    return ASD_OK;
+}
+
+static std::vector<int> batch_key(int ntraj, const int *Ls)
+{
+    std::vector<int> k(Ls, Ls + ntraj);
+    std::sort(k.begin(), k.end());
+    return k;
+}
+
+// deterministic unit spins for the autotuner's timing runs (the values do not matter)
+__global__ void fill_unit_spins_kernel(double *p, long nspins)
+{
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < nspins; i += (long)gridDim.x * blockDim.x) {
+        const double z = 2.0 * ((double)((i * 2654435761L) & 1023) + 0.5) / 1024.0 - 1.0, a = 0.61803398875 * (double)i;
+        const double r = sqrt(1.0 - z * z);
+        p[3 * i] = r * cos(a);
+        p[3 * i + 1] = r * sin(a);
+        p[3 * i + 2] = z;
+    }
+}
+
+int asd_batch_num_variants(void) { return rk4_batch_num_variants(); }
+const char *asd_batch_variant_name(int v) { return rk4_batch_variant_name(v); }
+
+int asd_batch_set_variant(asd_ctx *c, int v)
+{
+    ASD_CHECK(c && v >= -1 && v < rk4_batch_num_variants(), "variant must be -1 (automatic) or in [0, %d)",
+              rk4_batch_num_variants());
+    c->batch_variant = v;
+    return ASD_OK;
+}
+
+int asd_batch_variant_for(asd_ctx *c, int ntraj, const int *Ls)
+{
+    ASD_CHECK(c && ntraj >= 1 && Ls, "bad arguments");
+    if (c->batch_variant >= 0) return c->batch_variant;
+    auto it = c->batch_tuned.find(batch_key(ntraj, Ls));
+    return it == c->batch_tuned.end() ? -1 : it->second;
+}
+
+int asd_autotune_batch(asd_ctx *c, int ntraj, const int *Ls, const asd_params *params, int *best, double *times_ms)
+{
+    ASD_CHECK(c, "ctx NULL");
+    long total = 0;
+    ASD_TRY(check_batch(ntraj, Ls, params, &total));
+    ASD_HIP(hipSetDevice(c->device));
+    hipStream_t st = c->stream;
+    DevBuf a;
+    ASD_TRY(a.ensure(sizeof(double) * total));
+    ASD_TRY(c->batch_b.ensure(sizeof(double) * total));
+    hipLaunchKernelGGL(fill_unit_spins_kernel, dim3(1024), dim3(256), 0, st, a.as<double>(), total / 3);
+    std::vector<BatchTraj> tab(ntraj);
+    for (long t = 0, off = 0; t < ntraj; off += 3L * Ls[t] * Ls[t], t++) {
+        tab[t].a = a.as<double>() + off;
+        tab[t].b = c->batch_b.as<double>() + off;
+        tab[t].snaps = nullptr;
+        tab[t].p = make_llg(params[t]);
+        tab[t].L = Ls[t];
+    }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    ASD_HIP(hipEventCreate(&e0));
+    hipError_t ee = hipEventCreate(&e1);
+    if (ee != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        ASD_HIP(ee);
+    }
+    // 20 steps per timing run (10 launches of the two-step variants), best of two runs
+    constexpr int kSteps = 20;
+    int rc = ASD_OK, bi = 0;
+    double best_ms = 1e300;
+    for (int v = 0; v < rk4_batch_num_variants() && rc == ASD_OK; v++) {
+        int nitems = 0;
+        rc = batch_tables(c, tab, Ls, v, &nitems);
+        const int fs = rk4_batch_variant_steps(v);
+        double ms = 1e300;
+        for (int rep = 0; rep < 3 && rc == ASD_OK; rep++) {
+            if (hipEventRecord(e0, st) != hipSuccess) rc = set_error(ASD_EHIP, "hipEventRecord failed");
+            for (int k = 0; k < kSteps && rc == ASD_OK; k += fs)
+                rc = rk4_batch_launch(c, v, fs, c->tabs.as<BatchTraj>(), c->work.as<int2>(), nitems, (k / fs) & 1,
+                                      nullptr, 0, 0);
+            float t = 0.f;
+            if (rc == ASD_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                                 hipEventElapsedTime(&t, e0, e1) != hipSuccess))
+                rc = set_error(ASD_EHIP, "batch autotune timing failed");
+            if (rep > 0) ms = std::min(ms, (double)t);  // the first run warms up
+        }
+        if (times_ms) times_ms[v] = ms;
+        if (ms < best_ms) {
+            best_ms = ms;
+            bi = v;
+        }
+    }
+    (void)hipStreamSynchronize(st);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != ASD_OK) return rc;
+    c->batch_tuned[batch_key(ntraj, Ls)] = bi;
+    if (best) *best = bi;
+    return ASD_OK;
+}
+
+int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *params, double *const *d_spins,
+                      long nsteps, int snap_every, int nsnap, double *const *d_snaps)
+{
+    ASD_CHECK(c && d_spins, "bad arguments");
+    ASD_CHECK(nsteps >= 0, "nsteps < 0");
+    ASD_CHECK(!d_snaps || (snap_every >= 1 && nsnap >= 0), "snap_every must be >= 1");
+    long total = 0;
+    ASD_TRY(check_batch(ntraj, Ls, params, &total));
+    for (int t = 0; t < ntraj; t++) ASD_CHECK(d_spins[t], "d_spins[%d] is NULL", t);
+    ASD_HIP(hipSetDevice(c->device));
+    // the kernel variant: forced, recorded for this batch composition, or autotuned now
+    int v = c->batch_variant;
+    if (v < 0) {
+        auto it = c->batch_tuned.find(batch_key(ntraj, Ls));
+        if (it == c->batch_tuned.end()) {
+            ASD_TRY(asd_autotune_batch(c, ntraj, Ls, params, &v, nullptr));
+        } else {
+            v = it->second;
+        }
+    }
+    std::vector<BatchTraj> tab(ntraj);
     ASD_TRY(c->batch_b.ensure(sizeof(double) * total));
     ASD_TRY(c->counter.ensure(sizeof(int)));
-    long off = 0;
-    for (int t = 0; t < ntraj; t++) {
-        const int L = Ls[t], nt = (L + T - 1) / T;
+    for (long t = 0, off = 0; t < ntraj; off += 3L * Ls[t] * Ls[t], t++) {
         tab[t].a = d_spins[t];
         tab[t].b = c->batch_b.as<double>() + off;
         tab[t].snaps = d_snaps ? d_snaps[t] : nullptr;
         tab[t].p = make_llg(params[t]);
-        tab[t].L = L;
-        off += 3L * L * L;
-        for (int k = 0; k < nt * nt; k++) items.push_back(make_int2(t, k));
+        tab[t].L = Ls[t];
     }
-    // interleave trajectories tile by tile so that every stretch of the work list
-    // mixes lattice sizes (balanced persistent workgroups)
-    std::stable_sort(items.begin(), items.end(), [](const int2 &x, const int2 &y) { return x.y < y.y; });
-    const int nitems = (int)items.size();
-    ASD_TRY(c->tabs.ensure(sizeof(BatchTraj) * ntraj));
-    ASD_TRY(c->work.ensure(sizeof(int2) * nitems));
+    int nitems = 0;
+    ASD_TRY(batch_tables(c, tab, Ls, v, &nitems));
     hipStream_t st = c->stream;
-    ASD_HIP(hipMemcpyAsync(c->tabs.ptr, tab.data(), sizeof(BatchTraj) * ntraj, hipMemcpyHostToDevice, st));
-    ASD_HIP(hipMemcpyAsync(c->work.ptr, items.data(), sizeof(int2) * nitems, hipMemcpyHostToDevice, st));
-    ASD_HIP(hipStreamSynchronize(st));  // host tables go out of scope
     const BatchTraj *dt = c->tabs.as<BatchTraj>();
     const int2 *dw = c->work.as<int2>();
     int *ctr = c->counter.as<int>();
     ASD_TRY(mark(c, 5));
     hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, ctr, 0);
+    ASD_TRY(c->guard.ensure(sizeof(unsigned long long)));
+    unsigned long long *first = c->guard.as<unsigned long long>();
+    hipLaunchKernelGGL(set_u64_kernel, dim3(1), dim3(1), 0, st, first, kGuardNone);
 
     const int every = d_snaps ? snap_every : 10;
+    // two fused steps per launch where the variant has them and no snapshot falls between them
+    const int fs = rk4_batch_variant_steps(v) == 2 && every % 2 == 0 ? 2 : 1;
     const long nsnap_in_loop = d_snaps ? std::min<long>(nsnap, (nsteps + every - 1) / every) : 0;
-    const int chunk = every % 2 ? 2 * every : every;
+    // a graph chunk holds an even number of launches, so the ping-pong parity repeats
+    long chunk = every;
+    if ((chunk / fs) % 2) chunk *= 2;
     long nchunks = nsteps / chunk;
     if (d_snaps) nchunks = std::min<long>(nchunks, nsnap_in_loop / (chunk / every));
+    if (chunk > kMaxGraphSteps) nchunks = 0;
     long s = 0;
     if (nchunks >= 1) {
         BatchGraph &g = c->bgraph;
         const uint64_t key = ((uint64_t)chunk << 40) ^ ((uint64_t)every << 32) ^ ((uint64_t)(d_snaps != nullptr) << 31) ^
-                             (uint64_t)nitems;
-        if (!(g.exec && g.key == key && g.trajs == (const void *)dt && g.items == (const void *)dw)) {
+                             ((uint64_t)c->rk4_exact << 30) ^ ((uint64_t)fs << 28) ^ (uint64_t)nitems;
+        if (!(g.exec && g.key == key && g.variant == v && g.trajs == (const void *)dt && g.items == (const void *)dw &&
+              g.guard == (const void *)first)) {
             if (g.exec) (void)hipGraphExecDestroy(g.exec);
             g = BatchGraph();
             hipStream_t cap;
@@ -519,10 +733,16 @@ int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *pa
             hipStream_t saved = c->stream;
             c->stream = cap;
             int r = ASD_OK;
-            for (int k = 0; k < chunk && r == ASD_OK; k++) {
+            for (int k = 0; k < chunk && r == ASD_OK; k += fs) {
                 const int snap_here = d_snaps && k % every == 0;
-                r = rk4_batch_launch(c, dt, dw, nitems, k & 1, ctr, snap_here, snap_here);
+                // with snapshots the counter is bumped after each snapshot launch; without,
+                // once at the end of the chunk, so it counts chunks
+                Guard gk = d_snaps ? chunk_guard(first, ctr, k, every) : chunk_guard(first, ctr, 0, (int)chunk);
+                if (!d_snaps) gk.step = k;
+                r = rk4_batch_launch(c, v, fs, dt, dw, nitems, (k / fs) & 1, ctr, snap_here, snap_here, gk);
             }
+            if (r == ASD_OK && !d_snaps)
+                hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(1), 0, c->stream, ctr);
             c->stream = saved;
             hipError_t e1 = hipStreamEndCapture(cap, &graph);
             (void)hipStreamDestroy(cap);
@@ -532,30 +752,40 @@ int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *pa
             (void)hipGraphDestroy(graph);
             ASD_HIP(e2);
             g.key = key;
+            g.variant = v;
             g.trajs = dt;
             g.items = dw;
-            g.chunk = chunk;
+            g.guard = first;
+            g.chunk = (int)chunk;
             g.nitems = nitems;
         }
         for (long ch = 0; ch < nchunks; ch++) ASD_HIP(hipGraphLaunch(g.exec, st));
         s = nchunks * chunk;
     }
-    // remainder: the counter already holds the next snapshot slot
-    for (; s < nsteps; s++) {
+    // remainder, eagerly: the counter already holds the next snapshot slot; the state is
+    // back in the a buffers after the graph's even number of launches
+    int cur = 0;
+    while (s < nsteps) {
         const int snap_here = d_snaps && s % every == 0 && s / every < nsnap;
-        ASD_TRY(rk4_batch_launch(c, dt, dw, nitems, (int)(s & 1), ctr, snap_here, snap_here));
+        const bool mid_snap = d_snaps && (s + 1) % every == 0 && (s + 1) / every < nsnap;
+        const int k = fs == 2 && s + 2 <= nsteps && !mid_snap ? 2 : 1;
+        ASD_TRY(rk4_batch_launch(c, v, k, dt, dw, nitems, cur, ctr, snap_here, snap_here, eager_guard(first, s)));
+        cur ^= 1;
+        s += k;
     }
-    const bool odd = nsteps & 1;
     for (int t = 0; t < ntraj; t++) {
         const size_t bytes = sizeof(double) * 3 * (size_t)Ls[t] * Ls[t];
-        const double *fin = odd ? tab[t].b : tab[t].a;
+        const double *fin = cur ? tab[t].b : tab[t].a;
         if (d_snaps && d_snaps[t] && nsteps % every == 0 && nsteps / every < nsnap)
             ASD_HIP(hipMemcpyAsync(d_snaps[t] + (nsteps / every) * 3L * Ls[t] * Ls[t], fin, bytes,
                                    hipMemcpyDeviceToDevice, st));
-        if (odd) ASD_HIP(hipMemcpyAsync(d_spins[t], tab[t].b, bytes, hipMemcpyDeviceToDevice, st));
+        if (cur) ASD_HIP(hipMemcpyAsync(d_spins[t], tab[t].b, bytes, hipMemcpyDeviceToDevice, st));
     }
     ASD_TRY(mark(c, 6));
-    return ASD_OK;
+    unsigned long long h = kGuardNone;
+    ASD_HIP(hipMemcpyAsync(&h, first, sizeof h, hipMemcpyDeviceToHost, st));
+    ASD_HIP(hipStreamSynchronize(st));
+    return guard_verdict(h, true);
 }
 
 // ------------------------------------------------ reference-compatible entries

@@ -56,8 +56,25 @@ struct LLGParams {
     // operands of a VOP3 FMA, where a literal cannot (gfx9 encodings)
     double six, three, three_halves;
     double half_dt;     // dt / 2 (exact)
+    // the tolerance mode's constants: 2 D, and the RK4 weights dt/6 and dt/3 of the increments
+    double two_D, w6, w3;
 };
 LLGParams make_llg(const asd_params &p);
+
+// Numeric guard of the RK4 kernels.  A site whose unnormalised state c has |c|^2 outside
+// [2^-900, 2^900] (NaN and infinities included) is outside the range where the step is
+// defined (the renormalisation's reciprocal square root needs it; after a valid step
+// |S| = spin_length to rounding).  The kernels then record (step << 20 | trajectory) by
+// atomicMin into *first, so the host learns the first such step.  The step of a launch is
+// `step`, plus `every` times the device counter *ctr when ctr is set (graph replay, where
+// the launch's arguments are fixed at capture).  first == NULL: no check.
+struct Guard {
+    unsigned long long *first = nullptr;
+    const int *ctr = nullptr;
+    long step = 0;
+    int every = 0;
+};
+constexpr unsigned long long kGuardNone = ~0ull;
 
 // one trajectory of a batch: state ping-pong a <-> b, snapshots [nsnap][L*L][3]
 struct BatchTraj {
@@ -67,16 +84,16 @@ struct BatchTraj {
 };
 struct BatchGraph {
     hipGraphExec_t exec = nullptr;
-    int chunk = 0, nitems = 0;
-    const void *trajs = nullptr, *items = nullptr;
+    int chunk = 0, nitems = 0, variant = -1;
+    const void *trajs = nullptr, *items = nullptr, *guard = nullptr;
     uint64_t key = 0;
 };
 
 struct RK4Graph {
     hipGraphExec_t exec = nullptr;
     // key
-    int L = 0, nrep = 0, every = 0, chunk = 0, tile = 0, fuse = 1;
-    const void *a = nullptr, *b = nullptr, *snaps = nullptr;
+    int L = 0, nrep = 0, every = 0, chunk = 0, tile = 0, fuse = 1, exact = 1;
+    const void *a = nullptr, *b = nullptr, *snaps = nullptr, *guard = nullptr;
     LLGParams p{};
     long snap_stride = 0;
 };
@@ -87,21 +104,26 @@ struct asd_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
     int num_cu = 0;
-    // RK4 ping-pong state and the device-side snapshot counter
-    asd::DevBuf state_b, counter;
+    // RK4 ping-pong state, the device-side snapshot counter and the numeric guard word
+    asd::DevBuf state_b, counter, guard;
     asd::RK4Graph graph;
     // S(q,t) workspace
-    asd::DevBuf opA, opB, opA2, opB2, bins, fq, twiddle, qdev, outdev, stats;
+    asd::DevBuf opA, opB, opA2, opB2, bins, fq, twiddle, qdev, outdev, fix;
     // compat (host-pointer) staging
     asd::DevBuf c0, c1, c2, c3, c4, c5, c6;
     // batching: work list, trajectory table, second state buffer
     asd::DevBuf work, tabs, batch_b;
     asd::BatchGraph bgraph;
+    // batch kernel variant: forced (>= 0) or -1 (autotuned per batch composition, the sorted Ls)
+    int batch_variant = -1;
+    std::map<std::vector<int>, int> batch_tuned;
     // autotuner choice per precision (-1 = automatic); tuned per (L, K)
     int forced_cfg[4] = {-1, -1, -1, -1};
     std::map<std::pair<int, int>, int> tuned[4];
     // RK4 tile edge: 0 = automatic; 8, 16 or 32 (LDS tiles) or 64 (register streaming)
     int rk4_tile = 0;
+    // RK4 arithmetic: 1 = the reference's operation order (IEEE-equal values), 0 = the tolerance mode
+    int rk4_exact = 1;
     // 1: S(q,tau) always takes the per-q path (the adjoint needs its F_q(t) sums)
     int sqt_generic_only = 0;
     // stage timing (asd_set_timing): events around pack, gemm, fourier, finish, rk4
@@ -118,16 +140,26 @@ namespace asd {
 // record stage event k on the context stream when timing is on
 int mark(asd_ctx *ctx, int k);
 // internal entry points shared between translation units
-int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items, int nitems, int parity,
-                     const int *snap_counter, int write_snap, int bump_counter);
+// one launch of batch kernel variant `variant` doing `steps` (1, or 2 for a two-step variant)
+int rk4_batch_launch(asd_ctx *ctx, int variant, int steps, const BatchTraj *d_trajs, const int2 *d_items,
+                     int nitems, int parity, const int *snap_counter, int write_snap, int bump_counter,
+                     const Guard &guard = Guard());
+int rk4_batch_num_variants();
+const char *rk4_batch_variant_name(int v);
+int rk4_batch_variant_tile(int v);
+int rk4_batch_variant_steps(int v);
 // one launch of `steps` RK4 steps (1, or 2 where rk4_fusable_steps allows it); the
 // snapshot, if any, is of the input state
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in,
                     double *out, double *snap_out, const int *snap_counter, long snap_stride,
-                    int bump_counter, int steps = 1);
-// steps one rk4_launch_step may fuse for this lattice: 2 where the automatic choice is the
-// small-lattice 8 x 8 tiles (one step is latency-bound there), else 1
+                    int bump_counter, int steps = 1, const Guard &guard = Guard());
+// steps one rk4_launch_step may fuse for this lattice: 2 where the automatic choice is an
+// LDS tile path whose single step is latency-bound (8 x 8, 32 x 32 and 16 x 16 tiles with
+// 1024 threads), else 1
 int rk4_fusable_steps(const asd_ctx *ctx, int L, int nrep);
+// tiny kernels shared by the drivers
+__global__ void set_u64_kernel(unsigned long long *p, unsigned long long v);
+__global__ void bump_counter_kernel(int *c);
 int rk4_compat_stages(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *spin,
                       double *cur, double *k1, double *k2, double *k3, double *k4);
 int rk4_stages_store(asd_ctx *ctx, const LLGParams &p, int L, const double *S, double *Y2, double *Y3, double *Y4,

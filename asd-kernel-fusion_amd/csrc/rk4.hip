@@ -19,6 +19,9 @@
 // increment f dt/2 (k/2 exactly), the quotients k/6, k/3 and the renormalisation take
 // FMA-corrected reciprocals, and the square root and division drop their range wrappers
 // (|S|^2 stays within 2^+-900 of 1).
+#include <map>
+#include <mutex>
+
 #include "asd_internal.h"
 
 namespace asd {
@@ -39,6 +42,9 @@ LLGParams make_llg(const asd_params &p)
     q.three = 3.0;
     q.three_halves = 1.5;
     q.half_dt = 0.5 * p.dt;
+    q.two_D = 2.0 * p.D;
+    q.w6 = p.dt / 6.0;
+    q.w3 = p.dt / 3.0;
     return q;
 }
 
@@ -48,7 +54,9 @@ LLGParams make_llg(const asd_params &p)
 // UNDAMPED (alpha == 0): (p + 0 q) * 1 is p, so the damping term is dropped; the value
 // is the same (IEEE ==), only an exact -0 in p may come out as +0 the other way.
 // NO_BZ (B_z == 0): the add of B_z is dropped (hz + 0.0 is hz, up to the sign of a zero).
-template <bool UNIT_J = false, bool UNDAMPED = false, bool NO_BZ = false>
+// FAST (the tolerance mode): the anisotropy term, the cross products and the damping term
+// as fused multiply-adds (one rounding where the reference's expressions round twice).
+template <bool UNIT_J = false, bool UNDAMPED = false, bool NO_BZ = false, bool FAST = false>
 __device__ __forceinline__ void llg_f(const LLGParams &p, double sx, double sy, double sz,
                                       const double *n0, const double *n1, const double *n2,
                                       const double *n3, double &fx, double &fy, double &fz)
@@ -68,6 +76,24 @@ __device__ __forceinline__ void llg_f(const LLGParams &p, double sx, double sy, 
         hx += n1[0] * p.J; hy += n1[1] * p.J; hz += n1[2] * p.J;
         hx += n2[0] * p.J; hy += n2[1] * p.J; hz += n2[2] * p.J;
         hx += n3[0] * p.J; hy += n3[1] * p.J; hz += n3[2] * p.J;
+    }
+    if constexpr (FAST) {
+        hz = __builtin_fma(p.two_D, sz, hz);
+        if constexpr (!NO_BZ) hz += p.Bz;
+        const double px = __builtin_fma(hy, sz, -(hz * sy));
+        const double py = __builtin_fma(hz, sx, -(hx * sz));
+        const double pz = __builtin_fma(hx, sy, -(hy * sx));
+        if constexpr (UNDAMPED) {
+            fx = px; fy = py; fz = pz;
+        } else {
+            const double qx = __builtin_fma(sy, pz, -(sz * py));
+            const double qy = __builtin_fma(sz, px, -(sx * pz));
+            const double qz = __builtin_fma(sx, py, -(sy * px));
+            fx = __builtin_fma(p.a_over_len, qx, px) * p.inv_den;
+            fy = __builtin_fma(p.a_over_len, qy, py) * p.inv_den;
+            fz = __builtin_fma(p.a_over_len, qz, pz) * p.inv_den;
+        }
+        return;
     }
     hz += 2.0 * p.D * sz;
     if constexpr (!NO_BZ) hz += p.Bz;
@@ -128,9 +154,10 @@ __device__ __forceinline__ void renorm(double len, double ax, double ay, double 
 // nonzero len a_i stay above 2^-900 and below 2^900 (the launcher's parameter range
 // keeps spin_length there).
 // UNIT_LEN (spin_length == 1): the multiply 1 * a (exact) is dropped.
+// Returns |a|^2, which the numeric guard checks.
 template <bool UNIT_LEN = false>
-__device__ __forceinline__ void renorm_fast(double len, double ax, double ay, double az, double &ox, double &oy,
-                                            double &oz)
+__device__ __forceinline__ double renorm_fast(double len, double ax, double ay, double az, double &ox, double &oy,
+                                              double &oz)
 {
     const double x = ax * ax + ay * ay + az * az;
     const double rs = __builtin_amdgcn_rsq(x);
@@ -153,8 +180,36 @@ __device__ __forceinline__ void renorm_fast(double len, double ax, double ay, do
         return __builtin_fma(r, y, q);
     };
     ox = one(ax); oy = one(ay); oz = one(az);
+    return x;
+}
+// The tolerance mode's renormalisation: y = 1/sqrt(|a|^2) from v_rsq_f64 refined by two
+// Newton steps y <- y + (y/2)(1 - |a|^2 y^2) (each squares the relative error; the result
+// is within a few ulp of 1/|a|), then len * y * a: |S| = len to ~1e-16 in place of the
+// correctly rounded quotients.  Returns |a|^2 for the numeric guard.
+template <bool UNIT_LEN = false>
+__device__ __forceinline__ double renorm_lean(double len, double ax, double ay, double az, double &ox, double &oy,
+                                              double &oz)
+{
+    const double x = __builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax));
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int it = 0; it < 2; it++) {
+        const double r = __builtin_fma(-(x * y), y, 1.0);
+        y = __builtin_fma(0.5 * y, r, y);
+    }
+    const double s = UNIT_LEN ? y : len * y;
+    ox = ax * s; oy = ay * s; oz = az * s;
+    return x;
 }
 constexpr double kSixth = 1.0 / 6.0, kThird = 1.0 / 3.0, kTwoThirds = 2.0 / 3.0;
+
+// the numeric guard (Guard in asd_internal.h): |c|^2 outside [2^-900, 2^900] or NaN
+__device__ __forceinline__ bool out_of_range(double x) { return !(x >= 0x1p-900 && x <= 0x1p900); }
+__device__ __forceinline__ void guard_report(const Guard &g, int traj)
+{
+    const long s = g.ctr ? g.step + (long)g.every * *g.ctr : g.step;
+    atomicMin(g.first, ((unsigned long long)s << 20) | (unsigned)traj);
+}
 
 __device__ __forceinline__ int wrap(int x, int L)
 {
@@ -185,10 +240,10 @@ template <int TR, int TC, int NS = 1> struct TileLDS {
     double Y[2][3][NR];
 };
 
-template <int TR, int TC, bool UJ = false, bool UD = false, int NT = TR * TC, int NS = 1>
+template <int TR, int TC, bool UJ = false, bool UD = false, int NT = TR * TC, int NS = 1, bool FAST = false>
 __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParams &p, int L, int tile_r, int tile_c,
                                          const double *__restrict__ in, double *__restrict__ out,
-                                         double *__restrict__ snap)
+                                         double *__restrict__ snap, const Guard &gd, int traj)
 {
     constexpr int R = 4 * NS, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
     constexpr int W = RC - 2, NW = (RR - 2) * W;  // the widest stage region
@@ -262,8 +317,25 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
                     n[q][0] = Yin[0][nb[q]]; n[q][1] = Yin[1][nb[q]]; n[q][2] = Yin[2][nb[q]];
                 }
                 double fx, fy, fz;
-                llg_f<UJ, UD>(p, y[j][0], y[j][1], y[j][2], n[0], n[1], n[2], n[3], fx, fy, fz);
+                llg_f<UJ, UD, false, FAST>(p, y[j][0], y[j][1], y[j][2], n[0], n[1], n[2], n[3], fx, fy, fz);
                 const double h = stage <= 2 ? p.half_dt : p.dt;
+                if constexpr (FAST) {
+                    // Y = S + f h and the partial sum S + f dt/6 + f dt/3 + ... as single FMAs
+                    const double f[3] = {fx, fy, fz};
+                    const double wt = (stage == 1 || stage == 4) ? p.w6 : p.w3;
+                    if (stage < 4) {
+#pragma unroll
+                        for (int d = 0; d < 3; d++) {
+                            y[j][d] = __builtin_fma(f[d], h, S[j][d]);
+                            Yout[d][i] = y[j][d];
+                        }
+                    }
+                    if (depth[j] >= 4 * m + 4) {
+#pragma unroll
+                        for (int d = 0; d < 3; d++) A[j][d] = __builtin_fma(f[d], wt, stage == 1 ? S[j][d] : A[j][d]);
+                    }
+                    continue;
+                }
                 const double k[3] = {fx * h, fy * h, fz * h};
                 if (stage < 4) {  // the next stage's input: S + kh (stages 1, 2) or S + k3
 #pragma unroll
@@ -294,7 +366,9 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
 #pragma unroll
             for (int j = 0; j < NE; j++) {
                 if (depth[j] < 4 * m + 4) continue;
-                renorm_fast(p.len, A[j][0], A[j][1], A[j][2], S[j][0], S[j][1], S[j][2]);
+                const double x = FAST ? renorm_lean(p.len, A[j][0], A[j][1], A[j][2], S[j][0], S[j][1], S[j][2])
+                                      : renorm_fast(p.len, A[j][0], A[j][1], A[j][2], S[j][0], S[j][1], S[j][2]);
+                if (gd.first && out_of_range(x)) guard_report(gd, traj);
                 sS[0][idx[j]] = S[j][0]; sS[1][idx[j]] = S[j][1]; sS[2][idx[j]] = S[j][2];
             }
             __syncthreads();
@@ -306,7 +380,9 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
                 const int grow = tile_r * TR + er - R, gcol = tile_c * TC + ec - R;
                 if (grow >= L || gcol >= L) continue;
                 double *o = out + 3 * ((long)grow * L + gcol);
-                renorm_fast(p.len, A[j][0], A[j][1], A[j][2], o[0], o[1], o[2]);
+                const double x = FAST ? renorm_lean(p.len, A[j][0], A[j][1], A[j][2], o[0], o[1], o[2])
+                                      : renorm_fast(p.len, A[j][0], A[j][1], A[j][2], o[0], o[1], o[2]);
+                if (gd.first && out_of_range(x)) guard_report(gd, traj);
             }
         }
     }
@@ -314,32 +390,35 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
 
 // single lattice (nrep replicas along blockIdx.z); snapshot slot from *snap_counter
 // when given (graph replay), else slot 0 of snap
-template <int TR, int TC, bool UJ, bool UD, int NT = TR * TC, int NS = 1>
+template <int TR, int TC, bool UJ, bool UD, int NT = TR * TC, int NS = 1, bool FAST = false>
 __global__ __launch_bounds__(NT) void rk4_step_kernel(LLGParams p, int L,
                                                           const double *__restrict__ in,
                                                           double *__restrict__ out,
                                                           double *__restrict__ snap,
                                                           const int *__restrict__ snap_counter,
-                                                          long snap_stride)
+                                                          long snap_stride, Guard gd)
 {
     __shared__ TileLDS<TR, TC, NS> sm;
     const long base = (long)blockIdx.z * L * L * 3;
     double *sn = nullptr;
     if (snap) sn = snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride + base;
-    rk4_tile<TR, TC, UJ, UD, NT, NS>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn);
+    rk4_tile<TR, TC, UJ, UD, NT, NS, FAST>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn, gd,
+                                           blockIdx.z);
 }
 
 // ------------------------------------------------------------------ batching
-// Uneven-size independent trajectories in one persistent launch per step: the
-// grid strides over a work list of (trajectory, tile) items; every trajectory
-// has its own L, parameters and state/snapshot buffers (BatchTraj table).
-template <int TR, int TC>
-__global__ __launch_bounds__(TR *TC) void rk4_batch_kernel(const BatchTraj *__restrict__ trajs,
-                                                           const int2 *__restrict__ items, int nitems,
-                                                           int parity, const int *__restrict__ snap_counter,
-                                                           int write_snap)
+// Uneven-size independent trajectories in one persistent launch per step (or per two
+// fused steps): the grid strides over a work list of (trajectory, tile) items; every
+// trajectory has its own L, parameters and state/snapshot buffers (BatchTraj table).  The
+// tile shape, threads per tile and steps per launch are one of the variants below, chosen
+// by the batch autotuner; every variant gives the same bits (rk4_tile's arithmetic does
+// not depend on the shape or on NS).
+template <int TR, int TC, int NT = TR * TC, int NS = 1, bool FAST = false>
+__global__ __launch_bounds__(NT) void rk4_batch_kernel(const BatchTraj *__restrict__ trajs,
+                                                       const int2 *__restrict__ items, int nitems, int parity,
+                                                       const int *__restrict__ snap_counter, int write_snap, Guard gd)
 {
-    __shared__ TileLDS<TR, TC> sm;
+    __shared__ TileLDS<TR, TC, NS> sm;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
         const int2 w = items[it];
         const BatchTraj &tr = trajs[w.x];
@@ -349,9 +428,9 @@ __global__ __launch_bounds__(TR *TC) void rk4_batch_kernel(const BatchTraj *__re
         double *sn = (write_snap && tr.snaps) ? tr.snaps + (long)*snap_counter * 3L * tr.L * tr.L : nullptr;
         __syncthreads();  // LDS of the previous item fully consumed
         if (tr.p.J == 1.0 && tr.p.alpha == 0.0)
-            rk4_tile<TR, TC, true, true>(sm, tr.p, tr.L, w.y / ntc, w.y % ntc, in, out, sn);
+            rk4_tile<TR, TC, true, true, NT, NS, FAST>(sm, tr.p, tr.L, w.y / ntc, w.y % ntc, in, out, sn, gd, w.x);
         else
-            rk4_tile<TR, TC>(sm, tr.p, tr.L, w.y / ntc, w.y % ntc, in, out, sn);
+            rk4_tile<TR, TC, false, false, NT, NS, FAST>(sm, tr.p, tr.L, w.y / ntc, w.y % ntc, in, out, sn, gd, w.x);
     }
 }
 
@@ -392,11 +471,10 @@ __device__ __forceinline__ double lane_from_right(double v)  // lane l receives 
     return __hiloint2double(hi, lo);
 }
 
-// k = h f(c) (h = dt, or dt/2 for the halved increment) with row+1 = up, row-1 = dn and
-// the column neighbours from the adjacent lanes
-template <bool UJ, bool UD, bool PL = false>
-__device__ __forceinline__ void stream_k(const LLGParams &p, const double (&c)[3], const double (&up)[3],
-                                         const double (&dn)[3], double h, double (&k)[3])
+// f(c) with row+1 = up, row-1 = dn and the column neighbours from the adjacent lanes
+template <bool UJ, bool UD, bool PL = false, bool FAST = false>
+__device__ __forceinline__ void stream_f(const LLGParams &p, const double (&c)[3], const double (&up)[3],
+                                         const double (&dn)[3], double (&f)[3])
 {
     double lf[3], rt[3];
 #pragma unroll
@@ -404,19 +482,28 @@ __device__ __forceinline__ void stream_k(const LLGParams &p, const double (&c)[3
         lf[d] = lane_from_left(c[d]);
         rt[d] = lane_from_right(c[d]);
     }
-    double fx, fy, fz;
-    llg_f<UJ, UD, PL>(p, c[0], c[1], c[2], up, dn, lf, rt, fx, fy, fz);
-    k[0] = fx * h; k[1] = fy * h; k[2] = fz * h;
+    llg_f<UJ, UD, PL, FAST>(p, c[0], c[1], c[2], up, dn, lf, rt, f[0], f[1], f[2]);
+}
+// k = h f(c) (h = dt, or dt/2 for the halved increment)
+template <bool UJ, bool UD, bool PL = false>
+__device__ __forceinline__ void stream_k(const LLGParams &p, const double (&c)[3], const double (&up)[3],
+                                         const double (&dn)[3], double h, double (&k)[3])
+{
+    stream_f<UJ, UD, PL>(p, c, up, dn, k);
+    k[0] *= h; k[1] *= h; k[2] *= h;
 }
 
 // grid: 4 waves per block, one work item (replica, band, strip) per wave; R % 4 == 0,
 // 8 <= R <= L - 8, L >= 64 (checked by the launcher)
 // PL (B_z == 0 and spin_length == 1, the plain workloads): both exact shortcuts above.
-template <bool UJ, bool UD, bool PL>
+// FAST: the tolerance mode (fused multiply-adds for the stage inputs and the partial sums,
+// renorm_lean).
+template <bool UJ, bool UD, bool PL, bool FAST = false>
 __global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int nrep, int R, int nband,
                                                          int nstrip, const double *__restrict__ in,
                                                          double *__restrict__ out, double *__restrict__ snap,
-                                                         const int *__restrict__ snap_counter, long snap_stride)
+                                                         const int *__restrict__ snap_counter, long snap_stride,
+                                                         Guard gd)
 {
     const int lane = threadIdx.x & 63;
     // the wave's index as a scalar: everything derived from it (band, rows, the row of
@@ -518,6 +605,38 @@ __global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int
             // exactly (RN(f dt)/2 = RN(f dt/2) while k is above the subnormal range), so
             // Y = S + k/2 is S + kh and k/6, k/3 are the correctly rounded kh/3, kh/1.5.
             double k[3];
+            const int orow = t - 2 * kStreamHalo;  // band-relative output row
+            if constexpr (FAST) {
+                double f[3];
+                stream_f<UJ, UD, PL, true>(p, B[b1], B[b0], B[b2], f);  // stage 1 at row t-1
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    A[i1][d][lane] = __builtin_fma(f[d], p.w6, B[b1][d]);
+                    Y2[i1][d] = __builtin_fma(f[d], p.half_dt, B[b1][d]);
+                }
+                stream_f<UJ, UD, PL, true>(p, Y2[i2], Y2[i1], Y2[i3], f);  // stage 2 at row t-2
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    A[i2][d][lane] = __builtin_fma(f[d], p.w3, A[i2][d][lane]);
+                    Y3[i2][d] = __builtin_fma(f[d], p.half_dt, B[b2][d]);
+                }
+                stream_f<UJ, UD, PL, true>(p, Y3[i3], Y3[i2], Y3[i4], f);  // stage 3 at row t-3
+#pragma unroll
+                for (int d = 0; d < 3; d++) {
+                    A[i3][d][lane] = __builtin_fma(f[d], p.w3, A[i3][d][lane]);
+                    Y4[i3][d] = __builtin_fma(f[d], p.dt, B[b3][d]);
+                }
+                stream_f<UJ, UD, PL, true>(p, Y4[i4], Y4[i3], Y4[i5], f);  // stage 4 at row t-4
+#pragma unroll
+                for (int d = 0; d < 3; d++) A[i4][d][lane] = __builtin_fma(f[d], p.w6, A[i4][d][lane]);
+                if (ocol && orow >= 0 && orow < rows) {
+                    double *o = at_w(dst + (r0 + orow) * row_stride);
+                    const double x =
+                        renorm_lean<PL>(p.len, A[i4][0][lane], A[i4][1][lane], A[i4][2][lane], o[0], o[1], o[2]);
+                    if (gd.first && out_of_range(x)) guard_report(gd, rep);
+                }
+                continue;
+            }
             // stage 1 at row t-1: Y1 = S
             stream_k<UJ, UD, PL>(p, B[b1], B[b0], B[b2], p.half_dt, k);
 #pragma unroll
@@ -541,18 +660,20 @@ __global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int
             }
             // stage 4 at row t-4, then the new state of that row
             stream_k<UJ, UD, PL>(p, Y4[i4], Y4[i3], Y4[i5], p.dt, k);
-            const int orow = t - 2 * kStreamHalo;  // band-relative output row
 #pragma unroll
             for (int d = 0; d < 3; d++) A[i4][d][lane] = add_div(A[i4][d][lane], k[d], p.six, kSixth);
             if (ocol && orow >= 0 && orow < rows) {
                 double *o = at_w(dst + (r0 + orow) * row_stride);
-                renorm_fast<PL>(p.len, A[i4][0][lane], A[i4][1][lane], A[i4][2][lane], o[0], o[1], o[2]);
+                const double x =
+                    renorm_fast<PL>(p.len, A[i4][0][lane], A[i4][1][lane], A[i4][2][lane], o[0], o[1], o[2]);
+                if (gd.first && out_of_range(x)) guard_report(gd, rep);
             }
         }
     }
 }
 
 __global__ void bump_counter_kernel(int *c) { *c += 1; }
+__global__ void set_u64_kernel(unsigned long long *p, unsigned long long v) { *p = v; }
 
 // ------------------------------------------------- compat: euler / runge_kutta
 
@@ -653,29 +774,92 @@ int rk4_stages_store(asd_ctx *ctx, const LLGParams &p, int L, const double *S, d
 
 // ------------------------------------------------------------- step launcher
 
-int rk4_batch_launch(asd_ctx *ctx, const BatchTraj *d_trajs, const int2 *d_items, int nitems, int parity,
-                     const int *snap_counter, int write_snap, int bump_counter)
+// Batch kernel variants (tile edge, threads per tile, steps per launch) the autotuner times.
+// The two-step variants also have a one-step kernel of the same shape for launches that may
+// not fuse (a single trailing step, odd snapshot spacing).
+struct BatchVariant {
+    int tile, threads, steps;
+    const char *name;
+};
+static const BatchVariant kBatchVariants[] = {
+    {16, 256, 1, "16x16 tiles, 256 threads, one step per launch"},
+    {16, 1024, 2, "16x16 tiles, 1024 threads, two steps per launch"},
+    {8, 512, 2, "8x8 tiles, 512 threads, two steps per launch"},
+    {32, 1024, 2, "32x32 tiles, 1024 threads, two steps per launch"},
+    {8, 256, 1, "8x8 tiles, 256 threads, one step per launch"},
+};
+constexpr int kNumBatchVariants = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
+int rk4_batch_num_variants() { return kNumBatchVariants; }
+const char *rk4_batch_variant_name(int v) { return v >= 0 && v < kNumBatchVariants ? kBatchVariants[v].name : "invalid"; }
+int rk4_batch_variant_tile(int v) { return kBatchVariants[v].tile; }
+int rk4_batch_variant_steps(int v) { return kBatchVariants[v].steps; }
+
+using BatchKernel = void (*)(const BatchTraj *, const int2 *, int, int, const int *, int, Guard);
+
+template <bool F>
+static BatchKernel batch_kernel(int v, int ns)
 {
-    constexpr int TR = 16, TC = 16;
-    const int grid = std::min(nitems, 4 * std::max(ctx->num_cu, 1));
-    hipLaunchKernelGGL((rk4_batch_kernel<TR, TC>), dim3(grid), dim3(TR * TC), 0, ctx->stream, d_trajs, d_items,
-                       nitems, parity, snap_counter, write_snap);
+    switch (v) {
+    case 1: return ns == 2 ? rk4_batch_kernel<16, 16, 1024, 2, F> : rk4_batch_kernel<16, 16, 1024, 1, F>;
+    case 2: return ns == 2 ? rk4_batch_kernel<8, 8, 512, 2, F> : rk4_batch_kernel<8, 8, 512, 1, F>;
+    case 3: return ns == 2 ? rk4_batch_kernel<32, 32, 1024, 2, F> : rk4_batch_kernel<32, 32, 1024, 1, F>;
+    case 4: return rk4_batch_kernel<8, 8, 256, 1, F>;
+    default: return rk4_batch_kernel<16, 16, 256, 1, F>;
+    }
+}
+
+// persistent grid: the blocks of kernel k resident at once (cached per kernel)
+static int batch_grid(BatchKernel k, int threads, int nitems, int num_cu)
+{
+    static std::mutex mu;
+    static std::map<const void *, int> per_cu;
+    int n = 0;
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = per_cu.find((const void *)k);
+        if (it == per_cu.end()) {
+            int b = 0;
+            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, threads, 0) != hipSuccess || b < 1) b = 1;
+            it = per_cu.emplace((const void *)k, b).first;
+        }
+        n = it->second;
+    }
+    return std::max(1, std::min(nitems, n * std::max(num_cu, 1)));
+}
+
+int rk4_batch_launch(asd_ctx *ctx, int variant, int steps, const BatchTraj *d_trajs, const int2 *d_items,
+                     int nitems, int parity, const int *snap_counter, int write_snap, int bump_counter,
+                     const Guard &guard)
+{
+    ASD_CHECK(variant >= 0 && variant < kNumBatchVariants, "batch variant %d out of range", variant);
+    ASD_CHECK(steps == 1 || (steps == 2 && kBatchVariants[variant].steps == 2), "variant %d cannot fuse %d steps",
+              variant, steps);
+    const BatchKernel k = ctx->rk4_exact ? batch_kernel<false>(variant, steps) : batch_kernel<true>(variant, steps);
+    const int threads = kBatchVariants[variant].threads;
+    hipLaunchKernelGGL(k, dim3(batch_grid(k, threads, nitems, ctx->num_cu)), dim3(threads), 0, ctx->stream, d_trajs,
+                       d_items, nitems, parity, snap_counter, write_snap, guard);
     if (bump_counter)
         hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(1), 0, ctx->stream, (int *)snap_counter);
     ASD_HIP(hipGetLastError());
     return ASD_OK;
 }
 
+template <int TR, int TC, int NT, int NS, bool F>
+static auto tile_kernel(bool uj, bool ud)
+{
+    return uj && ud ? rk4_step_kernel<TR, TC, true, true, NT, NS, F>
+                    : (uj ? rk4_step_kernel<TR, TC, true, false, NT, NS, F>
+                          : (ud ? rk4_step_kernel<TR, TC, false, true, NT, NS, F>
+                                : rk4_step_kernel<TR, TC, false, false, NT, NS, F>));
+}
+
 template <int TR, int TC, int NT, int NS = 1>
 static void launch_tiles(asd_ctx *ctx, const LLGParams &p, bool uj, bool ud, int L, int nrep, const double *in,
-                         double *out, double *snap_out, const int *snap_counter, long snap_stride)
+                         double *out, double *snap_out, const int *snap_counter, long snap_stride, const Guard &g)
 {
     const dim3 grid((unsigned)((L + TC - 1) / TC), (unsigned)((L + TR - 1) / TR), (unsigned)nrep);
-    auto k = uj && ud ? rk4_step_kernel<TR, TC, true, true, NT, NS>
-                      : (uj ? rk4_step_kernel<TR, TC, true, false, NT, NS>
-                            : (ud ? rk4_step_kernel<TR, TC, false, true, NT, NS>
-                                  : rk4_step_kernel<TR, TC, false, false, NT, NS>));
-    hipLaunchKernelGGL(k, grid, dim3(NT), 0, ctx->stream, p, L, in, out, snap_out, snap_counter, snap_stride);
+    auto k = ctx->rk4_exact ? tile_kernel<TR, TC, NT, NS, false>(uj, ud) : tile_kernel<TR, TC, NT, NS, true>(uj, ud);
+    hipLaunchKernelGGL(k, grid, dim3(NT), 0, ctx->stream, p, L, in, out, snap_out, snap_counter, snap_stride, g);
 }
 
 // rows per band of the streaming kernel: 104 (measured at L = 1024 with 64 / 32 / 8
@@ -736,8 +920,17 @@ int rk4_fusable_steps(const asd_ctx *ctx, int L, int nrep)
                : 1;
 }
 
+template <bool F>
+static auto stream_kernel(bool uj, bool ud, bool pl)
+{
+    return uj && ud ? (pl ? rk4_stream_kernel<true, true, true, F> : rk4_stream_kernel<true, true, false, F>)
+                    : (uj ? rk4_stream_kernel<true, false, false, F>
+                          : (ud ? rk4_stream_kernel<false, true, false, F> : rk4_stream_kernel<false, false, false, F>));
+}
+
 int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *in, double *out,
-                    double *snap_out, const int *snap_counter, long snap_stride, int bump_counter, int steps)
+                    double *snap_out, const int *snap_counter, long snap_stride, int bump_counter, int steps,
+                    const Guard &guard)
 {
     ASD_CHECK(steps == 1 || (steps == 2 && rk4_fusable_steps(ctx, L, nrep) == 2), "cannot fuse %d steps here",
               steps);
@@ -755,13 +948,11 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
     const long waves = (long)nrep * nband * nstrip;
     if (rk4_streams(ctx, L, nrep)) {
         const bool pl = p.Bz == 0.0 && p.len == 1.0;
-        auto k = uj && ud ? (pl ? rk4_stream_kernel<true, true, true> : rk4_stream_kernel<true, true, false>)
-                          : (uj ? rk4_stream_kernel<true, false, false>
-                                : (ud ? rk4_stream_kernel<false, true, false> : rk4_stream_kernel<false, false, false>));
+        auto k = ctx->rk4_exact ? stream_kernel<false>(uj, ud, pl) : stream_kernel<true>(uj, ud, pl);
         hipLaunchKernelGGL(k, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, p, L, nrep, R, nband,
-                           nstrip, in, out, snap_out, snap_counter, snap_stride);
+                           nstrip, in, out, snap_out, snap_counter, snap_stride, guard);
     } else if (big) {
-        launch_tiles<32, 32, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+        launch_tiles<32, 32, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride, guard);
     } else {
         // tiles by occupancy, n16 / n32 tiles of 16 x 16 / 32 x 32 in the launch (per step,
         // one lattice unless stated): 8 x 8 tiles (four threads per output site) while 16 x 16
@@ -777,19 +968,19 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
         // the widest stage region, 22 x 22 or 30 x 30, per thread), or 32 x 32 tiles with 1024
         // (three sites of the 46 x 46 region per thread)
         if (steps == 2 && rk4_big_tiles(ctx, L, nrep))
-            launch_tiles<32, 32, 1024, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+            launch_tiles<32, 32, 1024, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride, guard);
         else if (steps == 2 && rk4_mid_tiles(ctx, L, nrep))
-            launch_tiles<16, 16, 1024, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+            launch_tiles<16, 16, 1024, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride, guard);
         else if (steps == 2)
-            launch_tiles<8, 8, 512, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+            launch_tiles<8, 8, 512, 2>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride, guard);
         else if (rk4_small_tiles(ctx, L, nrep))
-            launch_tiles<8, 8, 256>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+            launch_tiles<8, 8, 256>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride, guard);
         else if (rk4_big_tiles(ctx, L, nrep))
-            launch_tiles<32, 32, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+            launch_tiles<32, 32, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride, guard);
         else if (rk4_mid_tiles(ctx, L, nrep))
-            launch_tiles<16, 16, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+            launch_tiles<16, 16, 1024>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride, guard);
         else
-            launch_tiles<16, 16, 256>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride);
+            launch_tiles<16, 16, 256>(ctx, p, uj, ud, L, nrep, in, out, snap_out, snap_counter, snap_stride, guard);
     }
     if (bump_counter)
         hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(1), 0, ctx->stream, (int *)snap_counter);

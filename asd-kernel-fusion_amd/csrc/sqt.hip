@@ -42,6 +42,111 @@ typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int TILE = 128;  // tile edge of the FP64 / FP32 kernels; the split kernels' edges (128, 256) divide 2 TILE
 
+// ------------------------------------------------------- fixed-point bins
+// The displacement bins are sums of many products from many workgroups.  Floating-point
+// atomics would make them depend on the order the adds land in, so the bins are integers
+// instead: a product value v enters as the integer round(v 2^s), which integer atomics
+// add exactly, so the sums (and S(q,tau)) repeat bit for bit.  The integer is formed by
+// the magic-number rounding: for |x| < 2^51, t = x + 1.5 2^52 lies in [2^52, 2^53), where
+// the double spacing is 1, so t = round(x) + 1.5 2^52 and bits(t) - bits(1.5 2^52) =
+// round(x).  The kernels add bits(t) itself; each bin of a lag receives exactly Ns
+// products (one j per i at every displacement), so the decoder subtracts Ns bits(1.5 2^52)
+// (mod 2^64).  FP64 keeps a second word for the residual x - round(x) at a finer scale,
+// which makes its bins the exactly rounded sums; the other precisions (errors >= 1e-7)
+// keep one word.  The scale s comes from the largest snapshot component M, noted by the
+// packing kernels: |G_ij| <= K M^2 (K = 3 T0), so s is chosen to keep every |x| below
+// 2^50 and every bin below 2^62.
+constexpr double kMagic = 6755399441055744.0;  // 1.5 * 2^52
+constexpr unsigned long long kMagicBits = 0x4338000000000000ull;
+
+struct FixScale {
+    unsigned long long vmax_bits;  // bits of max |snapshot component| (non-negative doubles order as integers)
+    double m0, m1;                 // 2^s0 (first word) and 2^s1 (second word, relative to the first)
+    double d0, d1;                 // 2^-s0 and 2^-(s0 + s1) for the decoder
+    int bad;                       // a snapshot component is NaN or infinite
+};
+
+// max |v| over snapshots [0, nt) (components [t * tstride, t * tstride + n3)) into *vmax:
+// grid-stride over components (x) and snapshots (y), one atomic per block.  The bits of
+// non-negative doubles order as integers (NaN above Inf above every finite value).
+__global__ __launch_bounds__(256) void vmax_kernel(const double *__restrict__ snaps, long tstride, int nt, long n3,
+                                                   unsigned long long *__restrict__ vmax)
+{
+    __shared__ unsigned long long red[4];
+    unsigned long long b = 0;
+    const long stride = (long)gridDim.x * blockDim.x;
+    for (int t = blockIdx.y; t < nt; t += gridDim.y)
+        for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < n3; e += stride) {
+            const unsigned long long x = (unsigned long long)__double_as_longlong(fabs(snaps[t * tstride + e]));
+            b = x > b ? x : b;
+        }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long x = (unsigned long long)__shfl_xor((long long)b, o, 64);
+        b = x > b ? x : b;
+    }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < 4; w++) b = red[w] > b ? red[w] : b;
+        atomicMax(vmax, b);
+    }
+}
+
+__global__ void fix_reset_kernel(FixScale *f)
+{
+    f->vmax_bits = 0;
+    f->bad = 0;
+}
+
+// scales from the snapshot maximum (one thread); K = GEMM depth, Ns = sites
+__global__ void fix_scale_kernel(FixScale *f, int K, int Ns)
+{
+    const double M = __longlong_as_double((long long)f->vmax_bits);
+    int clog_ns = 0;
+    while ((1L << clog_ns) < (long)Ns) clog_ns++;
+    int e_el = 0;  // |G_ij| <= K M^2 <= 2^e_el
+    if (!(M <= 1.7976931348623157e308)) {
+        f->bad = 1;  // NaN or Inf
+    } else if (M > 0.0) {
+        int eM = 0, eK = 0;
+        (void)frexp(M, &eM);
+        (void)frexp((double)K, &eK);
+        e_el = 2 * eM + eK;
+    }
+    int s0 = min(50, 62 - clog_ns) - e_el;
+    s0 = max(-1000, min(1000, s0));
+    const int s1 = min(50, 62 - clog_ns);  // residuals are <= 1/2
+    f->m0 = ldexp(1.0, s0);
+    f->m1 = ldexp(1.0, s1);
+    f->d0 = ldexp(1.0, -s0);
+    f->d1 = ldexp(1.0, -s0 - s1);
+}
+
+// one product value into the bin words of LDS (word w of bin b at sb[w * nbins + b])
+template <int NW>
+__device__ __forceinline__ void fix_add(unsigned long long *sb, int nbins, int b, double v, double m0, double m1)
+{
+    const double t = __builtin_fma(v, m0, kMagic);
+    atomicAdd(&sb[b], (unsigned long long)__double_as_longlong(t));
+    if constexpr (NW == 2) {
+        const double r = __builtin_fma(v, m0, kMagic - t);  // v 2^s0 - round(v 2^s0), exact
+        atomicAdd(&sb[nbins + b], (unsigned long long)__double_as_longlong(__builtin_fma(r, m1, kMagic)));
+    }
+}
+
+// bins[w][lag][d] (integer words) -> g[lag][d] (doubles); NaN where a snapshot was not finite
+__global__ void fix_decode_kernel(const unsigned long long *__restrict__ w, long n, int nw, int Ns,
+                                  const FixScale *__restrict__ f, double *__restrict__ g)
+{
+    const long e = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const unsigned long long off = (unsigned long long)Ns * kMagicBits;
+    double v = (double)(long long)(w[e] - off) * f->d0;
+    if (nw == 2) v = __builtin_fma((double)(long long)(w[n + e] - off), f->d1, v);
+    g[e] = f->bad ? __builtin_nan("") : v;
+}
+
 // ------------------------------------------------------------------ packing
 
 // snapshots [t][i][c] -> k-major operand out[k][ld], k = 3 t + c; zero where t >= nt
@@ -145,14 +250,20 @@ __device__ __forceinline__ BinGeom bin_geom(int i0, int j0, int Ns, int L, int B
     return g;
 }
 
-__device__ __forceinline__ void flush_bins(const double *sb, const BinGeom &g, int L, double *gbins)
+// the tile's LDS bin words (word w of bin b at sb[w * nbins + b]) into the lag's global
+// bins (word w at gbins[w * wstride + d]); integer adds, so the order does not matter
+template <int NW>
+__device__ __forceinline__ void flush_bins(const unsigned long long *sb, const BinGeom &g, int L,
+                                           unsigned long long *gbins, long wstride)
 {
     for (int b = threadIdx.x; b < g.nbins; b += blockDim.x) {
-        const double v = sb[b];
-        if (v != 0.0) {
-            int dr = (g.drmin + b / L) % L;
-            if (dr < 0) dr += L;
-            unsafeAtomicAdd(&gbins[(long)dr * L + b % L], v);
+        int dr = (g.drmin + b / L) % L;
+        if (dr < 0) dr += L;
+        const long d = (long)dr * L + b % L;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            const unsigned long long v = sb[w * g.nbins + b];
+            if (v != 0) atomicAdd(&gbins[w * wstride + d], v);
         }
     }
 }
@@ -167,7 +278,9 @@ struct GemmArgs {
     int L, Ns, Kpad, nt, nlags;
     int K;           // k extent without the zero padding (3 T0)
     int group;       // tile-rows per group in the work order
-    double *bins;    // [nlags][Ns]
+    unsigned long long *bins;  // integer bin words [word][nlags][Ns]
+    long wstride;    // nlags * Ns: words between the first and second word of a bin
+    const FixScale *fix;
 };
 
 // ------------------------------------------------------------ FP64 / FP32
@@ -196,7 +309,7 @@ template <> struct KMajorMfma<float> {
 };
 
 template <class T>
-__global__ __launch_bounds__(256) void sqt_gemm_kmajor_kernel(GemmArgs args)
+__global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
 {
     constexpr int BK = 16, PAD = 16 / sizeof(T) * 2, LDS_LD = TILE + PAD;
     typedef typename KMajorMfma<T>::acc_t acc_t;
@@ -271,11 +384,14 @@ __global__ __launch_bounds__(256) void sqt_gemm_kmajor_kernel(GemmArgs args)
         __syncthreads();
     }
 
-    // ---- epilogue: displacement binning
+    // ---- epilogue: displacement binning (FP64 keeps the residual word)
+    constexpr int NW = std::is_same<T, double>::value ? 2 : 1;
+    static_assert(sizeof(smem_raw) >= NW * 8 * (2 * TILE + 3 * 1024), "bins of a tile at L = 1024");
     const int L = args.L, Ns = args.Ns;
     const BinGeom g = bin_geom(i0, j0, Ns, L);
-    double *sb = reinterpret_cast<double *>(smem_raw);
-    for (int b = tid; b < g.nbins; b += 256) sb[b] = 0.0;
+    const double m0 = args.fix->m0, m1 = args.fix->m1;
+    unsigned long long *sb = reinterpret_cast<unsigned long long *>(smem_raw);
+    for (int b = tid; b < NW * g.nbins; b += 256) sb[b] = 0;
     int rj[4], cj[4];
 #pragma unroll
     for (int n = 0; n < 4; n++) {
@@ -297,11 +413,11 @@ __global__ __launch_bounds__(256) void sqt_gemm_kmajor_kernel(GemmArgs args)
                 if (j >= Ns) continue;
                 int dc = ci - cj[n];
                 dc += dc < 0 ? L : 0;
-                atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], (double)acc[m][n][v]);
+                fix_add<NW>(sb, g.nbins, (ri - rj[n] - g.drmin) * L + dc, (double)acc[m][n][v], m0, m1);
             }
         }
     __syncthreads();
-    flush_bins(sb, g, L, args.bins + (long)w.lag * Ns);
+    flush_bins<NW>(sb, g, L, args.bins + (long)w.lag * Ns, args.wstride);
     }
 }
 
@@ -339,18 +455,18 @@ template <> struct SplitMfma<ASD_FP16X2> {
 
 // Epilogue of the 32x32x16 split kernels: bins one wave's accumulators (rows
 // i0 + r0 + 32 m + 8 (v / 4) + 4 (lane / 32) + v % 4, columns j0 + c0 + 32 n + lane % 32)
-// into the tile's displacement bins in LDS (sb, zeroed here; the caller has finished
-// with that LDS) and adds them to the lag's global bins.  FP16X2 keeps the cross
-// products, scaled by 2^11, in acc2.
+// into the tile's displacement bins in LDS (sb: one fixed-point word per bin, zeroed here;
+// the caller has finished with that LDS) and adds them to the lag's global bins.  FP16X2
+// keeps the cross products, scaled by 2^11, in acc2.
 template <int WM, int WN, bool DUAL>
 __device__ __forceinline__ void bin_split_tile(const f16v (&acc)[WM][WN],
-                                               const f16v (&acc2)[DUAL ? WM : 1][DUAL ? WN : 1], double *sb,
-                                               int i0, int j0, int r0, int c0, int BM, int BN, int Ns, int L,
-                                               double *gbins)
+                                               const f16v (&acc2)[DUAL ? WM : 1][DUAL ? WN : 1],
+                                               unsigned long long *sb, int i0, int j0, int r0, int c0, int BM,
+                                               int BN, int Ns, int L, unsigned long long *gbins, double m0)
 {
     const int lane = threadIdx.x & 63;
     const BinGeom g = bin_geom(i0, j0, Ns, L, BM, BN);
-    for (int b = threadIdx.x; b < g.nbins; b += blockDim.x) sb[b] = 0.0;
+    for (int b = threadIdx.x; b < g.nbins; b += blockDim.x) sb[b] = 0;
     __syncthreads();
     if (i0 + BM <= Ns && j0 + BN <= Ns && L >= 32) {
         // full tile: with i = ri L + ci and j = rj L + cj the bin
@@ -377,7 +493,7 @@ __device__ __forceinline__ void bin_split_tile(const f16v (&acc)[WM][WN],
                 for (int n = 0; n < WN; n++) {
                     double val = (double)acc[m][n][v];
                     if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
-                    atomicAdd(&sb[im - jb[n] + (ci < cj[n] ? L : 0) + off], val);
+                    fix_add<1>(sb, g.nbins, im - jb[n] + (ci < cj[n] ? L : 0) + off, val, m0, 0.0);
                 }
             }
         }
@@ -404,12 +520,12 @@ __device__ __forceinline__ void bin_split_tile(const f16v (&acc)[WM][WN],
                     dc += dc < 0 ? L : 0;
                     double val = (double)acc[m][n][v];
                     if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
-                    atomicAdd(&sb[(ri - rj[n] - g.drmin) * L + dc], val);
+                    fix_add<1>(sb, g.nbins, (ri - rj[n] - g.drmin) * L + dc, val, m0, 0.0);
                 }
             }
     }
     __syncthreads();
-    flush_bins(sb, g, L, gbins);
+    flush_bins<1>(sb, g, L, gbins, 0);
 }
 
 template <int PREC, int WM, int WN, int WVM = 2, int WVN = 2>
@@ -554,9 +670,9 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
 #undef ASD_SPLIT_GLOAD
 #undef ASD_SPLIT_SSTORE
 
-    bin_split_tile<WM, WN, DUAL>(acc, acc2, reinterpret_cast<double *>(&sA[0][0][0][0]), i0, j0,
+    bin_split_tile<WM, WN, DUAL>(acc, acc2, reinterpret_cast<unsigned long long *>(&sA[0][0][0][0]), i0, j0,
                                  wi * WM * 32, wj * WN * 32, BM, BN, args.Ns, args.L,
-                                 args.bins + (long)w.lag * args.Ns);
+                                 args.bins + (long)w.lag * args.Ns, args.fix->m0);
     }
 }
 
@@ -766,8 +882,9 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
     // every load in flight lands before the bins reuse the LDS
     __builtin_amdgcn_s_waitcnt(wait_vmcnt(0));
     __syncthreads();
-    bin_split_tile<WM, WN, DUAL>(acc, acc2, reinterpret_cast<double *>(lds), i0, j0, wi * WM * 32,
-                                 wj * WN * 32, BM, BN, args.Ns, args.L, args.bins + (long)w.lag * args.Ns);
+    bin_split_tile<WM, WN, DUAL>(acc, acc2, reinterpret_cast<unsigned long long *>(lds), i0, j0, wi * WM * 32,
+                                 wj * WN * 32, BM, BN, args.Ns, args.L, args.bins + (long)w.lag * args.Ns,
+                                 args.fix->m0);
     }
 }
 
@@ -1116,11 +1233,27 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ASD_HIP(hipMemcpyAsync(ctx->qdev.ptr, h_q, sizeof(int) * 2 * nq, hipMemcpyHostToDevice, st));
         ASD_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
     }
-    // bins
+    // bins: fixed-point words [word][lag][d] (two words for FP64), then the decoded doubles
     const long nbins = (long)nlags * Ns;
-    ASD_TRY(ctx->bins.ensure(sizeof(double) * nbins));
+    const int nw = prec == ASD_FP64 ? 2 : 1;
+    ASD_TRY(ctx->bins.ensure(sizeof(double) * (nw + 1) * nbins));
+    ASD_TRY(ctx->fix.ensure(sizeof(FixScale)));
+    unsigned long long *words = ctx->bins.as<unsigned long long>();
+    double *gbins = ctx->bins.as<double>() + nw * nbins;
+    FixScale *fix = ctx->fix.as<FixScale>();
     ASD_TRY(mark(ctx, 0));
-    hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(256), 0, st, ctx->bins.as<double>(), nbins);
+    hipLaunchKernelGGL(zero_kernel, dim3(1024), dim3(256), 0, st, ctx->bins.as<double>(), nw * nbins);
+    hipLaunchKernelGGL(fix_reset_kernel, dim3(1), dim3(1), 0, st, fix);
+    {
+        const long n3 = 3L * Ns;
+        const unsigned g = (unsigned)std::min<long>(1024, (n3 + 255) / 256);
+        hipLaunchKernelGGL(vmax_kernel, dim3(g, (unsigned)std::min(nt_a, 16)), dim3(256), 0, st, d_snaps_a, tstride,
+                           nt_a, n3, &fix->vmax_bits);
+        if (d_snaps_b != d_snaps_a)
+            hipLaunchKernelGGL(vmax_kernel, dim3(g, (unsigned)std::min(T0, 16)), dim3(256), 0, st, d_snaps_b,
+                               tstride, T0, n3, &fix->vmax_bits);
+        hipLaunchKernelGGL(fix_scale_kernel, dim3(1), dim3(1), 0, st, fix, (int)K, Ns);
+    }
 
     GemmArgs ga{};
     ga.L = L;
@@ -1128,7 +1261,9 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
     ga.nt = nt_tiles;
     ga.nlags = nlags;
     ga.K = (int)K;
-    ga.bins = ctx->bins.as<double>();
+    ga.bins = words;
+    ga.wstride = nbins;
+    ga.fix = fix;
     ga.group = kKmajorCfgs[prec <= ASD_FP32 ? cfg : 0].group;
 
     if (prec == ASD_FP64 || prec == ASD_FP32) {
@@ -1190,6 +1325,8 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
     }
     ASD_HIP(hipGetLastError());
     ASD_TRY(mark(ctx, 2));
+    hipLaunchKernelGGL(fix_decode_kernel, dim3((unsigned)((nbins + 255) / 256)), dim3(256), 0, st,
+                       (const unsigned long long *)words, nbins, nw, Ns, (const FixScale *)fix, gbins);
 
     const bool full = nq > 2 * L && !ctx->sqt_generic_only;  // many momenta: separable DFTs over the grid
     if (full) {
@@ -1220,7 +1357,7 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         dft(fbar, 0, nb * L, 3, pt);                 // over col
         dft(pt, 1, nb, 3 * L, Fb);                   // over r
         ASD_TRY(mark(ctx, 3));
-        dft(ctx->bins.as<double>(), 0, (long)nlags * L, 1, Ht);  // bins over dc
+        dft(gbins, 0, (long)nlags * L, 1, Ht);  // bins over dc
         dft(Ht, 1, nlags, L, Sc);                                 // over dr
         const double *Fz = two ? Fb + 2 * wm : Fb;
         hipLaunchKernelGGL(finish_full_kernel, dim3((unsigned)(((long)nq * nlags + 255) / 256)), dim3(256), 0, st,
@@ -1245,7 +1382,7 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         Fz = Fb;
     }
     ASD_TRY(mark(ctx, 3));
-    hipLaunchKernelGGL(finish_kernel, dim3(nq, nlags), dim3(256), 0, st, ctx->bins.as<double>(), L,
+    hipLaunchKernelGGL(finish_kernel, dim3(nq, nlags), dim3(256), 0, st, (const double *)gbins, L,
                        ctx->qdev.as<int>(), ctx->twiddle.as<double2>(), (const double *)Fa, Fz, nq, T0, d_out);
     ASD_HIP(hipGetLastError());
     ASD_TRY(mark(ctx, 4));

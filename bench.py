@@ -8,16 +8,20 @@ lags 0..399 averaged over 400 time origins (GEMM K = 1200 per lag) on the
 Gamma-X-M-Gamma path (192 lattice momenta).
 
 One "step" of this benchmark is one pass of the whole hot path over that input:
-the 8000-step RK4 run (snapshot writes included) followed by S(q,tau) in the
-split precision.  `value` is the S(q,tau) throughput in algorithmic GEMM flops
-(sum over lags of 2 Ns^2 K = 2.577e14 per pass; split-precision extra products
-not counted) per second of S(q,tau) time; RK4 spin-steps/s, the FP64 path and
-the mixed-vs-FP64 end-to-end ratio with the error of the split result against
-FP64 are reported beside it.  Inputs are resident on the GPU before timing.
+the 8000-step RK4 run (snapshot writes included) followed by S(q,tau) in FP64, the
+reference's own arithmetic (its fused_dgemm is an FP64 DGEMM).  `value` is the
+S(q,tau) throughput in algorithmic GEMM flops (sum over lags of 2 Ns^2 K = 2.577e14
+per pass) per second of S(q,tau) time, and `roofline` is the FP64 correlation GEMM
+against the FP64 MFMA peak.  Beside it: RK4 spin-steps/s (and on an HBM-bound batch),
+and the mixed-precision leg -- the same pass with the GEMM in split FP16X2 on MFMA,
+its own roofline (3 MFMA products per useful flop), its max / RMS error against the
+FP64 result on the same snapshots and the end-to-end time ratio FP64 / mixed.  Inputs
+are resident on the GPU before timing.
 
 The CPU baseline is the reference's own kernel_fusion code (runge_kutta and
 fused_dgemm from oracle/_ref, built from its sources with its AVX2/FMA flags)
-timed on this host's cores on a bounded sample of the same workload.
+timed on this host's cores on a bounded sample of the same workload, with the
+reference's thread rule min(128, omp_get_max_threads()).
 """
 import argparse
 import json
@@ -39,6 +43,39 @@ HBM_PEAK_GBS = 8000.0
 
 def log(msg):
     print(msg, file=sys.stderr, flush=True)
+
+
+def host_cpu_facts():
+    """Host CPU model and counts (the CPU baseline's context)."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        affinity = None
+    return {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": affinity,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
+
+
+def reference_threads():
+    """The reference's rule, min(128, omp_get_max_threads()) (llg_sqt_fusion.c:221-224);
+    omp_get_max_threads() is OMP_NUM_THREADS when set, else the CPUs this process may use."""
+    env = os.environ.get("OMP_NUM_THREADS", "").split(",")[0].strip()
+    if env.isdigit() and int(env) > 0:
+        n = int(env)
+    else:
+        try:
+            n = len(os.sched_getaffinity(0))
+        except (AttributeError, OSError):
+            n = os.cpu_count() or 1
+    return min(128, n)
 
 
 def cpu_baseline(L, T0, nq, nlags, nsteps_total, threads):
@@ -278,6 +315,7 @@ def workload_rk4sweep(args):
     import asd_amd
     ctx = asd_amd.Context(0)
     ctx.set_timing(True)
+    ctx.rk4_set_exact(not args.rk4_fast)
     p = asd_amd.params(J=1.0, D=0.1, dt=0.01)
     rows = []
     for L, nrep, steps in ((128, 1, 2000), (256, 1, 1000), (1024, 1, 200), (1024, 8, 100), (1024, 32, 50),
@@ -309,6 +347,7 @@ def workload_rk4large(args):
     import asd_amd
     ctx = asd_amd.Context(0)
     ctx.set_timing(True)
+    ctx.rk4_set_exact(not args.rk4_fast)
     p = asd_amd.params(J=1.0, D=0.1, dt=0.01)
     L, nrep, steps = 1024, 64, 30
     S = torch.tensor(asd_amd.init_spins(5, L * L), device="cuda").repeat(nrep, 1)
@@ -318,6 +357,7 @@ def workload_rk4large(args):
     ms = ctx.stage_times()["rk4"]
     rate = nrep * L * L * steps / (ms * 1e-3)
     res = {"workload": "rk4 large", "L": L, "replicas": nrep, "steps": steps,
+           "rk4_arithmetic": "tolerance" if args.rk4_fast else "exact",
            "us_per_step": round(ms * 1e3 / steps, 2), "spin_steps_per_s": rate,
            "algorithmic_GBps": round(48.0 * rate / 1e9, 1), "hbm_frac": round(48.0 * rate / (HBM_PEAK_GBS * 1e9), 4)}
     if args.calib_copy:
@@ -372,17 +412,19 @@ def profiled_traffic(cfg_name, precision, L, T0, nlags):
     """HBM bytes per GEMM launch (FETCH_SIZE + WRITE_SIZE, scaled by the plain-copy calibration) from the
     committed rocprofv3 summary, when that summary was taken with this configuration on this workload;
     None otherwise (the counters need a profiler run of their own)."""
-    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
-                        "sqt_gemm_dma_fp16x2_L128_hbm_traffic.md")
-    try:
-        with open(path) as f:
-            kv = dict(re.findall(r"^- (\w+): (.+)$", f.read(), re.M))
-    except OSError:
-        return None
-    if (kv.get("config"), kv.get("precision"), kv.get("workload")) != (cfg_name, precision,
-                                                                       f"L={L} T0={T0} nlags={nlags}"):
-        return None
-    return int(kv["traffic_bytes_per_launch"])
+    import glob
+    pattern = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                           f"sqt_gemm_*_{precision}_L{L}_hbm_traffic.md")
+    for path in sorted(glob.glob(pattern)):
+        try:
+            with open(path) as f:
+                kv = dict(re.findall(r"^- (\w+): (.+)$", f.read(), re.M))
+        except OSError:
+            continue
+        if (kv.get("config"), kv.get("precision"), kv.get("workload")) == (cfg_name, precision,
+                                                                           f"L={L} T0={T0} nlags={nlags}"):
+            return int(kv["traffic_bytes_per_launch"])
+    return None
 
 
 def main():
@@ -390,13 +432,16 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp16x2", choices=["fp32", "bf16x3", "fp16x2"])
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "fp16x2"],
+                    help="GEMM precision of the timed (headline) pass; fp64 is the reference's arithmetic")
+    ap.add_argument("--mixed", default="fp16x2", choices=["fp32", "bf16x3", "fp16x2"],
+                    help="split precision of the mixed-precision leg")
     ap.add_argument("--L", type=int, default=128)
     ap.add_argument("--nsteps", type=int, default=8000)
     ap.add_argument("--every", type=int, default=10)
     ap.add_argument("--T0", type=int, default=400)
     ap.add_argument("--nlags", type=int, default=400)
-    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64 comparison pass")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the mixed-precision leg")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--workload", default="plain128",
@@ -404,6 +449,8 @@ def main():
                     help="plain128 is the benchmark line; the others report their own JSON")
     ap.add_argument("--no-rk4-large", action="store_true",
                     help="skip the RK4 measurement on the 1024 x 1024 x 64-replica batch")
+    ap.add_argument("--rk4-fast", action="store_true",
+                    help="rk4large / rk4sweep: the RK4 tolerance mode instead of the exact arithmetic")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 1 GiB-in / 1 GiB-out elementwise copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -438,7 +485,7 @@ def main():
     S0 = torch.tensor(asd_amd.init_spins(1, ns), device="cuda")
     spins = torch.empty_like(S0)
     snaps = torch.empty((nsnap, ns, 3), dtype=torch.float64, device="cuda")
-    out_split = torch.empty((nlags, nq, 2), dtype=torch.float64, device="cuda")
+    out_main = torch.empty((nlags, nq, 2), dtype=torch.float64, device="cuda")
 
     def one_pass(prec, out):
         spins.copy_(S0)
@@ -450,13 +497,14 @@ def main():
         return st
 
     log(f"bench: L={L} nsteps={args.nsteps} nsnap={nsnap} T0={T0} nlags={nlags} nq={nq} prec={args.precision}")
+    mixed = None if args.no_mixed or args.mixed == args.precision else args.mixed
     tuned = {}
-    for prec in ([args.precision] + ([] if args.no_fp64 else ["fp64"])):
+    for prec in [args.precision] + ([mixed] if mixed else []):
         best, times = ctx.autotune(L, T0, prec)
         tuned[prec] = {"chosen": list(times)[best], "gemm_ms_8_lags": {k: round(v, 3) for k, v in times.items()}}
         log(f"  autotune {prec}: {tuned[prec]}")
     for w in range(args.warmup):
-        one_pass(args.precision, out_split)
+        one_pass(args.precision, out_main)
         torch.cuda.synchronize()
         log(f"  warmup {w} done")
 
@@ -464,7 +512,7 @@ def main():
     t0 = time.perf_counter()
     stages = []
     for k in range(args.steps):
-        stages.append(one_pass(args.precision, out_split))
+        stages.append(one_pass(args.precision, out_main))
         log(f"  step {k}: {stages[-1]}")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
@@ -476,6 +524,7 @@ def main():
     value = gemm_flop / (sqt_ms * 1e-3) / 1e12
     rk_rate = args.nsteps * ns / (rk_ms * 1e-3)
     gemm_tflops = gemm_flop / (avg["gemm"] * 1e-3) / 1e12
+    split = args.precision in ("fp16x2", "bf16x3")
 
     res = {
         "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,
@@ -492,9 +541,9 @@ def main():
                      "traffic": profiled_traffic(tuned[args.precision]["chosen"], args.precision, L, T0, nlags),
                      "kernel": f"S(q,t) GEMM, {args.precision}, {tuned[args.precision]['chosen']} "
                                f"(all {nlags} lags in one launch)",
-                     "mfma_products_per_flop": 3 if args.precision in ("fp16x2", "bf16x3") else 1},
+                     "mfma_products_per_flop": 3 if split else 1},
         # the reference applies the phase in the GEMM epilogue: 4 flops per G element and q; here the
-        # epilogue bins G by displacement (one FP64 add per element) and the phases act on the L^2
+        # epilogue bins G by displacement (one fixed-point add per element) and the phases act on the L^2
         # real bins per lag (a multiply-add each for cos and sin: 4 flops per bin and q)
         "epilogue": {"reference_fused_flops": 4.0 * nq * float(ns) * ns * nlags,
                      "binning_adds": float(ns) * ns * nlags,
@@ -507,34 +556,43 @@ def main():
         "sqt_s": round(sqt_ms / 1e3, 4),
     }
 
-    if not args.no_fp64:
-        out64 = torch.empty_like(out_split)
-        one_pass("fp64", out64)                    # warm the FP64 path (workspace)
+    if mixed:
+        out_mix = torch.empty_like(out_main)
+        one_pass(mixed, out_mix)                   # warm the split path (workspace)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        st64 = one_pass("fp64", out64)
+        stm = one_pass(mixed, out_mix)
         torch.cuda.synchronize()
-        wall64 = time.perf_counter() - t0
-        sqt64_ms = st64["pack"] + st64["gemm"] + st64["fourier"] + st64["finish"]
-        a = out_split.cpu().numpy()
-        b = out64.cpu().numpy()
+        wall_m = time.perf_counter() - t0
+        sqtm_ms = stm["pack"] + stm["gemm"] + stm["fourier"] + stm["finish"]
+        gm_tf = gemm_flop / stm["gemm"] / 1e9
+        a = out_mix.cpu().numpy()
+        b = out_main.cpu().numpy()
         sa, sb = a[..., 0] + 1j * a[..., 1], b[..., 0] + 1j * b[..., 1]
         scale = np.abs(sb).max(axis=1, keepdims=True)
         e = np.abs(sa - sb) / scale
-        res["fp64"] = {"sqt_s": round(sqt64_ms / 1e3, 4), "gemm_tflops": round(gemm_flop / st64["gemm"] / 1e9, 3),
-                       "gemm_frac_of_fp64_peak": round(gemm_flop / st64["gemm"] / 1e9 / PEAK_TFLOPS["fp64"], 4),
-                       "end_to_end_s": round(wall64, 4)}
-        res["mixed_vs_fp64"] = round((st64["rk4"] + sqt64_ms) / (rk_ms + sqt_ms), 3)
-        res["err_vs_fp64"] = {"max": float(e.max()), "rms": float(np.sqrt((e ** 2).mean())),
-                              "floor": "relative to max_q |S(q,tau)| of each lag"}
-        log(f"  fp64 pass: {st64}")
+        res["mixed_precision"] = {
+            "precision": mixed, "sqt_s": round(sqtm_ms / 1e3, 4), "gemm_s": round(stm["gemm"] / 1e3, 4),
+            "sqt_tflops": round(gemm_flop / sqtm_ms / 1e9, 3), "end_to_end_s": round(wall_m, 4),
+            "roofline": {"bound": "mfma", "achieved": round(gm_tf, 3), "peak": PEAK_TFLOPS[mixed], "unit": "TFLOP/s",
+                         "frac": round(gm_tf / PEAK_TFLOPS[mixed], 4), "mfma_products_per_flop": 3,
+                         "mfma_work_frac": round(3 * gm_tf / PEAK_TFLOPS[mixed], 4),
+                         "traffic": profiled_traffic(tuned[mixed]["chosen"], mixed, L, T0, nlags),
+                         "kernel": f"S(q,t) GEMM, {mixed}, {tuned[mixed]['chosen']}"},
+            "err_vs_fp64": {"max": float(e.max()), "rms": float(np.sqrt((e ** 2).mean())),
+                            "floor": "relative to max_q |S(q,tau)| of each lag, same snapshots"}}
+        if args.precision == "fp64":
+            res["mixed_vs_fp64"] = round((rk_ms + sqt_ms) / (stm["rk4"] + sqtm_ms), 3)
+        log(f"  mixed pass: {stm}")
 
     # calibration: the vendor library's plain fp16 GEMM (torch.matmul -> hipBLASLt) at one lag's
     # shape with K padded to a multiple of 64 (the library's faster shape: 16384 x 16384 x 1200
     # runs ~15% below x 1216), fp16 output, no epilogue, against the split kernel's MFMA work
     # rate (three products per useful flop, over the K its MFMAs cover: K rounded up to the
     # MFMA depth of 16)
-    if args.precision in ("fp16x2", "bf16x3"):
+    split_prec = args.precision if split else mixed
+    if split_prec in ("fp16x2", "bf16x3"):
+        split_tf = gemm_tflops if split else gemm_flop / stm["gemm"] / 1e9
         kpad = (3 * T0 + 63) // 64 * 64
         kexec = (3 * T0 + 15) // 16 * 16
         xa = torch.randn(ns, kpad, device="cuda", dtype=torch.float16)
@@ -548,7 +606,7 @@ def main():
             xc = xa @ xb
         torch.cuda.synchronize()
         lib_tf = 2.0 * ns * ns * kpad * nmm / (time.perf_counter() - t_mm) / 1e12
-        ours = 3.0 * gemm_tflops * kexec / (3 * T0)
+        ours = 3.0 * split_tf * kexec / (3 * T0)
         res["library_calibration"] = {"hipblaslt_fp16_gemm_tflops": round(lib_tf, 1), "shape": f"{ns}x{ns}x{kpad}",
                                       "split_kernel_mfma_tflops": round(ours, 1),
                                       "ratio": round(ours / lib_tf, 3)}
@@ -558,24 +616,33 @@ def main():
     # the same RK4 step on a lattice batch far larger than the caches (L = 1024, 64
     # replicas: 1.6 GB of state), the HBM-bound regime; the automatic kernel choice
     # takes the register-streaming strips here
+    # (the tolerance mode -- fused multiply-adds, within 1e-10 of the exact path over 100 steps
+    # -- as the row's figures, the exact mode beside it)
     if not args.no_rk4_large:
         Lb, nb, steps_b = 1024, 64, 30
-        Sb = torch.tensor(asd_amd.init_spins(5, Lb * Lb), device="cuda").repeat(nb, 1)
-        ctx.rk4_run(prm, Lb, Sb, steps_b)   # untimed: the clocks settle (the first run is ~10% slower)
-        torch.cuda.synchronize()
-        ctx.rk4_run(prm, Lb, Sb, steps_b)
-        ms_b = ctx.stage_times()["rk4"]
-        rate_b = nb * Lb * Lb * steps_b / (ms_b * 1e-3)
-        res["rk4_bandwidth_bound"] = {
-            "L": Lb, "replicas": nb, "sites": nb * Lb * Lb, "steps": steps_b,
-            "us_per_step": round(ms_b * 1e3 / steps_b, 2), "spin_steps_per_s": rate_b,
-            "algorithmic_GBps": round(48.0 * rate_b / 1e9, 1),
-            "hbm_frac": round(48.0 * rate_b / (HBM_PEAK_GBS * 1e9), 4)}
-        del Sb
+        Sb0 = torch.tensor(asd_amd.init_spins(5, Lb * Lb), device="cuda").repeat(nb, 1)
+        row = {}
+        for mode in ("exact", "tolerance"):
+            ctx.rk4_set_exact(mode == "exact")
+            Sb = Sb0.clone()
+            ctx.rk4_run(prm, Lb, Sb, steps_b)   # untimed: the clocks settle (the first run is ~10% slower)
+            torch.cuda.synchronize()
+            ctx.rk4_run(prm, Lb, Sb, steps_b)
+            ms_b = ctx.stage_times()["rk4"]
+            rate_b = nb * Lb * Lb * steps_b / (ms_b * 1e-3)
+            row[mode] = {"us_per_step": round(ms_b * 1e3 / steps_b, 2), "spin_steps_per_s": rate_b,
+                         "algorithmic_GBps": round(48.0 * rate_b / 1e9, 1),
+                         "hbm_frac": round(48.0 * rate_b / (HBM_PEAK_GBS * 1e9), 4)}
+            del Sb
+        ctx.rk4_set_exact(True)
+        res["rk4_bandwidth_bound"] = {"L": Lb, "replicas": nb, "sites": nb * Lb * Lb, "steps": steps_b,
+                                      "rk4_arithmetic": "tolerance mode (asd_rk4_set_exact(ctx, 0))",
+                                      **row["tolerance"], "exact_mode": row["exact"]}
+        del Sb0
         log(f"  rk4 bandwidth-bound: {res['rk4_bandwidth_bound']}")
 
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+        threads = args.cpu_threads or reference_threads()
         try:
             cb = cpu_baseline(L, T0, nq, nlags, args.nsteps, threads)
         except Exception as ex:  # the baseline is reported, never required
@@ -584,6 +651,8 @@ def main():
         if cb:
             res["cpu_baseline"] = {
                 "value": round(cb["sqt_tflops"], 5), "unit": "TFLOP/s", "cores": threads, "kind": "reference",
+                "threads_rule": "min(128, omp_get_max_threads()) as llg_sqt_fusion.c:221-224",
+                "host": host_cpu_facts(),
                 "sample": f"reference kernel_fusion fused_dgemm at L={L}, K={3 * T0} (one q, one lag, with the "
                           f"disconnected prefill; 1 call) and 200 reference runge_kutta steps at L={L}",
                 "rk4_spin_steps_per_s": cb["rk4_spin_steps_per_s"],

@@ -97,7 +97,11 @@ int asd_memcpy(asd_ctx *ctx, void *dst, const void *src, size_t bytes, int kind)
  * snapshot writer fused in.  d_spins: device [nrep][L*L][3], in/out.
  * d_snaps: device [nsnap][nrep][L*L][3] or NULL; snapshot k = state after
  * k*snap_every steps (k = 0 is the input state).  The step loop is replayed from
- * a captured hipGraph when use_graph != 0. */
+ * a captured hipGraph when use_graph != 0.  Returns after the work has completed:
+ * ASD_ENOTFINITE (message naming the replica and the first step) when a step produced
+ * an unnormalised state c with |c|^2 NaN, infinite or outside [2^-900, 2^900], the range
+ * where the step is defined (the reference's own NaN initial state,
+ * llg_sqt_fusion.c:204-214, is caught at step 0); the states are then unusable. */
 int asd_rk4_run(asd_ctx *ctx, const asd_params *p, int L, int nrep, double *d_spins,
                 long nsteps, int snap_every, int nsnap, double *d_snaps, int use_graph);
 
@@ -108,6 +112,16 @@ int asd_rk4_run(asd_ctx *ctx, const asd_params *p, int L, int nrep, double *d_sp
  * columns of a band of rows; lattices with L < 64 use 16 x 16 tiles).  Results do not
  * depend on the choice (bit for bit). */
 int asd_rk4_set_tile(asd_ctx *ctx, int tile);
+
+/* RK4 arithmetic for every RK4 entry point of this context.  exact = 1 (default): the
+ * reference's operation order (llg_sqt_fusion.c:30-145, :350-361) with correctly rounded
+ * quotients, so every value equals the FP64 CPU restatement's (IEEE ==).  exact = 0, the
+ * tolerance mode: fused multiply-adds in the field and cross products, the stage inputs
+ * S + f h and the partial sums S + f dt/6 + f dt/3 + ... as single FMAs, and the
+ * renormalisation by a Newton-refined reciprocal square root (|S| = spin_length to ~1e-16);
+ * fewer instructions per site (the HBM-bound sweep is issue-bound) at a trajectory
+ * difference of ~1e-13 after 100 steps.  No counterpart in the reference. */
+int asd_rk4_set_exact(asd_ctx *ctx, int exact);
 
 /* ---------------------------------------- stage 2: S(q,tau) correlation --- */
 /* Native replacement for the reference's S(q,t) stage (llg_sqt_fusion.c:370-442
@@ -163,10 +177,31 @@ int asd_stage_times(asd_ctx *ctx, double t[5]);
 /* Independent trajectories of uneven L in ONE persistent launch per step
  * (work list of (trajectory, tile) items), step loop captured as a hipGraph.
  * Ls[ntraj], params[ntraj], d_spins[ntraj] (device [L*L][3] each),
- * d_snaps[ntraj] (device [nsnap][L*L][3] each, or NULL array). */
+ * d_snaps[ntraj] (device [nsnap][L*L][3] each, or NULL array).  The kernel variant (tile
+ * shape, threads per tile, one or two fused steps per launch) is the one forced with
+ * asd_batch_set_variant, else the one the autotuner recorded for this batch composition
+ * (the multiset of L), else asd_autotune_batch runs first (a few ms on scratch states).
+ * Every variant gives the same bits.  Returns after the work has completed, with the
+ * numeric guard of asd_rk4_run (ASD_ENOTFINITE names the trajectory and the first step).
+ * The counterpart of the reference's fixed blocking (gemm_kernel.h:293-328) is the tuned
+ * variant. */
 int asd_batch_rk4_run(asd_ctx *ctx, int ntraj, const int *Ls, const asd_params *params,
                       double *const *d_spins, long nsteps, int snap_every, int nsnap,
                       double *const *d_snaps);
+
+/* Times every batch kernel variant for this composition (20 steps on scratch states of the
+ * same sizes and parameters, best of two runs after a warm-up) and records the fastest
+ * for the multiset Ls; best (may be NULL) receives its index, times_ms (may be NULL,
+ * asd_batch_num_variants() entries) the times. */
+int asd_autotune_batch(asd_ctx *ctx, int ntraj, const int *Ls, const asd_params *params, int *best,
+                       double *times_ms);
+int asd_batch_num_variants(void);
+const char *asd_batch_variant_name(int variant);
+/* force a variant for asd_batch_rk4_run (-1 = automatic) */
+int asd_batch_set_variant(asd_ctx *ctx, int variant);
+/* the variant asd_batch_rk4_run would take for this composition now: the forced one, the
+ * recorded one, or -1 (none recorded: the next run autotunes first) */
+int asd_batch_variant_for(asd_ctx *ctx, int ntraj, const int *Ls);
 
 /* -------------------------------------------------------- autotuning --- */
 /* Times every correlation-GEMM tile configuration for (L, K = 3 T0, prec) on

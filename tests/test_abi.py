@@ -84,3 +84,31 @@ def test_fused_dgemm_wrapper_rejects_bad_arrays():
             call(**kw)
     with pytest.raises(ValueError):   # ldc below 2 ceil4(m)
         asd_amd.fused_dgemm(m, n, k, 1.0, ok["a"], m, ok["b"], k, ok["c"], ldc - 2, ok["fuse_c"], ok["sums"])
+
+
+def test_context_wrappers_reject_bad_tensors_before_any_call():
+    """rk4_run / sqt hand device pointers to C, so the wrappers check
+    that every buffer is a contiguous float64 GPU tensor of the right size first (explicit
+    ValueError, not assert: python -O keeps them).  A CPU or mis-sized tensor raises
+    before the library is touched, so this runs without a GPU (no context is created)."""
+    import torch
+
+    import asd_amd
+    ctx = object.__new__(asd_amd.Context)   # no device: the checks run before any C call
+    ctx.h = None
+    L = 4
+    p = asd_amd.params()
+    cpu = torch.zeros(L * L, 3, dtype=torch.float64)
+    meta = torch.empty(L * L, 3, dtype=torch.float64, device="meta")
+    for fn in (ctx.rk4_run,):
+        with pytest.raises(ValueError):
+            fn(p, L, cpu, 10)                                   # spins on the host
+        with pytest.raises(ValueError):
+            fn(p, L, cpu.float(), 10)                           # wrong dtype (checked first: on the host)
+        with pytest.raises(ValueError):
+            fn(p, L, meta, 10, 5, 3, cpu.repeat(3, 1))          # snaps: not a GPU tensor
+    with pytest.raises(ValueError):
+        ctx.sqt(L, cpu.reshape(1, L * L, 3), 1, 1, [(0, 0)])    # snapshots on the host
+    with pytest.raises(ValueError):
+        ctx.sqt_unfused(L, cpu.reshape(1, L * L, 3), 1, 1, [(0, 0)])
+    ctx.h = None

@@ -79,3 +79,118 @@ def test_batch_ragged_sizes_bit_exact(gctx, orc):
         ref, ref_snaps = orc.run(orc.params(dt=0.01, **kw), L, S0[t], nsteps, every, nsnap)
         assert np.array_equal(spins[t].cpu().numpy(), ref), (t, L, kw)
         assert np.array_equal(snaps[t].cpu().numpy(), ref_snaps), (t, L, kw)
+
+
+def _separable(snaps, L, T0, nlags, q):
+    """FP64 separable form of S(q,tau) with numpy's FFT (qx with the row)."""
+    nsnap = snaps.shape[0]
+    F = np.fft.fft2(snaps.reshape(nsnap, L, L, 3), axes=(1, 2))
+    qa = np.asarray(q)
+    Fq = F[:, qa[:, 0], qa[:, 1], :]
+    out = np.empty((nlags, len(q)), complex)
+    for lag in range(nlags):
+        a, b = Fq[lag:lag + T0], Fq[:T0]
+        corr = np.einsum("tqc,tqc->q", a, b.conj()) / T0
+        out[lag] = (corr - (a.mean(axis=0) * b.mean(axis=0).conj()).sum(axis=-1)) / (L * L)
+    return out
+
+
+@pytest.mark.gpu
+def test_batching_workload_full_size(gctx):
+    """The batching workload at its own size: 96 trajectories, L in {32, 48, 64, 96, 128}
+    (32 / 24 / 20 / 12 / 8), D ~ U[0, 0.2], alpha = 0, seeds 100..195, 2000 FP64 RK4 steps with
+    a snapshot every 10 in one persistent launch per step.  Every trajectory and snapshot
+    equals its unbatched run bit for bit; then S(q,tau) for lags 0..99 over 100 origins on
+    each lattice's Gamma-X-M-Gamma path in FP16X2 (autotuned per L) is within 1e-5 of the
+    separable FP64 identity for all 96."""
+    torch = need_gpu()
+    import asd_amd
+    Ls = [32] * 32 + [48] * 24 + [64] * 20 + [96] * 12 + [128] * 8
+    Ds = np.random.default_rng(2024).uniform(0.0, 0.2, len(Ls))
+    nsteps, every, T0, nl = 2000, 10, 100, 100
+    nsnap = nsteps // every
+    plist = [asd_amd.params(J=1.0, D=float(d), dt=0.01) for d in Ds]
+    S0 = [asd_amd.init_spins(100 + t, L * L) for t, L in enumerate(Ls)]
+    spins = [torch.tensor(s, device="cuda") for s in S0]
+    snaps = [torch.empty((nsnap, L * L, 3), dtype=torch.float64, device="cuda") for L in Ls]
+    say(f"batch workload: {len(Ls)} trajectories, {nsteps} steps")
+    gctx.batch_rk4_run(Ls, plist, spins, nsteps, every, nsnap, snaps)
+    gctx.synchronize()
+    for t, L in enumerate(Ls):
+        s1 = torch.tensor(S0[t], device="cuda")
+        sn1 = torch.empty((nsnap, L * L, 3), dtype=torch.float64, device="cuda")
+        gctx.rk4_run(plist[t], L, s1, nsteps, every, nsnap, sn1)
+        assert torch.equal(spins[t], s1), (t, L)
+        assert torch.equal(snaps[t], sn1), (t, L)
+    say("  batched trajectories equal their single runs")
+    for L in sorted(set(Ls)):
+        gctx.autotune(L, T0, "fp16x2")
+    worst = 0.0
+    for t, L in enumerate(Ls):
+        q = [(k, 0) for k in range(L // 2)] + [(L // 2, k) for k in range(L // 2)] + \
+            [(L // 2 - k, L // 2 - k) for k in range(L // 2)]
+        got = gctx.sqt(L, snaps[t], T0, nl, q, "fp16x2")
+        ref = _separable(snaps[t].cpu().numpy(), L, T0, nl, q)
+        e = float((np.abs(got - ref) / np.abs(ref).max(axis=1, keepdims=True)).max())
+        worst = max(worst, e)
+        assert e <= 1e-5, (t, L, e)
+    say(f"  fp16x2 S(q,tau) of all {len(Ls)} trajectories: worst max error {worst:.3e}")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("nsteps,every", [(45, 10), (40, 5), (31, 4)])
+def test_every_batch_variant_bit_exact(gctx, orc, nsteps, every):
+    """Each batch kernel variant forced in turn (tile edge 8 / 16 / 32, one or two fused steps
+    per launch): every trajectory and snapshot equals the oracle bit for bit, with an odd
+    trailing step (45), snapshot spacing that forbids fusion (every 5), and a short graph
+    chunk (every 4)."""
+    torch = need_gpu()
+    import asd_amd
+    Ls = [9, 17, 33, 47, 64, 20]
+    kws = [dict(D=0.1), dict(J=0.7, D=0.05, Bz=0.1, alpha=0.05), dict(D=0.2, Bz=0.02),
+           dict(J=1.2, alpha=0.2), dict(D=0.1), dict(J=-0.5, D=0.1, alpha=0.01)]
+    nsnap = nsteps // every + 1
+    S0 = [orc.init_spins(90 + t, L * L) for t, L in enumerate(Ls)]
+    refs = [orc.run(orc.params(dt=0.01, **kw), L, S0[t], nsteps, every, nsnap) for t, (L, kw) in
+            enumerate(zip(Ls, kws))]
+    nv = asd_amd.lib().asd_batch_num_variants()
+    try:
+        for v in range(nv):
+            gctx.batch_set_variant(v)
+            spins = [torch.tensor(s, device="cuda") for s in S0]
+            snaps = [torch.zeros((nsnap, L * L, 3), dtype=torch.float64, device="cuda") for L in Ls]
+            gctx.batch_rk4_run(Ls, [asd_amd.params(dt=0.01, **kw) for kw in kws], spins, nsteps, every, nsnap,
+                               snaps)
+            for t in range(len(Ls)):
+                assert np.array_equal(spins[t].cpu().numpy(), refs[t][0]), (v, t)
+                assert np.array_equal(snaps[t].cpu().numpy(), refs[t][1]), (v, t)
+    finally:
+        gctx.batch_set_variant(-1)
+
+
+@pytest.mark.gpu
+def test_batch_autotuner_records_and_uses_its_choice(gctx, orc):
+    torch = need_gpu()
+    import asd_amd
+    Ls = [32] * 6 + [48] * 4 + [96] * 2
+    plist = [asd_amd.params(D=0.05 + 0.01 * t, dt=0.01) for t in range(len(Ls))]
+    assert gctx.batch_variant_for(Ls) == -1
+    best, times = gctx.autotune_batch(Ls, plist)
+    say(f"batch autotune: best {best} ({list(times)[best]}), ms {[round(t, 4) for t in times.values()]}")
+    assert all(np.isfinite(t) and t > 0 for t in times.values())
+    assert list(times.values())[best] == min(times.values())
+    assert gctx.batch_variant_for(Ls) == best
+    assert gctx.batch_variant_for(list(reversed(Ls))) == best      # keyed on the multiset of L
+    assert gctx.batch_variant_for([32, 48]) == -1
+    # a run with the recorded choice equals the oracle bit for bit
+    S0 = [orc.init_spins(300 + t, L * L) for t, L in enumerate(Ls)]
+    spins = [torch.tensor(s, device="cuda") for s in S0]
+    gctx.batch_rk4_run(Ls, plist, spins, 60)
+    for t, L in enumerate(Ls):
+        ref, _ = orc.run(orc.params(D=0.05 + 0.01 * t, dt=0.01), L, S0[t], 60)
+        assert np.array_equal(spins[t].cpu().numpy(), ref)
+    # an untuned composition is tuned by its first run and recorded
+    Ls2 = [40, 24]
+    gctx.batch_rk4_run(Ls2, [asd_amd.params(dt=0.01)] * 2,
+                       [torch.tensor(orc.init_spins(5, L * L), device="cuda") for L in Ls2], 4)
+    assert gctx.batch_variant_for(Ls2) >= 0

@@ -64,6 +64,8 @@ _SIGS = {
     "asd_synchronize": (C.c_int, [_vp]),
     "asd_rk4_run": (C.c_int, [_vp, C.POINTER(Params), C.c_int, C.c_int, _vp, C.c_long, C.c_int, C.c_int, _vp,
                               C.c_int]),
+    "asd_rk4_run_noalloc": (C.c_int, [_vp, C.POINTER(Params), C.c_int, C.c_int, _vp, C.c_long, C.c_int, C.c_int,
+                                      _vp]),
     "asd_sqt": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, C.c_int, _vp, C.c_int]),
     "asd_sqt_replicas": (C.c_int, [_vp, C.c_int, C.c_int, _vp, _vp, _vp, C.c_int, C.c_int, _vp]),
     "asd_sqt_unfused": (C.c_int, [_vp, C.c_int, _vp, C.c_int, C.c_int, C.c_int, _vp, C.c_int, _vp, C.c_int]),
@@ -223,6 +225,14 @@ class Context:
         nrep, sp = self._rk4_args(L, spins, snap_every, nsnap, snaps)
         _check(lib().asd_rk4_run(self.h, C.byref(p), L, nrep, C.c_void_p(spins.data_ptr()), nsteps, snap_every,
                                  nsnap, sp, 1 if use_graph else 0))
+
+    def rk4_run_noalloc(self, p: Params, L: int, spins, nsteps: int, snap_every: int = 0, nsnap: int = 0,
+                        snaps=None):
+        """asd_rk4_run_noalloc: as rk4_run(use_graph=False) with no device allocation (fixed 16 MiB
+        scratch: nrep * L * L * 3 <= 2^21); only enqueues the work (no numeric guard read-back)."""
+        nrep, sp = self._rk4_args(L, spins, snap_every, nsnap, snaps)
+        _check(lib().asd_rk4_run_noalloc(self.h, C.byref(p), L, nrep, C.c_void_p(spins.data_ptr()), nsteps,
+                                         snap_every, nsnap, sp))
 
     def batch_rk4_run(self, Ls, plist, spins_list, nsteps: int, snap_every: int = 0, nsnap: int = 0,
                       snaps_list=None):

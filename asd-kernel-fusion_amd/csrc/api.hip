@@ -5,6 +5,7 @@
 #include <cstdlib>
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 
 #include "asd_compat.h"
 #include "asd_internal.h"
@@ -78,6 +79,33 @@ __global__ void set_int_kernel(int *p, int v) { *p = v; }
 
 // longest step loop captured as one graph (longer snapshot intervals run eagerly)
 constexpr long kMaxGraphSteps = 4096;
+
+// ---------------------------------------------- scratch of asd_rk4_run_noalloc
+// The allocation-free RK4 entry point keeps its second state buffer (the ping-pong
+// partner of the caller's spins) in this file-scope array: 16 MiB per GPU, held for the
+// life of the process.  Host code never uses the array's name as an address: asd_create
+// looks up each device's copy once with hipGetSymbolAddress (that device current) and
+// the entry point passes that pointer to the kernels.  Calls that overlap on one GPU share
+// the array and race.
+constexpr long kScratchDoubles = 1L << 21;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxDevices = 64;
+static double *g_scratch_ptr[kMaxDevices];
+static std::mutex g_scratch_mu;
+
+static int scratch_lookup(int device)
+{
+    ASD_CHECK(device >= 0 && device < kMaxDevices, "device %d beyond the %d this build tracks", device, kMaxDevices);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return ASD_OK;
+    void *ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr)
+        return set_error(ASD_EHIP, "hipGetSymbolAddress(scratch) on device %d failed: %s", device,
+                         hipGetErrorString(e));
+    g_scratch_ptr[device] = static_cast<double *>(ptr);
+    return ASD_OK;
+}
 
 // the guard word's verdict: ASD_OK, or ASD_ENOTFINITE naming the first step
 static int guard_verdict(unsigned long long h, bool batch)
@@ -168,6 +196,7 @@ int asd_create(asd_ctx **out, int device)
     ASD_HIP(hipSetDevice(device));
     hipDeviceProp_t prop;
     ASD_HIP(hipGetDeviceProperties(&prop, device));
+    ASD_TRY(scratch_lookup(device));
     asd_ctx *c = new asd_ctx();
     c->device = device;
     c->num_cu = prop.multiProcessorCount;
@@ -290,7 +319,7 @@ int asd_synchronize(asd_ctx *c)
 
 // --------------------------------------------------------------------- RK4
 
-// The RK4 step loop of asd_rk4_run: `other` is the second
+// The RK4 step loop shared by asd_rk4_run and asd_rk4_run_noalloc: `other` is the second
 // state buffer, `counter` the snapshot counter (graph mode only), `first` the guard word
 // (NULL: no check).  Everything is stream-ordered; nothing is allocated here.
 static int rk4_drive(asd_ctx *c, const LLGParams &p, int L, int nrep, double *d_spins, double *other,
@@ -414,6 +443,21 @@ int asd_rk4_run(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spi
     ASD_HIP(hipMemcpyAsync(&h, first, sizeof h, hipMemcpyDeviceToHost, st));
     ASD_HIP(hipStreamSynchronize(st));
     return guard_verdict(h, false);
+}
+
+int asd_rk4_run_noalloc(asd_ctx *c, const asd_params *pp, int L, int nrep, double *d_spins, long nsteps,
+                        int snap_every, int nsnap, double *d_snaps)
+{
+    ASD_TRY(check_rk4_args(c, pp, L, nrep, d_spins, nsteps, snap_every, nsnap, d_snaps));
+    const long n = (long)nrep * L * L * 3;
+    ASD_CHECK(n <= kScratchDoubles,
+              "asd_rk4_run_noalloc: nrep * L * L * 3 = %ld doubles exceeds its fixed scratch of %ld doubles "
+              "(16 MiB); use asd_rk4_run",
+              n, kScratchDoubles);
+    ASD_HIP(hipSetDevice(c->device));
+    ASD_CHECK(c->device < kMaxDevices && g_scratch_ptr[c->device], "scratch of device %d not set up", c->device);
+    return rk4_drive(c, make_llg(*pp), L, nrep, d_spins, g_scratch_ptr[c->device], nullptr, nullptr, nsteps,
+                     snap_every, nsnap, d_snaps, false);
 }
 
 // ---------------------------------------------------------------- S(q,tau)

@@ -105,6 +105,17 @@ int asd_memcpy(asd_ctx *ctx, void *dst, const void *src, size_t bytes, int kind)
 int asd_rk4_run(asd_ctx *ctx, const asd_params *p, int L, int nrep, double *d_spins,
                 long nsteps, int snap_every, int nsnap, double *d_snaps, int use_graph);
 
+/* asd_rk4_run without any allocation, for callers that cannot allocate device memory
+ * (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc).  No
+ * counterpart in the reference.  Same arguments and results (bit for bit) as
+ * asd_rk4_run with use_graph = 0; the second state buffer is a fixed 16 MiB scratch array
+ * in the library (one per GPU, kept until the process ends), so nrep * L * L * 3 must be
+ * at most 2^21 doubles (ASD_EINVAL otherwise).  It only enqueues work on the context's
+ * stream (no hipMalloc, no synchronisation, no numeric guard read-back) and returns
+ * before the work completes.  Calls that overlap on one GPU share the scratch and race. */
+int asd_rk4_run_noalloc(asd_ctx *ctx, const asd_params *p, int L, int nrep, double *d_spins,
+                        long nsteps, int snap_every, int nsnap, double *d_snaps);
+
 /* RK4 kernel for asd_rk4_run: 0 = automatic (LDS tiles sized by occupancy: 8 x 8
  * while 16 x 16 tiles would leave CUs idle, 16 x 16 otherwise; the streaming strips
  * once nrep * ceil(L / 120) * ceil(L / 56) >= 4 waves per CU), 8, 16 or 32 = LDS

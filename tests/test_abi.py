@@ -87,7 +87,7 @@ def test_fused_dgemm_wrapper_rejects_bad_arrays():
 
 
 def test_context_wrappers_reject_bad_tensors_before_any_call():
-    """rk4_run / sqt hand device pointers to C, so the wrappers check
+    """rk4_run / rk4_run_noalloc / sqt hand device pointers to C, so the wrappers check
     that every buffer is a contiguous float64 GPU tensor of the right size first (explicit
     ValueError, not assert: python -O keeps them).  A CPU or mis-sized tensor raises
     before the library is touched, so this runs without a GPU (no context is created)."""
@@ -100,7 +100,7 @@ def test_context_wrappers_reject_bad_tensors_before_any_call():
     p = asd_amd.params()
     cpu = torch.zeros(L * L, 3, dtype=torch.float64)
     meta = torch.empty(L * L, 3, dtype=torch.float64, device="meta")
-    for fn in (ctx.rk4_run,):
+    for fn in (ctx.rk4_run, ctx.rk4_run_noalloc):
         with pytest.raises(ValueError):
             fn(p, L, cpu, 10)                                   # spins on the host
         with pytest.raises(ValueError):

@@ -116,7 +116,7 @@ def test_wrappers_reject_host_and_missized_device_buffers(gctx):
     bad_snaps = [torch.zeros((3, L * L, 3), dtype=torch.float64),                  # on the host
                  torch.zeros((2, L * L, 3), dtype=torch.float64, device="cuda"),   # too small for nsnap = 3
                  torch.zeros((3, L * L, 3), dtype=torch.float32, device="cuda")]   # wrong dtype
-    for fn in (gctx.rk4_run,):
+    for fn in (gctx.rk4_run, gctx.rk4_run_noalloc):
         for sn in bad_snaps:
             with pytest.raises(ValueError):
                 fn(p, L, spins, 20, 10, 3, sn)

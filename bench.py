@@ -203,6 +203,8 @@ def workload_batch96(args):
     spins = [s.clone() for s in S0]
     snaps = [torch.empty((nsnap, L * L, 3), dtype=torch.float64, device="cuda") for L in Ls]
     total_sites = sum(L * L for L in Ls)
+    vbest, vtimes = ctx.autotune_batch(Ls, plist)
+    log(f"  batch variants (ms per 20 steps): {vtimes}")
     ctx.batch_rk4_run(Ls, plist, spins, nsteps, every, nsnap, snaps)     # warm (graph capture)
     for t in range(len(Ls)):
         spins[t].copy_(S0[t])
@@ -234,6 +236,9 @@ def workload_batch96(args):
                       "rk4_steps": nsteps, "rk4_batched_s": round(t_batch, 4),
                       "rk4_spin_steps_per_s": total_sites * nsteps / t_batch,
                       "rk4_unbatched_s": round(t_single, 4), "batching_speedup": round(t_single / t_batch, 2),
+                      "rk4_hbm_frac": round(48.0 * total_sites * nsteps / t_batch / (HBM_PEAK_GBS * 1e9), 4),
+                      "batch_variant": list(vtimes)[vbest],
+                      "batch_variant_ms_per_20_steps": {k: round(v, 4) for k, v in vtimes.items()},
                       "sqt_fp16x2_s": round(t_sqt, 3), "sqt_tflops": round(flop / t_sqt / 1e12, 2),
                       "autotuned_cfg_per_L": tuned}), flush=True)
 

@@ -695,7 +695,7 @@ int asd_autotune_batch(asd_ctx *c, int ntraj, const int *Ls, const asd_params *p
             if (rep > 0) ms = std::min(ms, (double)t);  // the first run warms up
         }
         if (times_ms) times_ms[v] = ms;
-        if (ms < best_ms) {
+        if (ms < 0.99 * best_ms) {  // near-ties keep the earlier variant
             best_ms = ms;
             bi = v;
         }

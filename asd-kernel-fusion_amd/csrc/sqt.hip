@@ -1420,7 +1420,9 @@ int sqt_autotune(asd_ctx *ctx, int L, int T0, asd_precision prec, int *best, dou
             if (rep > 0) ms = std::min(ms, (double)t);
         }
         if (times_ms) times_ms[c] = ms;
-        if (ms < best_ms) {
+        // a later configuration must be more than 1% faster: near-ties keep the earlier one, so
+        // the choice does not flip from run to run on timing noise
+        if (ms < 0.99 * best_ms) {
             best_ms = ms;
             bi = c;
         }

@@ -33,7 +33,12 @@ def test_autotuned_choice_is_recorded_and_used(gctx, orc, prec, bound):
     say(f"autotune {prec} L={L} T0={T0}: best {best} ({names[best]}), ms {[round(t, 3) for t in times.values()]}")
     assert 0 <= best < len(names)
     assert all(np.isfinite(t) and t > 0 for t in times.values())
-    assert times[names[best]] == min(times.values())
+    vals = list(times.values())
+    expect = 0                       # a later choice must be more than 1% faster than the best so far
+    for i, t in enumerate(vals):
+        if t < 0.99 * vals[expect]:
+            expect = i
+    assert best == expect and vals[best] <= 1.0102 * min(vals)
     auto = gctx.sqt(L, snaps, T0, nl, q, prec)
     try:
         gctx.set_config(prec, best)

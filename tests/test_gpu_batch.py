@@ -178,7 +178,12 @@ def test_batch_autotuner_records_and_uses_its_choice(gctx, orc):
     best, times = gctx.autotune_batch(Ls, plist)
     say(f"batch autotune: best {best} ({list(times)[best]}), ms {[round(t, 4) for t in times.values()]}")
     assert all(np.isfinite(t) and t > 0 for t in times.values())
-    assert list(times.values())[best] == min(times.values())
+    vals = list(times.values())
+    expect = 0                       # a later choice must be more than 1% faster than the best so far
+    for i, t in enumerate(vals):
+        if t < 0.99 * vals[expect]:
+            expect = i
+    assert best == expect and vals[best] <= 1.0102 * min(vals)
     assert gctx.batch_variant_for(Ls) == best
     assert gctx.batch_variant_for(list(reversed(Ls))) == best      # keyed on the multiset of L
     assert gctx.batch_variant_for([32, 48]) == -1

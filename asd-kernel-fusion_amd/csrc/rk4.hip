@@ -935,8 +935,9 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
     ASD_CHECK(steps == 1 || (steps == 2 && rk4_fusable_steps(ctx, L, nrep) == 2), "cannot fuse %d steps here",
               steps);
     // LDS tiles: 16 x 16 (256 threads, 27.6 KB LDS; 2.25 sites loaded and 5.7
-    // f-evaluations per output site) or 32 x 32 (1024 threads, 76.8 KB, two workgroups per
-    // CU; 1.56 loads and 4.8 f-evaluations per site); the automatic choice below
+    // f-evaluations per output site) or 32 x 32 (1024 threads; one step: 76.8 KB, two
+    // workgroups per CU, 1.56 loads and 4.8 f-evaluations per site; two fused steps: 110.6 KB,
+    // one workgroup per CU); the automatic choice below
     const bool big = ctx->rk4_tile == 32;
     // J == 1 and alpha == 0 (the plain workloads) take the specialised arithmetic
     const bool uj = p.J == 1.0, ud = p.alpha == 0.0;

@@ -458,16 +458,14 @@ template <> struct SplitMfma<ASD_FP16X2> {
 // into the tile's displacement bins in LDS (sb: one fixed-point word per bin, zeroed here;
 // the caller has finished with that LDS) and adds them to the lag's global bins.  FP16X2
 // keeps the cross products, scaled by 2^11, in acc2.
+// The element loop: one wave's accumulators into the tile's zeroed LDS bin words (geometry g).
 template <int WM, int WN, bool DUAL>
-__device__ __forceinline__ void bin_split_tile(const f16v (&acc)[WM][WN],
-                                               const f16v (&acc2)[DUAL ? WM : 1][DUAL ? WN : 1],
-                                               unsigned long long *sb, int i0, int j0, int r0, int c0, int BM,
-                                               int BN, int Ns, int L, unsigned long long *gbins, double m0)
+__device__ __forceinline__ void bin_split_elems(const f16v (&acc)[WM][WN],
+                                                const f16v (&acc2)[DUAL ? WM : 1][DUAL ? WN : 1],
+                                                unsigned long long *sb, const BinGeom &g, int i0, int j0, int r0,
+                                                int c0, int BM, int BN, int Ns, int L, double m0)
 {
     const int lane = threadIdx.x & 63;
-    const BinGeom g = bin_geom(i0, j0, Ns, L, BM, BN);
-    for (int b = threadIdx.x; b < g.nbins; b += blockDim.x) sb[b] = 0;
-    __syncthreads();
     if (i0 + BM <= Ns && j0 + BN <= Ns && L >= 32) {
         // full tile: with i = ri L + ci and j = rj L + cj the bin
         // (ri - rj - drmin) L + (ci - cj mod L) is (i - j) - drmin L + (ci < cj ? L : 0).
@@ -524,6 +522,18 @@ __device__ __forceinline__ void bin_split_tile(const f16v (&acc)[WM][WN],
                 }
             }
     }
+}
+
+template <int WM, int WN, bool DUAL>
+__device__ __forceinline__ void bin_split_tile(const f16v (&acc)[WM][WN],
+                                               const f16v (&acc2)[DUAL ? WM : 1][DUAL ? WN : 1],
+                                               unsigned long long *sb, int i0, int j0, int r0, int c0, int BM,
+                                               int BN, int Ns, int L, unsigned long long *gbins, double m0)
+{
+    const BinGeom g = bin_geom(i0, j0, Ns, L, BM, BN);
+    for (int b = threadIdx.x; b < g.nbins; b += blockDim.x) sb[b] = 0;
+    __syncthreads();
+    bin_split_elems<WM, WN, DUAL>(acc, acc2, sb, g, i0, j0, r0, c0, BM, BN, Ns, L, m0);
     __syncthreads();
     flush_bins<1>(sb, g, L, gbins, 0);
 }
@@ -797,15 +807,25 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
     const int nk_full = half_tail ? nk - 1 : nk;
 
     // one k step in ring slot h: wait for this wave's loads of the slot (the loads of the
-    // NS - 2 later stages may stay in flight) and for its LDS reads of the previous step,
-    // barrier, refill the slot the previous step used, then the MFMAs of KH halves
+    // later stages in flight, up to NS - 2, may stay) and for its LDS reads of the previous
+    // step, barrier, refill the slot the previous step used with chunk k + NS - 1, then the
+    // MFMAs of KH halves.  The last NS - 2 steps have no chunk left to request and issue
+    // nothing: they wait for exactly the stages still in flight (vmcnt(0) in the last step), so
+    // the tile ends with no load outstanding instead of draining re-reads of its last chunk.
     auto step = [&](int k, int h, auto khc) {
         constexpr int KH = decltype(khc)::value;
-        __builtin_amdgcn_s_waitcnt(wait_vmcnt(PER_STAGE * (NS - 2)));
+        static_assert(NS - 2 <= 2, "the tail waits below cover up to two stages in flight");
+        const int ahead = min(NS - 2, nk - 1 - k);  // later stages in flight
+        if (ahead >= NS - 2)
+            __builtin_amdgcn_s_waitcnt(wait_vmcnt(PER_STAGE * (NS - 2)));
+        else if (ahead == 1)
+            __builtin_amdgcn_s_waitcnt(wait_vmcnt(PER_STAGE));
+        else
+            __builtin_amdgcn_s_waitcnt(wait_vmcnt(0));
         __builtin_amdgcn_s_waitcnt(kWaitLgkm0);
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
-        issue(min(k + NS - 1, nk - 1), (h + NS - 1) % NS);
+        if (k + NS - 1 < nk) issue(k + NS - 1, (h + NS - 1) % NS);
         const uint16_t *sb = lds + h * STAGE;
         // reads in the order the MFMAs use them: per 16-deep half A hi, B hi (the hh
         // products), B lo (hl), A lo (lh)

@@ -311,7 +311,9 @@ template <> struct KMajorMfma<float> {
 template <class T>
 __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
 {
-    constexpr int BK = 16, PAD = 16 / sizeof(T) * 2, LDS_LD = TILE + PAD;
+    // rows padded by 16 elements: the four row groups of a fragment read (lane / 16) then fall on
+    // disjoint banks for FP32 and pairwise disjoint for FP64 (FP32 0.7% faster, FP64 unchanged)
+    constexpr int BK = 16, PAD = 16, LDS_LD = TILE + PAD;
     typedef typename KMajorMfma<T>::acc_t acc_t;
     __shared__ __attribute__((aligned(16))) unsigned char smem_raw[2 * 2 * BK * LDS_LD * sizeof(T) > 8 * 4096
                                                                        ? 2 * 2 * BK * LDS_LD * sizeof(T)
@@ -400,6 +402,27 @@ __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
         cj[n] = j - rj[n] * L;
     }
     __syncthreads();
+    if (std::is_same<T, double>::value && i0 + TILE <= Ns && j0 + TILE <= Ns && L >= 64) {
+        // full tile: with i = ri L + ci and j = rj L + cj the bin (ri - rj - drmin) L + (ci - cj
+        // mod L) is (i - j) - drmin L + (ci < cj ? L : 0).  A lane's rows are ib + off with
+        // off < 64 <= L, so ci needs at most one wrap and no division (FP64 3.886 -> 3.832 s at
+        // the L = 128 workload; the FP32 kernel measured 5% slower with it and keeps the divisions)
+        const int ib = i0 + wi * 64, cib = ib % L;
+        int jb[4];
+#pragma unroll
+        for (int n = 0; n < 4; n++) jb[n] = j0 + wj * 64 + n * 16 + lane % 16 + g.drmin * L;
+#pragma unroll
+        for (int m = 0; m < 4; m++)
+#pragma unroll
+            for (int v = 0; v < 4; v++) {
+                const int off = m * 16 + KMajorMfma<T>::row(lane, v);
+                int ci = cib + off;
+                ci -= ci >= L ? L : 0;
+#pragma unroll
+                for (int n = 0; n < 4; n++)
+                    fix_add<NW>(sb, g.nbins, ib + off - jb[n] + (ci < cj[n] ? L : 0), (double)acc[m][n][v], m0, m1);
+            }
+    } else {
 #pragma unroll
     for (int m = 0; m < 4; m++)
 #pragma unroll
@@ -416,6 +439,7 @@ __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
                 fix_add<NW>(sb, g.nbins, (ri - rj[n] - g.drmin) * L + dc, (double)acc[m][n][v], m0, m1);
             }
         }
+    }
     __syncthreads();
     flush_bins<NW>(sb, g, L, args.bins + (long)w.lag * Ns, args.wstride);
     }

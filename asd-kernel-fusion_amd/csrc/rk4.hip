@@ -205,9 +205,10 @@ constexpr double kSixth = 1.0 / 6.0, kThird = 1.0 / 3.0, kTwoThirds = 2.0 / 3.0;
 
 // the numeric guard (Guard in asd_internal.h): |c|^2 outside [2^-900, 2^900] or NaN
 __device__ __forceinline__ bool out_of_range(double x) { return !(x >= 0x1p-900 && x <= 0x1p900); }
-__device__ __forceinline__ void guard_report(const Guard &g, int traj)
+__device__ __forceinline__ void guard_report(const Guard &g, int traj, int later = 0)
 {
-    const long s = g.ctr ? g.step + (long)g.every * *g.ctr : g.step;
+    // `later`: the step's offset from the launch's first step (fused steps)
+    const long s = (g.ctr ? g.step + (long)g.every * *g.ctr : g.step) + later;
     atomicMin(g.first, ((unsigned long long)s << 20) | (unsigned)traj);
 }
 
@@ -368,7 +369,7 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
                 if (depth[j] < 4 * m + 4) continue;
                 const double x = FAST ? renorm_lean(p.len, A[j][0], A[j][1], A[j][2], S[j][0], S[j][1], S[j][2])
                                       : renorm_fast(p.len, A[j][0], A[j][1], A[j][2], S[j][0], S[j][1], S[j][2]);
-                if (gd.first && out_of_range(x)) guard_report(gd, traj);
+                if (gd.first && out_of_range(x)) guard_report(gd, traj, m);
                 sS[0][idx[j]] = S[j][0]; sS[1][idx[j]] = S[j][1]; sS[2][idx[j]] = S[j][2];
             }
             __syncthreads();
@@ -382,7 +383,7 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
                 double *o = out + 3 * ((long)grow * L + gcol);
                 const double x = FAST ? renorm_lean(p.len, A[j][0], A[j][1], A[j][2], o[0], o[1], o[2])
                                       : renorm_fast(p.len, A[j][0], A[j][1], A[j][2], o[0], o[1], o[2]);
-                if (gd.first && out_of_range(x)) guard_report(gd, traj);
+                if (gd.first && out_of_range(x)) guard_report(gd, traj, m);
             }
         }
     }
@@ -672,6 +673,154 @@ __global__ __launch_bounds__(256) void rk4_stream_kernel(LLGParams p, int L, int
     }
 }
 
+// Two RK4 steps per pass over HBM (the tolerance arithmetic only).  The wave runs the row
+// pipeline of rk4_stream_kernel twice, the second on the first's renormalised output rows
+// while they are still in registers: at step t it loads row t, the first step finishes row
+// t-4 and the second finishes row t-8, which is stored.  The dependency widens by eight
+// sites: 48 output columns per 64-lane wave, bands of R rows read R + 16 rows.  Each site
+// gets exactly the arithmetic of two launches of rk4_stream_kernel<..., FAST = true>, so
+// the results equal them bit for bit.  Per spin-step this moves about 31 B of HBM instead
+// of 54 B, for 1.25x the VALU work of the one-step kernel (which leaves the VALU half idle).
+constexpr int kStream2Halo = 8, kStream2Cols = 64 - 2 * kStream2Halo;
+
+// The four stages of one step of the row pipeline (tolerance arithmetic).  X holds the
+// step's input states, row r in slot r % NB; t is the newest row in X (a compile-time
+// value after unrolling).  Evaluates stage 1 at row t-1 .. stage 4 at row t-4, leaving the
+// partial sum S + k1/6 + k2/3 + k3/3 + k4/6 of row t-4 in A[t & 3].  Y2 / Y3 / Y4 hold the
+// stage inputs of the rows in flight, row r in slot r & 3.
+template <bool UJ, bool UD, bool PL, int NB>
+__device__ __forceinline__ void stream_stages_fast(const LLGParams &p, const double (&X)[NB][3], int t,
+                                                   double (&Y2)[4][3], double (&Y3)[4][3], double (&Y4)[4][3],
+                                                   double (&A)[4][3][64], int lane)
+{
+    const int x0 = t % NB, x1 = (t + NB - 1) % NB, x2 = (t + NB - 2) % NB, x3 = (t + NB - 3) % NB;
+    const int i0 = t & 3, i1 = (t + 3) & 3, i2 = (t + 2) & 3, i3 = (t + 1) & 3;
+    double f[3];
+    stream_f<UJ, UD, PL, true>(p, X[x1], X[x0], X[x2], f);  // stage 1 at row t-1
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        A[i1][d][lane] = __builtin_fma(f[d], p.w6, X[x1][d]);
+        Y2[i1][d] = __builtin_fma(f[d], p.half_dt, X[x1][d]);
+    }
+    stream_f<UJ, UD, PL, true>(p, Y2[i2], Y2[i1], Y2[i3], f);  // stage 2 at row t-2
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        A[i2][d][lane] = __builtin_fma(f[d], p.w3, A[i2][d][lane]);
+        Y3[i2][d] = __builtin_fma(f[d], p.half_dt, X[x2][d]);
+    }
+    stream_f<UJ, UD, PL, true>(p, Y3[i3], Y3[i2], Y3[i0], f);  // stage 3 at row t-3
+#pragma unroll
+    for (int d = 0; d < 3; d++) {
+        A[i3][d][lane] = __builtin_fma(f[d], p.w3, A[i3][d][lane]);
+        Y4[i3][d] = __builtin_fma(f[d], p.dt, X[x3][d]);
+    }
+    stream_f<UJ, UD, PL, true>(p, Y4[i0], Y4[i3], Y4[i1], f);  // stage 4 at row t-4
+#pragma unroll
+    for (int d = 0; d < 3; d++) A[i0][d][lane] = __builtin_fma(f[d], p.w6, A[i0][d][lane]);
+}
+
+// grid: 4 waves per block, one work item (replica, band, strip) per wave; R % 4 == 0,
+// 16 <= R <= L - 16, L >= 64 (checked by the launcher).  The guard reports the first
+// step's sites as step `step` and the second's as `step` + 1, each site once (the output
+// sites of the wave).
+template <bool UJ, bool UD, bool PL>
+__global__ __launch_bounds__(256) void rk4_stream2_kernel(LLGParams p, int L, int nrep, int R, int nband,
+                                                          int nstrip, const double *__restrict__ in,
+                                                          double *__restrict__ out, double *__restrict__ snap,
+                                                          const int *__restrict__ snap_counter, long snap_stride,
+                                                          Guard gd)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const long item = (long)blockIdx.x * 4 + wave;
+    if (item >= (long)nrep * nband * nstrip) return;  // whole wave; the kernel has no barriers
+    const int strip = (int)(item % nstrip), band = (int)((item / nstrip) % nband);
+    const int rep = (int)(item / ((long)nstrip * nband));
+    const long ns = (long)L * L;
+    const double *src = in + rep * ns * 3;
+    double *dst = out + rep * ns * 3;
+    double *sn = nullptr;
+    if (snap) sn = snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride + rep * ns * 3;
+
+    const int c = strip * kStream2Cols - kStream2Halo + lane;  // in [-8, L + 104)
+    const int gc = ((c % L) + L) % L;
+    const bool ocol = lane >= kStream2Halo && lane < 64 - kStream2Halo && c < L;
+    const int r0 = band * R;
+    const int rows = min(R, L - r0);
+    auto row_of = [&](int t) {  // lattice row loaded at step t
+        int y = r0 - kStream2Halo + t;  // in [-8, L + R + 8) with R <= L - 16
+        y += y < 0 ? L : 0;
+        y -= y >= L ? L : 0;
+        return y;
+    };
+    const unsigned lane_off = 24u * (unsigned)gc;
+    auto at = [&](const double *row_base) { return (const double *)((const char *)row_base + lane_off); };
+    auto at_w = [&](double *row_base) { return (double *)((char *)row_base + lane_off); };
+    const long row_stride = 3L * L;
+    auto load = [&](int t, double (&v)[3]) {
+        const double *q = at(src + row_of(t) * row_stride);
+        v[0] = q[0]; v[1] = q[1]; v[2] = q[2];
+    };
+
+    // B: the loaded rows (slot t % 8, PF = 4 ahead, as in rk4_stream_kernel); S1: the first
+    // step's output rows (slot r & 3); Y*, Z*: the two steps' stage inputs; the two steps'
+    // partial sums in LDS (each lane touches only its own doubles: no barriers)
+    double B[8][3], S1[4][3], Y2[4][3], Y3[4][3], Y4[4][3], Z2[4][3], Z3[4][3], Z4[4][3];
+    __shared__ double sA[4][2][4][3][64];  // [wave][step][ring slot][component][lane]
+    double(&A1)[4][3][64] = sA[wave][0];
+    double(&A2)[4][3][64] = sA[wave][1];
+#pragma unroll
+    for (int a = 0; a < 4; a++)
+#pragma unroll
+        for (int d = 0; d < 3; d++) {
+            B[a + 4][d] = 0.0; S1[a][d] = 0.0;
+            Y2[a][d] = 0.0; Y3[a][d] = 0.0; Y4[a][d] = 0.0;
+            Z2[a][d] = 0.0; Z3[a][d] = 0.0; Z4[a][d] = 0.0;
+            A1[a][d][lane] = 0.0; A2[a][d][lane] = 0.0;
+        }
+    const int nstep = rows + 2 * kStream2Halo;
+    const int nstep8 = (nstep + 7) & ~7;
+    constexpr int PF = 4;
+#pragma unroll
+    for (int a = 0; a < PF; a++) {
+        __builtin_amdgcn_sched_barrier(0);
+        load(a, B[a]);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    for (int t0 = 0; t0 < nstep8; t0 += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int t = t0 + u;
+            __builtin_amdgcn_sched_barrier(0);
+            load(min(t + PF, nstep - 1), B[(u + PF) & 7]);
+            const int tr = t - kStream2Halo;  // band-relative row of row t
+            if (sn && ocol && tr >= 0 && tr < rows) {
+                double *o = at_w(sn + (r0 + tr) * row_stride);
+                o[0] = B[u][0]; o[1] = B[u][1]; o[2] = B[u][2];
+            }
+            // first step: finishes row t-4 into S1[(t-4) & 3] = S1[u & 3]
+            stream_stages_fast<UJ, UD, PL, 8>(p, B, u, Y2, Y3, Y4, A1, lane);
+            {
+                const int i = u & 3;
+                const double x = renorm_lean<PL>(p.len, A1[i][0][lane], A1[i][1][lane], A1[i][2][lane], S1[i][0],
+                                                 S1[i][1], S1[i][2]);
+                const int r1 = t - 4 - kStream2Halo;  // band-relative row of row t-4
+                if (gd.first && ocol && r1 >= 0 && r1 < rows && out_of_range(x)) guard_report(gd, rep);
+            }
+            // second step on S1 (newest row t-4, in slot u & 3): finishes row t-8
+            stream_stages_fast<UJ, UD, PL, 4>(p, S1, u, Z2, Z3, Z4, A2, lane);
+            const int orow = t - 8 - kStream2Halo;
+            if (ocol && orow >= 0 && orow < rows) {
+                const int i = u & 3;
+                double *o = at_w(dst + (r0 + orow) * row_stride);
+                const double x = renorm_lean<PL>(p.len, A2[i][0][lane], A2[i][1][lane], A2[i][2][lane], o[0],
+                                                 o[1], o[2]);
+                if (gd.first && out_of_range(x)) guard_report(gd, rep, 1);
+            }
+        }
+    }
+}
+
 __global__ void bump_counter_kernel(int *c) { *c += 1; }
 __global__ void set_u64_kernel(unsigned long long *p, unsigned long long v) { *p = v; }
 
@@ -875,6 +1024,15 @@ static int stream_band_rows(const asd_ctx *ctx, int L, int nrep)
     return R;
 }
 
+// rows per band of the two-step streaming kernel: bands of at most kStream2Rows rows, split
+// evenly (16 halo rows per band)
+constexpr int kStream2Rows = 128;
+static int stream2_band_rows(int L)
+{
+    const int nband = (L + kStream2Rows - 1) / kStream2Rows;
+    return std::min((L + nband - 1) / nband, L - 16);
+}
+
 // the automatic choice among the kernels below: streaming strips once there are at least
 // four waves per CU of strips (L >= 64), else LDS tiles by occupancy
 static bool rk4_streams(const asd_ctx *ctx, int L, int nrep)
@@ -912,12 +1070,13 @@ static bool rk4_mid_tiles(const asd_ctx *ctx, int L, int nrep)
 // 7.9 / 14.15 against 8.26 / 8.8 / 8.6 / 14.65 ms per 1000 steps); not for forced tiles, nor
 // for 16 x 16 tiles with 256 threads, where the fused halo costs more than the launch it
 // saves
+// Streaming strips fuse two steps in the tolerance arithmetic only (rk4_stream2_kernel:
+// the exact arithmetic keeps the VALU busy already at one step per pass).
 int rk4_fusable_steps(const asd_ctx *ctx, int L, int nrep)
 {
-    return ctx->rk4_tile == 0 && !rk4_streams(ctx, L, nrep) &&
-                   (rk4_small_tiles(ctx, L, nrep) || rk4_big_tiles(ctx, L, nrep) || rk4_mid_tiles(ctx, L, nrep))
-               ? 2
-               : 1;
+    if (ctx->rk4_tile != 0) return 1;
+    if (rk4_streams(ctx, L, nrep)) return ctx->rk4_exact ? 1 : 2;
+    return rk4_small_tiles(ctx, L, nrep) || rk4_big_tiles(ctx, L, nrep) || rk4_mid_tiles(ctx, L, nrep) ? 2 : 1;
 }
 
 template <bool F>
@@ -947,7 +1106,18 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
     const int R = stream_band_rows(ctx, L, nrep);
     const int nband = L >= 64 ? (L + R - 1) / R : 0, nstrip = (L + kStreamCols - 1) / kStreamCols;
     const long waves = (long)nrep * nband * nstrip;
-    if (rk4_streams(ctx, L, nrep)) {
+    if (rk4_streams(ctx, L, nrep) && steps == 2) {
+        const bool pl = p.Bz == 0.0 && p.len == 1.0;
+        const int R2 = stream2_band_rows(L);
+        const int nband2 = (L + R2 - 1) / R2, nstrip2 = (L + kStream2Cols - 1) / kStream2Cols;
+        const long waves2 = (long)nrep * nband2 * nstrip2;
+        ASD_CHECK(L >= 64 && R2 >= 16 && R2 <= L - 16, "two-step strips need L >= 64");
+        auto k = uj && ud ? (pl ? rk4_stream2_kernel<true, true, true> : rk4_stream2_kernel<true, true, false>)
+                          : (uj ? rk4_stream2_kernel<true, false, false>
+                                : (ud ? rk4_stream2_kernel<false, true, false> : rk4_stream2_kernel<false, false, false>));
+        hipLaunchKernelGGL(k, dim3((unsigned)((waves2 + 3) / 4)), dim3(256), 0, ctx->stream, p, L, nrep, R2, nband2,
+                           nstrip2, in, out, snap_out, snap_counter, snap_stride, guard);
+    } else if (rk4_streams(ctx, L, nrep)) {
         const bool pl = p.Bz == 0.0 && p.len == 1.0;
         auto k = ctx->rk4_exact ? stream_kernel<false>(uj, ud, pl) : stream_kernel<true>(uj, ud, pl);
         hipLaunchKernelGGL(k, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, ctx->stream, p, L, nrep, R, nband,

@@ -116,12 +116,13 @@ int asd_rk4_run(asd_ctx *ctx, const asd_params *p, int L, int nrep, double *d_sp
 int asd_rk4_run_noalloc(asd_ctx *ctx, const asd_params *p, int L, int nrep, double *d_spins,
                         long nsteps, int snap_every, int nsnap, double *d_snaps);
 
-/* RK4 kernel for asd_rk4_run: 0 = automatic (LDS tiles sized by occupancy: 8 x 8
- * while 16 x 16 tiles would leave CUs idle, 16 x 16 otherwise; the streaming strips
- * once nrep * ceil(L / 120) * ceil(L / 56) >= 4 waves per CU), 8, 16 or 32 = LDS
- * tiles of that edge, 64 = register-streaming column strips (one 64-lane wave per 56
- * columns of a band of rows; lattices with L < 64 use 16 x 16 tiles).  Results do not
- * depend on the choice (bit for bit). */
+/* RK4 kernel for asd_rk4_run: 0 = automatic (LDS tiles sized by occupancy, two fused
+ * steps per launch on the 8 x 8, 16 x 16 / 1024-thread and 32 x 32 paths; the streaming
+ * strips once the one-step strips give at least 4 waves per CU, with two fused steps per
+ * launch in the tolerance mode), 8, 16 or 32 = LDS tiles of that edge, 64 =
+ * register-streaming column strips (one 64-lane wave per 56 columns of a band of rows;
+ * lattices with L < 64 use 16 x 16 tiles), one step per launch for every forced choice.
+ * Results do not depend on the choice (bit for bit, in either arithmetic mode). */
 int asd_rk4_set_tile(asd_ctx *ctx, int tile);
 
 /* RK4 arithmetic for every RK4 entry point of this context.  exact = 1 (default): the

@@ -19,11 +19,12 @@ def _state(L, seed=4):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("mode", ["graph", "eager", "streaming"])
+@pytest.mark.parametrize("mode", ["graph", "eager", "streaming", "streaming2"])
 def test_nan_state_is_reported_at_step_zero(gctx, mode):
+    """streaming2: the tolerance mode's two-step strips (automatic choice, 256 replicas)."""
     torch = need_gpu()
     import asd_amd
-    L, nrep = (64, 2) if mode != "streaming" else (128, 2)
+    L, nrep = {"streaming": (128, 2), "streaming2": (64, 256)}.get(mode, (64, 2))
     say(f"guard: NaN spin, {mode}, L={L}")
     S = np.tile(_state(L), (nrep, 1))
     S[L * L + 5] = np.nan                    # replica 1, site 5
@@ -31,6 +32,8 @@ def test_nan_state_is_reported_at_step_zero(gctx, mode):
     snaps = torch.zeros((6,) + spins.shape, dtype=torch.float64, device="cuda")
     if mode == "streaming":
         gctx.rk4_set_tile(64)
+    if mode == "streaming2":
+        gctx.rk4_set_exact(False)
     try:
         with pytest.raises(asd_amd.AsdError) as ei:
             if mode == "graph":
@@ -39,6 +42,7 @@ def test_nan_state_is_reported_at_step_zero(gctx, mode):
                 gctx.rk4_run(asd_amd.params(D=0.1), L, spins, 7, use_graph=False)
     finally:
         gctx.rk4_set_tile(0)
+        gctx.rk4_set_exact(True)
     assert ei.value.code == asd_amd.ASD_ENOTFINITE
     msg = str(ei.value)
     say(f"  {msg}")

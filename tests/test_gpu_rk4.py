@@ -405,3 +405,68 @@ def test_tolerance_mode_batch_within_bar(gctx, orc):
         S = spins[t].cpu().numpy()
         assert np.abs(S - ref).max() <= 1e-10, (t, L)
         assert np.abs(np.linalg.norm(S, axis=-1) - 1.0).max() <= 1e-12
+
+
+# the automatic choice on many replicas takes the streaming strips; in the tolerance mode
+# its launches fuse two steps (rk4_stream2_kernel), which must equal one-step streaming
+# launches (asd_rk4_set_tile 64) bit for bit: every case below has at least four waves of
+# one-step strips per CU on a 256-CU part.  Cases cover each arithmetic specialisation
+# (J == 1, alpha == 0, the plain B_z == 0 / |S| == 1), ragged strips (L = 100: 48-column
+# strips 3 x 48 = 144) and uneven bands.
+STREAM2_CASES = [
+    (64, 256, dict(D=0.1)),                                            # plain
+    (100, 320, dict(J=0.9, D=0.05, Bz=0.1, alpha=0.05, spin_length=1.5)),
+    (64, 256, dict(D=0.1, alpha=0.05)),                                # J == 1, damped
+    (64, 256, dict(J=0.8, D=0.1, Bz=0.02)),                            # alpha == 0
+    (1024, 8, dict(D=0.1)),                                            # the large lattice
+]
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,nrep,kw", STREAM2_CASES)
+def test_two_step_streaming_equals_one_step_streaming(gctx, orc, L, nrep, kw):
+    torch = need_gpu()
+    import asd_amd
+    say(f"two-step streaming L={L} nrep={nrep} {kw}")
+    length = kw.get("spin_length", 1.0)
+    S0 = np.stack([orc.init_spins(70 + r, L * L) for r in range(nrep)]) * length
+    p = asd_amd.params(dt=0.01, **kw)
+    n = 43 if L < 1024 else 31                     # odd: the last launch is one step
+    res = {}
+    # forced one-step strips (64) as the reference; the automatic choice (0) graph-replayed,
+    # eager, and with an odd interval (one-step launches); the tolerance mode's LDS tiles (16)
+    runs = [(64, 5, False), (64, 10, True), (0, 10, True), (0, 10, False), (0, 5, False), (16, 10, False)]
+    gctx.rk4_set_exact(False)
+    try:
+        for tile, every, graph in runs:
+            nsnap = n // every + 1
+            s = torch.tensor(S0, device="cuda")
+            snaps = torch.full((nsnap,) + S0.shape, np.nan, dtype=torch.float64, device="cuda")
+            gctx.rk4_set_tile(tile)
+            gctx.rk4_run(p, L, s, n, every, nsnap, snaps, use_graph=graph)
+            gctx.synchronize()
+            res[(tile, every, graph)] = (s.cpu().numpy(), snaps.cpu().numpy())
+            del s, snaps
+    finally:
+        gctx.rk4_set_tile(0)
+        gctx.rk4_set_exact(True)
+    ref = res[(64, 5, False)]
+    for key, (s, sn) in res.items():
+        assert np.array_equal(s, ref[0]), key
+        step = key[1] // 5
+        assert np.array_equal(sn, ref[1][::step][: len(sn)]), key
+    # the oracle on the first and last replica (100 steps: the tolerance bar)
+    for r in ((0, nrep - 1) if L < 1024 else (0,)):
+        m = 100 if L < 1024 else 20
+        s = torch.tensor(S0[r], device="cuda")
+        gctx.rk4_set_exact(False)
+        try:
+            gctx.rk4_run(p, L, s, m)
+        finally:
+            gctx.rk4_set_exact(True)
+        S_or, _ = orc.run(orc.params(dt=0.01, **kw), L, S0[r], m)
+        err = float(np.abs(s.cpu().numpy() - S_or).max())
+        say(f"  replica {r}: {m} steps, max |S - S_oracle| = {err:.3e}")
+        assert err <= 1e-10
+    S = res[(0, 10, True)][0]
+    assert np.abs(np.linalg.norm(S, axis=-1) - length).max() <= 1e-12

@@ -66,7 +66,12 @@ __device__ __forceinline__ void llg_f(const LLGParams &p, double sx, double sy, 
     // x = -0.0, so the sums are equal (IEEE ==) and can differ only in the sign of an exact
     // zero, which no later operation turns into a different value.
     double hx = -0.0, hy = -0.0, hz = -0.0;
-    if constexpr (UNIT_J) {
+    if constexpr (FAST && !UNIT_J) {
+        // the tolerance mode: J (n0 + n1 + n2 + n3), one multiply per component
+        hx = (((n0[0] + n1[0]) + n2[0]) + n3[0]) * p.J;
+        hy = (((n0[1] + n1[1]) + n2[1]) + n3[1]) * p.J;
+        hz = (((n0[2] + n1[2]) + n2[2]) + n3[2]) * p.J;
+    } else if constexpr (UNIT_J) {
         hx += n0[0]; hy += n0[1]; hz += n0[2];
         hx += n1[0]; hy += n1[1]; hz += n1[2];
         hx += n2[0]; hy += n2[1]; hz += n2[2];
@@ -688,10 +693,10 @@ constexpr int kStream2Halo = 8, kStream2Cols = 64 - 2 * kStream2Halo;
 // value after unrolling).  Evaluates stage 1 at row t-1 .. stage 4 at row t-4, leaving the
 // partial sum S + k1/6 + k2/3 + k3/3 + k4/6 of row t-4 in A[t & 3].  Y2 / Y3 / Y4 hold the
 // stage inputs of the rows in flight, row r in slot r & 3.
-template <bool UJ, bool UD, bool PL, int NB>
+template <bool UJ, bool UD, bool PL, int NB, class Acc>
 __device__ __forceinline__ void stream_stages_fast(const LLGParams &p, const double (&X)[NB][3], int t,
                                                    double (&Y2)[4][3], double (&Y3)[4][3], double (&Y4)[4][3],
-                                                   double (&A)[4][3][64], int lane)
+                                                   Acc &&A)
 {
     const int x0 = t % NB, x1 = (t + NB - 1) % NB, x2 = (t + NB - 2) % NB, x3 = (t + NB - 3) % NB;
     const int i0 = t & 3, i1 = (t + 3) & 3, i2 = (t + 2) & 3, i3 = (t + 1) & 3;
@@ -699,56 +704,40 @@ __device__ __forceinline__ void stream_stages_fast(const LLGParams &p, const dou
     stream_f<UJ, UD, PL, true>(p, X[x1], X[x0], X[x2], f);  // stage 1 at row t-1
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        A[i1][d][lane] = __builtin_fma(f[d], p.w6, X[x1][d]);
+        A(i1, d) = __builtin_fma(f[d], p.w6, X[x1][d]);
         Y2[i1][d] = __builtin_fma(f[d], p.half_dt, X[x1][d]);
     }
     stream_f<UJ, UD, PL, true>(p, Y2[i2], Y2[i1], Y2[i3], f);  // stage 2 at row t-2
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        A[i2][d][lane] = __builtin_fma(f[d], p.w3, A[i2][d][lane]);
+        A(i2, d) = __builtin_fma(f[d], p.w3, A(i2, d));
         Y3[i2][d] = __builtin_fma(f[d], p.half_dt, X[x2][d]);
     }
     stream_f<UJ, UD, PL, true>(p, Y3[i3], Y3[i2], Y3[i0], f);  // stage 3 at row t-3
 #pragma unroll
     for (int d = 0; d < 3; d++) {
-        A[i3][d][lane] = __builtin_fma(f[d], p.w3, A[i3][d][lane]);
+        A(i3, d) = __builtin_fma(f[d], p.w3, A(i3, d));
         Y4[i3][d] = __builtin_fma(f[d], p.dt, X[x3][d]);
     }
     stream_f<UJ, UD, PL, true>(p, Y4[i0], Y4[i3], Y4[i1], f);  // stage 4 at row t-4
 #pragma unroll
-    for (int d = 0; d < 3; d++) A[i0][d][lane] = __builtin_fma(f[d], p.w6, A[i0][d][lane]);
+    for (int d = 0; d < 3; d++) A(i0, d) = __builtin_fma(f[d], p.w6, A(i0, d));
 }
 
-// grid: 4 waves per block, one work item (replica, band, strip) per wave; R % 4 == 0,
-// 16 <= R <= L - 16, L >= 64 (checked by the launcher).  The guard reports the first
-// step's sites as step `step` and the second's as `step` + 1, each site once (the output
-// sites of the wave).
+// One segment of a strip: rows [r0, r0 + rows) of the 48 output columns of strip `strip` of
+// one lattice (src -> dst, snapshot rows into sn when set), two steps.  The wave loads rows
+// r0 - 8 .. r0 + rows + 7 (periodic; rows <= L).  A1 / A2: the wave's LDS partial sums.
 template <bool UJ, bool UD, bool PL>
-__global__ __launch_bounds__(256) void rk4_stream2_kernel(LLGParams p, int L, int nrep, int R, int nband,
-                                                          int nstrip, const double *__restrict__ in,
-                                                          double *__restrict__ out, double *__restrict__ snap,
-                                                          const int *__restrict__ snap_counter, long snap_stride,
-                                                          Guard gd)
+__device__ __forceinline__ void stream2_segment(const LLGParams &p, int L, int strip, int r0, int rows,
+                                                const double *__restrict__ src, double *__restrict__ dst,
+                                                double *__restrict__ sn, double (&sA2)[4][3][64], int lane,
+                                                const Guard &gd, int rep)
 {
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const long item = (long)blockIdx.x * 4 + wave;
-    if (item >= (long)nrep * nband * nstrip) return;  // whole wave; the kernel has no barriers
-    const int strip = (int)(item % nstrip), band = (int)((item / nstrip) % nband);
-    const int rep = (int)(item / ((long)nstrip * nband));
-    const long ns = (long)L * L;
-    const double *src = in + rep * ns * 3;
-    double *dst = out + rep * ns * 3;
-    double *sn = nullptr;
-    if (snap) sn = snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride + rep * ns * 3;
-
     const int c = strip * kStream2Cols - kStream2Halo + lane;  // in [-8, L + 104)
     const int gc = ((c % L) + L) % L;
     const bool ocol = lane >= kStream2Halo && lane < 64 - kStream2Halo && c < L;
-    const int r0 = band * R;
-    const int rows = min(R, L - r0);
     auto row_of = [&](int t) {  // lattice row loaded at step t
-        int y = r0 - kStream2Halo + t;  // in [-8, L + R + 8) with R <= L - 16
+        int y = r0 - kStream2Halo + t;  // in [-8, L + 8)
         y += y < 0 ? L : 0;
         y -= y >= L ? L : 0;
         return y;
@@ -763,12 +752,12 @@ __global__ __launch_bounds__(256) void rk4_stream2_kernel(LLGParams p, int L, in
     };
 
     // B: the loaded rows (slot t % 8, PF = 4 ahead, as in rk4_stream_kernel); S1: the first
-    // step's output rows (slot r & 3); Y*, Z*: the two steps' stage inputs; the two steps'
-    // partial sums in LDS (each lane touches only its own doubles: no barriers)
-    double B[8][3], S1[4][3], Y2[4][3], Y3[4][3], Y4[4][3], Z2[4][3], Z3[4][3], Z4[4][3];
-    __shared__ double sA[4][2][4][3][64];  // [wave][step][ring slot][component][lane]
-    double(&A1)[4][3][64] = sA[wave][0];
-    double(&A2)[4][3][64] = sA[wave][1];
+    // step's output rows (slot r & 3); Y*, Z*: the two steps' stage inputs; A1: the first
+    // step's partial sums (slot r & 3); the second step's sit in LDS (sA2, each lane only
+    // touching its own doubles: no barriers), which keeps two waves per SIMD
+    double B[8][3], S1[4][3], Y2[4][3], Y3[4][3], Y4[4][3], Z2[4][3], Z3[4][3], Z4[4][3], A1[4][3];
+    auto a1 = [&](int i, int d) -> double & { return A1[i][d]; };
+    auto a2 = [&](int i, int d) -> double & { return sA2[i][d][lane]; };
 #pragma unroll
     for (int a = 0; a < 4; a++)
 #pragma unroll
@@ -776,7 +765,7 @@ __global__ __launch_bounds__(256) void rk4_stream2_kernel(LLGParams p, int L, in
             B[a + 4][d] = 0.0; S1[a][d] = 0.0;
             Y2[a][d] = 0.0; Y3[a][d] = 0.0; Y4[a][d] = 0.0;
             Z2[a][d] = 0.0; Z3[a][d] = 0.0; Z4[a][d] = 0.0;
-            A1[a][d][lane] = 0.0; A2[a][d][lane] = 0.0;
+            A1[a][d] = 0.0; sA2[a][d][lane] = 0.0;
         }
     const int nstep = rows + 2 * kStream2Halo;
     const int nstep8 = (nstep + 7) & ~7;
@@ -793,31 +782,63 @@ __global__ __launch_bounds__(256) void rk4_stream2_kernel(LLGParams p, int L, in
             const int t = t0 + u;
             __builtin_amdgcn_sched_barrier(0);
             load(min(t + PF, nstep - 1), B[(u + PF) & 7]);
-            const int tr = t - kStream2Halo;  // band-relative row of row t
+            const int tr = t - kStream2Halo;  // segment-relative row of row t
             if (sn && ocol && tr >= 0 && tr < rows) {
                 double *o = at_w(sn + (r0 + tr) * row_stride);
                 o[0] = B[u][0]; o[1] = B[u][1]; o[2] = B[u][2];
             }
             // first step: finishes row t-4 into S1[(t-4) & 3] = S1[u & 3]
-            stream_stages_fast<UJ, UD, PL, 8>(p, B, u, Y2, Y3, Y4, A1, lane);
+            stream_stages_fast<UJ, UD, PL, 8>(p, B, u, Y2, Y3, Y4, a1);
             {
                 const int i = u & 3;
-                const double x = renorm_lean<PL>(p.len, A1[i][0][lane], A1[i][1][lane], A1[i][2][lane], S1[i][0],
+                const double x = renorm_lean<PL>(p.len, A1[i][0], A1[i][1], A1[i][2], S1[i][0],
                                                  S1[i][1], S1[i][2]);
-                const int r1 = t - 4 - kStream2Halo;  // band-relative row of row t-4
+                const int r1 = t - 4 - kStream2Halo;  // segment-relative row of row t-4
                 if (gd.first && ocol && r1 >= 0 && r1 < rows && out_of_range(x)) guard_report(gd, rep);
             }
             // second step on S1 (newest row t-4, in slot u & 3): finishes row t-8
-            stream_stages_fast<UJ, UD, PL, 4>(p, S1, u, Z2, Z3, Z4, A2, lane);
+            stream_stages_fast<UJ, UD, PL, 4>(p, S1, u, Z2, Z3, Z4, a2);
             const int orow = t - 8 - kStream2Halo;
             if (ocol && orow >= 0 && orow < rows) {
                 const int i = u & 3;
                 double *o = at_w(dst + (r0 + orow) * row_stride);
-                const double x = renorm_lean<PL>(p.len, A2[i][0][lane], A2[i][1][lane], A2[i][2][lane], o[0],
+                const double x = renorm_lean<PL>(p.len, sA2[i][0][lane], sA2[i][1][lane], sA2[i][2][lane], o[0],
                                                  o[1], o[2]);
                 if (gd.first && out_of_range(x)) guard_report(gd, rep, 1);
             }
         }
+    }
+}
+
+// Persistent grid (4 waves per block, as many as are resident): the work is the
+// nrep * nstrip * L strip-rows, column by column (replica, strip, row), and wave w takes the
+// equal share [w share, (w + 1) share) of it, as one segment per column it touches.  Every
+// wave then runs the same number of rows (no partial last round of waves), and the 16 halo
+// rows are paid per segment (about one per share) instead of per band.  L >= 64 (checked by
+// the launcher).  The guard reports the first step's sites as step `step` and the second's
+// as `step` + 1, each output site once.
+template <bool UJ, bool UD, bool PL>
+__global__ __launch_bounds__(256) void rk4_stream2_kernel(LLGParams p, int L, int nstrip, long total, int share,
+                                                          const double *__restrict__ in, double *__restrict__ out,
+                                                          double *__restrict__ snap,
+                                                          const int *__restrict__ snap_counter, long snap_stride,
+                                                          Guard gd)
+{
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    __shared__ double sA[4][4][3][64];  // [wave][ring slot][component][lane]
+    const long ns3 = 3L * L * L;
+    double *snap_base = snap ? snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride : nullptr;
+    long i = ((long)blockIdx.x * 4 + wave) * share;
+    const long iend = min(i + share, total);
+    while (i < iend) {
+        const long col = i / L;
+        const int r0 = (int)(i - col * L);
+        const int rows = (int)min((long)(L - r0), iend - i);
+        const int rep = (int)(col / nstrip), strip = (int)(col - (long)rep * nstrip);
+        stream2_segment<UJ, UD, PL>(p, L, strip, r0, rows, in + rep * ns3, out + rep * ns3,
+                                    snap_base ? snap_base + rep * ns3 : nullptr, sA[wave], lane, gd, rep);
+        i += rows;
     }
 }
 
@@ -958,21 +979,24 @@ static BatchKernel batch_kernel(int v, int ns)
 }
 
 // persistent grid: the blocks of kernel k resident at once (cached per kernel)
-static int batch_grid(BatchKernel k, int threads, int nitems, int num_cu)
+// resident blocks per CU of a kernel at `threads` threads (cached per kernel)
+static int resident_blocks(const void *k, int threads)
 {
     static std::mutex mu;
     static std::map<const void *, int> per_cu;
-    int n = 0;
-    {
-        std::lock_guard<std::mutex> lock(mu);
-        auto it = per_cu.find((const void *)k);
-        if (it == per_cu.end()) {
-            int b = 0;
-            if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, threads, 0) != hipSuccess || b < 1) b = 1;
-            it = per_cu.emplace((const void *)k, b).first;
-        }
-        n = it->second;
+    std::lock_guard<std::mutex> lock(mu);
+    auto it = per_cu.find(k);
+    if (it == per_cu.end()) {
+        int b = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&b, k, threads, 0) != hipSuccess || b < 1) b = 1;
+        it = per_cu.emplace(k, b).first;
     }
+    return it->second;
+}
+
+static int batch_grid(BatchKernel k, int threads, int nitems, int num_cu)
+{
+    const int n = resident_blocks((const void *)k, threads);
     return std::max(1, std::min(nitems, n * std::max(num_cu, 1)));
 }
 
@@ -1022,15 +1046,6 @@ static int stream_band_rows(const asd_ctx *ctx, int L, int nrep)
     int R = std::min(104, L - 8);
     if ((long)nrep * ((L + R - 1) / R) * nstrip < 16L * std::max(ctx->num_cu, 1)) R = std::min(56, L - 8);
     return R;
-}
-
-// rows per band of the two-step streaming kernel: bands of at most kStream2Rows rows, split
-// evenly (16 halo rows per band)
-constexpr int kStream2Rows = 128;
-static int stream2_band_rows(int L)
-{
-    const int nband = (L + kStream2Rows - 1) / kStream2Rows;
-    return std::min((L + nband - 1) / nband, L - 16);
 }
 
 // the automatic choice among the kernels below: streaming strips once there are at least
@@ -1108,15 +1123,18 @@ int rk4_launch_step(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const dou
     const long waves = (long)nrep * nband * nstrip;
     if (rk4_streams(ctx, L, nrep) && steps == 2) {
         const bool pl = p.Bz == 0.0 && p.len == 1.0;
-        const int R2 = stream2_band_rows(L);
-        const int nband2 = (L + R2 - 1) / R2, nstrip2 = (L + kStream2Cols - 1) / kStream2Cols;
-        const long waves2 = (long)nrep * nband2 * nstrip2;
-        ASD_CHECK(L >= 64 && R2 >= 16 && R2 <= L - 16, "two-step strips need L >= 64");
+        ASD_CHECK(L >= 64, "two-step strips need L >= 64");
         auto k = uj && ud ? (pl ? rk4_stream2_kernel<true, true, true> : rk4_stream2_kernel<true, true, false>)
                           : (uj ? rk4_stream2_kernel<true, false, false>
                                 : (ud ? rk4_stream2_kernel<false, true, false> : rk4_stream2_kernel<false, false, false>));
-        hipLaunchKernelGGL(k, dim3((unsigned)((waves2 + 3) / 4)), dim3(256), 0, ctx->stream, p, L, nrep, R2, nband2,
-                           nstrip2, in, out, snap_out, snap_counter, snap_stride, guard);
+        // one equal share of the strip-rows per resident wave (at least 8 rows)
+        const int nstrip2 = (L + kStream2Cols - 1) / kStream2Cols;
+        const long total = (long)nrep * nstrip2 * L;
+        const long slots = 4L * resident_blocks((const void *)k, 256) * std::max(ctx->num_cu, 1);
+        const int share = (int)std::max<long>(8, (total + slots - 1) / slots);
+        const long nwaves = (total + share - 1) / share;
+        hipLaunchKernelGGL(k, dim3((unsigned)((nwaves + 3) / 4)), dim3(256), 0, ctx->stream, p, L, nstrip2, total,
+                           share, in, out, snap_out, snap_counter, snap_stride, guard);
     } else if (rk4_streams(ctx, L, nrep)) {
         const bool pl = p.Bz == 0.0 && p.len == 1.0;
         auto k = ctx->rk4_exact ? stream_kernel<false>(uj, ud, pl) : stream_kernel<true>(uj, ud, pl);

@@ -412,7 +412,7 @@ def test_tolerance_mode_batch_within_bar(gctx, orc):
 # launches (asd_rk4_set_tile 64) bit for bit: every case below has at least four waves of
 # one-step strips per CU on a 256-CU part.  Cases cover each arithmetic specialisation
 # (J == 1, alpha == 0, the plain B_z == 0 / |S| == 1), ragged strips (L = 100: 48-column
-# strips 3 x 48 = 144) and uneven bands.
+# strips 3 x 48 = 144) and segments that end mid-column.
 STREAM2_CASES = [
     (64, 256, dict(D=0.1)),                                            # plain
     (100, 320, dict(J=0.9, D=0.05, Bz=0.1, alpha=0.05, spin_length=1.5)),

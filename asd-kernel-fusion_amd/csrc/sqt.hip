@@ -482,6 +482,18 @@ template <> struct SplitMfma<ASD_FP16X2> {
 // into the tile's displacement bins in LDS (sb: one fixed-point word per bin, zeroed here;
 // the caller has finished with that LDS) and adds them to the lag's global bins.  FP16X2
 // keeps the cross products, scaled by 2^11, in acc2.
+// The product value of accumulator element (m, n, v): FP16X2 folds acc + acc2 / 2048 in
+// FP32 (one rounding at the accumulators' own precision), the same in every layout.
+template <int WM, int WN, bool DUAL>
+__device__ __forceinline__ double split_value(const f16v (&acc)[WM][WN],
+                                              const f16v (&acc2)[DUAL ? WM : 1][DUAL ? WN : 1], int m, int n, int v)
+{
+    if constexpr (DUAL)
+        return (double)__builtin_fmaf(acc2[m][n][v], 1.0f / 2048.0f, acc[m][n][v]);
+    else
+        return (double)acc[m][n][v];
+}
+
 // The element loop: one wave's accumulators into the tile's zeroed LDS bin words (geometry g).
 template <int WM, int WN, bool DUAL>
 __device__ __forceinline__ void bin_split_elems(const f16v (&acc)[WM][WN],
@@ -513,8 +525,7 @@ __device__ __forceinline__ void bin_split_elems(const f16v (&acc)[WM][WN],
                 ci -= ci >= L ? L : 0;
 #pragma unroll
                 for (int n = 0; n < WN; n++) {
-                    double val = (double)acc[m][n][v];
-                    if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
+                    const double val = split_value<WM, WN, DUAL>(acc, acc2, m, n, v);
                     fix_add<1>(sb, g.nbins, im - jb[n] + (ci < cj[n] ? L : 0) + off, val, m0, 0.0);
                 }
             }
@@ -540,8 +551,7 @@ __device__ __forceinline__ void bin_split_elems(const f16v (&acc)[WM][WN],
                     if (j >= Ns) continue;
                     int dc = ci - cj[n];
                     dc += dc < 0 ? L : 0;
-                    double val = (double)acc[m][n][v];
-                    if constexpr (DUAL) val += (double)acc2[m][n][v] * (1.0 / 2048.0);
+                    const double val = split_value<WM, WN, DUAL>(acc, acc2, m, n, v);
                     fix_add<1>(sb, g.nbins, (ri - rj[n] - g.drmin) * L + dc, val, m0, 0.0);
                 }
             }
@@ -560,6 +570,92 @@ __device__ __forceinline__ void bin_split_tile(const f16v (&acc)[WM][WN],
     bin_split_elems<WM, WN, DUAL>(acc, acc2, sb, g, i0, j0, r0, c0, BM, BN, Ns, L, m0);
     __syncthreads();
     flush_bins<1>(sb, g, L, gbins, 0);
+}
+
+// The same epilogue over an unwrapped LDS layout, for kernels whose LDS holds twice the
+// tile's bins (at most 2 (BM + BN + 3 L) words, 55 KB at L = 1024): product (i, j) with
+// lattice rows ri, rj and columns ci, cj goes to word (ri - rj - drmin) 2L + L + (ci - cj),
+// with no reduction of ci - cj mod L; the flush adds the words of ci - cj = dc and dc - L,
+// which are the one bin (ri - rj - drmin) L + dc of the wrapped layout.  A wave whose
+// 32 WM rows and 32 WN columns each stay within one lattice row (every wave of the tiles
+// at L = 64, 128, 256, ...) then has ri - rj fixed and ci - cj = const + 32 (m - n) +
+// 8 (v / 4) + v % 4 - lane % 32 + 4 (lane / 32): its word addresses are one per-lane base
+// plus compile-time offsets, and the blocks with equal m - n (same words) are summed in
+// registers first.  Integer words, so the sums do not depend on the order.
+template <int WM, int WN, bool DUAL>
+__device__ __forceinline__ void bin_split_elems_unwrapped(const f16v (&acc)[WM][WN],
+                                                          const f16v (&acc2)[DUAL ? WM : 1][DUAL ? WN : 1],
+                                                          unsigned long long *su, const BinGeom &g, int i0, int j0,
+                                                          int r0, int c0, int BM, int BN, int Ns, int L, double m0)
+{
+    const int lane = threadIdx.x & 63;
+    const int wi0 = i0 + r0, wj0 = j0 + c0;  // the wave's first row and column
+    const int ci0 = wi0 % L, cj0 = wj0 % L;
+    if (i0 + BM <= Ns && j0 + BN <= Ns && ci0 + 32 * WM <= L && cj0 + 32 * WN <= L) {
+        const int base = (wi0 / L - wj0 / L - g.drmin) * 2 * L + L + ci0 - cj0 + 4 * (lane / 32) - lane % 32 -
+                         32 * (WN - 1);  // word of m - n = -(WN - 1), v = 0
+        unsigned long long *sw = su + base;
+#pragma unroll
+        for (int d = -(WN - 1); d <= WM - 1; d++) {  // d = m - n
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                unsigned long long word = 0;
+#pragma unroll
+                for (int m = 0; m < WM; m++) {
+                    const int n = m - d;
+                    if (n < 0 || n >= WN) continue;
+                    const double t = __builtin_fma(split_value<WM, WN, DUAL>(acc, acc2, m, n, v), m0, kMagic);
+                    word += (unsigned long long)__double_as_longlong(t);
+                }
+                atomicAdd(&sw[32 * (d + WN - 1) + 8 * (v / 4) + (v % 4)], word);
+            }
+        }
+    } else {
+        int rj[WN], cj[WN];
+#pragma unroll
+        for (int n = 0; n < WN; n++) {
+            const int j = wj0 + n * 32 + lane % 32;
+            rj[n] = j / L;
+            cj[n] = j - rj[n] * L;
+        }
+#pragma unroll
+        for (int m = 0; m < WM; m++)
+#pragma unroll
+            for (int v = 0; v < 16; v++) {
+                const int i = wi0 + m * 32 + 8 * (v / 4) + 4 * (lane / 32) + (v % 4);
+                if (i >= Ns) continue;
+                const int ri = i / L, ci = i - ri * L;
+#pragma unroll
+                for (int n = 0; n < WN; n++) {
+                    const int j = wj0 + n * 32 + lane % 32;
+                    if (j >= Ns) continue;
+                    const double t = __builtin_fma(split_value<WM, WN, DUAL>(acc, acc2, m, n, v), m0, kMagic);
+                    atomicAdd(&su[(ri - rj[n] - g.drmin) * 2 * L + L + ci - cj[n]],
+                              (unsigned long long)__double_as_longlong(t));
+                }
+            }
+    }
+}
+
+template <int WM, int WN, bool DUAL>
+__device__ __forceinline__ void bin_split_tile_unwrapped(const f16v (&acc)[WM][WN],
+                                                         const f16v (&acc2)[DUAL ? WM : 1][DUAL ? WN : 1],
+                                                         unsigned long long *su, int i0, int j0, int r0, int c0,
+                                                         int BM, int BN, int Ns, int L, unsigned long long *gbins,
+                                                         double m0)
+{
+    const BinGeom g = bin_geom(i0, j0, Ns, L, BM, BN);
+    for (int b = threadIdx.x; b < 2 * g.nbins; b += blockDim.x) su[b] = 0;
+    __syncthreads();
+    bin_split_elems_unwrapped<WM, WN, DUAL>(acc, acc2, su, g, i0, j0, r0, c0, BM, BN, Ns, L, m0);
+    __syncthreads();
+    for (int b = threadIdx.x; b < g.nbins; b += blockDim.x) {
+        const int row = b / L, dc = b - row * L;
+        const unsigned long long v = su[2 * L * row + dc] + su[2 * L * row + L + dc];
+        int dr = (g.drmin + row) % L;
+        if (dr < 0) dr += L;
+        if (v != 0) atomicAdd(&gbins[(long)dr * L + dc], v);
+    }
 }
 
 template <int PREC, int WM, int WN, int WVM = 2, int WVN = 2>
@@ -743,6 +839,8 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
     constexpr int STAGE = 2 * (BM + BN) * BK;   // halves
     typedef typename SplitMfma<PREC>::v_t v_t;
     __shared__ __attribute__((aligned(1024))) uint16_t lds[NS * STAGE];
+    // the epilogue's unwrapped bins (2 (BM + BN + 3 L) words at most, L <= 1024) reuse the ring
+    static_assert(NS * STAGE * 2 >= 2 * (BM + BN + 3 * 1024) * 8, "ring too small for the bins");
 
     const int nti = (args.Ns + BM - 1) / BM, ntj = (args.Ns + BN - 1) / BN;
     const long nslots = num_slots((long)nti * ntj, args.nlags);
@@ -926,9 +1024,9 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
     // every load in flight lands before the bins reuse the LDS
     __builtin_amdgcn_s_waitcnt(wait_vmcnt(0));
     __syncthreads();
-    bin_split_tile<WM, WN, DUAL>(acc, acc2, reinterpret_cast<unsigned long long *>(lds), i0, j0, wi * WM * 32,
-                                 wj * WN * 32, BM, BN, args.Ns, args.L, args.bins + (long)w.lag * args.Ns,
-                                 args.fix->m0);
+    bin_split_tile_unwrapped<WM, WN, DUAL>(acc, acc2, reinterpret_cast<unsigned long long *>(lds), i0, j0,
+                                           wi * WM * 32, wj * WN * 32, BM, BN, args.Ns, args.L,
+                                           args.bins + (long)w.lag * args.Ns, args.fix->m0);
     }
 }
 

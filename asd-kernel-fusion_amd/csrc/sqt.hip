@@ -163,21 +163,24 @@ __global__ void pack_kmajor_kernel(const double *__restrict__ snaps, long tstrid
     out[(long)k * ld + i] = (T)v;
 }
 
-// snapshots -> i-major hi/lo split operands (k contiguous), k = 3 t + c.
+// snapshots -> hi/lo split operands, k = 3 t + c, in k-blocks of 8 halves (16 B):
+// element (i, k) at ((k / 8) ldi + i) 8 + k % 8, so one k-block of consecutive rows is
+// contiguous and a 16-row, 32-deep chunk is four runs of 256 B.
 // BF16X3: hi = bf16(v), lo = bf16(v - hi).  FP16X2: hi = f16(v), lo = f16((v - hi) * 2^11).
-// `nshift` shifted copies are written, copy r holding out_r[i][k] = value(k + r), so a
-// window starting at any k offset o is read from copy o % 8 at the 16-byte aligned
-// offset o - o % 8.  grid: (rows i, k chunks of 256).
+// `nshift` shifted copies are written, copy r holding value(i, k + r) at (i, k), so a
+// window starting at any k offset o is read from copy o % 8 at the k-block (o - o % 8) / 8.
+// grid: (ldi / 32 row groups, nkb k-blocks); thread t: row 32 blockIdx.x + t / 8, k % 8 = t % 8.
 template <int PREC>
-__global__ void pack_split_kernel(const double *__restrict__ snaps, long tstride, int nt, int Ns,
-                                  uint16_t *__restrict__ hi, uint16_t *__restrict__ lo, long ldk,
-                                  long copy_stride, int nshift)
+__global__ __launch_bounds__(256) void pack_split_kernel(const double *__restrict__ snaps, long tstride, int nt,
+                                                         int Ns, uint16_t *__restrict__ hi,
+                                                         uint16_t *__restrict__ lo, long ldi, long copy_stride,
+                                                         int nshift)
 {
-    const int i = blockIdx.x;
-    const long k = (long)blockIdx.y * blockDim.x + threadIdx.x;
-    if (k >= ldk) return;
+    const int i = blockIdx.x * 32 + threadIdx.x / 8, e = threadIdx.x % 8;
+    const long kb = blockIdx.y;
+    const long o = (kb * ldi + i) * 8 + e;
     for (int r = 0; r < nshift; r++) {
-        const long kk = k + r;
+        const long kk = kb * 8 + e + r;
         const long t = kk / 3;
         const int c = (int)(kk - 3 * t);
         const float v = (t < nt && i < Ns) ? (float)snaps[t * tstride + 3L * i + c] : 0.0f;
@@ -195,8 +198,8 @@ __global__ void pack_split_kernel(const double *__restrict__ snaps, long tstride
             h = *reinterpret_cast<uint16_t *>(&fh);
             l = *reinterpret_cast<uint16_t *>(&fl);
         }
-        hi[r * copy_stride + (long)i * ldk + k] = h;
-        lo[r * copy_stride + (long)i * ldk + k] = l;
+        hi[r * copy_stride + o] = h;
+        lo[r * copy_stride + o] = l;
     }
 }
 
@@ -273,7 +276,7 @@ struct GemmArgs {
     const void *B;   // B operand
     const void *A2;  // lo parts (split precisions)
     const void *B2;
-    long ld;         // k-major: row stride in elements; i-major: row stride (k) in elements
+    long ld;         // k-major: row stride in elements; split: rows per k-block of 8 halves
     long copy_stride;  // i-major A: elements between the 8 shifted copies
     int L, Ns, Kpad, nt, nlags;
     int K;           // k extent without the zero padding (3 T0)
@@ -676,18 +679,19 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
     if (!decode_work(slot, nti, ntj, args.nlags, args.group, w)) continue;
     __syncthreads();  // LDS of the previous work item fully consumed
     const int i0 = w.ti * BM, j0 = w.tj * BN;
-    const long ld = args.ld;
+    const long ld = args.ld;  // rows per k-block
     const long koff = 3L * w.lag, shift = koff & 7;
-    const long acopy = shift * args.copy_stride + (long)i0 * ld + (koff - shift);
+    const long acopy = shift * args.copy_stride + ((koff - shift) / 8 * ld + i0) * 8;
     const uint16_t *Ah = static_cast<const uint16_t *>(args.A) + acopy;
     const uint16_t *Al = static_cast<const uint16_t *>(args.A2) + acopy;
-    const uint16_t *Bh = static_cast<const uint16_t *>(args.B) + (long)j0 * ld;
-    const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (long)j0 * ld;
+    const uint16_t *Bh = static_cast<const uint16_t *>(args.B) + (long)j0 * 8;
+    const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (long)j0 * 8;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = wave / WVN, wj = wave % WVN;
-    // global -> LDS: a row of one operand tile is 32 halves = 4 x 16 B; piece e of an
-    // operand is (row e/4, quarter e%4); thread t moves pieces t, t+NT, ...
+    // global -> LDS: a row of one operand tile is 32 halves = 4 x 16 B (k-blocks); piece e of
+    // an operand is (row e % rows, quarter e / rows), so consecutive threads read consecutive
+    // rows of one k-block (contiguous 16 B each); thread t moves pieces t, t+NT, ...
     constexpr int PA = BM * 4 / NT, PB = BN * 4 / NT;  // pieces per thread per array
     static_assert(PA * NT == BM * 4 && PB * NT == BN * 4, "tile rows must split evenly over threads");
 
@@ -715,16 +719,16 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
     u32x4 ra[2][2][PA], rb[2][2][PB];  // [stage][hi, lo][piece]
 #define ASD_SPLIT_GLOAD(KT, ST)                                                          \
     do {                                                                                 \
-        const long k0_ = (long)(KT) * BK;                                                \
+        const long kb_ = (long)(KT) * (BK / 8);                                          \
         _Pragma("unroll") for (int u = 0; u < PA; u++) {                                 \
-            const int e = tid + NT * u, row = e >> 2, qq = e & 3;                        \
-            const long off = (long)row * ld + k0_ + qq * 8;                              \
+            const int e = tid + NT * u, row = e % BM, qq = e / BM;                       \
+            const long off = ((kb_ + qq) * ld + row) * 8;                                \
             ra[ST][0][u] = *reinterpret_cast<const u32x4 *>(Ah + off);                   \
             ra[ST][1][u] = *reinterpret_cast<const u32x4 *>(Al + off);                   \
         }                                                                                \
         _Pragma("unroll") for (int u = 0; u < PB; u++) {                                 \
-            const int e = tid + NT * u, row = e >> 2, qq = e & 3;                        \
-            const long off = (long)row * ld + k0_ + qq * 8;                              \
+            const int e = tid + NT * u, row = e % BN, qq = e / BN;                       \
+            const long off = ((kb_ + qq) * ld + row) * 8;                                \
             rb[ST][0][u] = *reinterpret_cast<const u32x4 *>(Bh + off);                   \
             rb[ST][1][u] = *reinterpret_cast<const u32x4 *>(Bl + off);                   \
         }                                                                                \
@@ -732,12 +736,12 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
 #define ASD_SPLIT_SSTORE(BUF, ST)                                                        \
     do {                                                                                 \
         _Pragma("unroll") for (int u = 0; u < PA; u++) {                                 \
-            const int e = tid + NT * u, row = e >> 2, qq = e & 3;                        \
+            const int e = tid + NT * u, row = e % BM, qq = e / BM;                       \
             *reinterpret_cast<u32x4 *>(&sA[BUF][0][row][qq * 8]) = ra[ST][0][u];         \
             *reinterpret_cast<u32x4 *>(&sA[BUF][1][row][qq * 8]) = ra[ST][1][u];         \
         }                                                                                \
         _Pragma("unroll") for (int u = 0; u < PB; u++) {                                 \
-            const int e = tid + NT * u, row = e >> 2, qq = e & 3;                        \
+            const int e = tid + NT * u, row = e % BN, qq = e / BN;                       \
             *reinterpret_cast<u32x4 *>(&sB[BUF][0][row][qq * 8]) = rb[ST][0][u];         \
             *reinterpret_cast<u32x4 *>(&sB[BUF][1][row][qq * 8]) = rb[ST][1][u];         \
         }                                                                                \
@@ -815,9 +819,10 @@ constexpr int kWaitLgkm0 = 15 | (7 << 4) | (3 << 14);
 // (buffer_load_dwordx4 ... lds: 64 lanes x 16 B into 1 KiB of LDS at a wave-uniform base)
 // through a ring of NS stages, so no staging registers and no LDS stores are needed
 // and every chunk is requested NS - 1 steps before it is used.  A stage holds
-// [A hi][A lo][B hi][B lo], rows of 32 halves (64 B) with no padding; the 16-byte
-// quarter q of row r sits in slot q ^ ((r >> 2) & 3), which spreads any 16
-// consecutive rows read at one quarter over all 64 LDS banks.
+// [A hi][A lo][B hi][B lo] in 1 KiB chunks of 16 rows x 32 halves, each stored as the
+// operand is in HBM: k-block (16-byte quarter) q of row r at (16 q + r) 16 B.  A chunk's
+// load then reads four contiguous 256-byte runs, and a fragment read (32 rows at one
+// quarter) covers 512 contiguous bytes, every LDS bank evenly.
 //
 // Synchronisation per k step (one barrier): wait for this wave's loads of the step's
 // stage (the loads of the NS - 2 later stages may stay in flight) and for its LDS
@@ -849,45 +854,45 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
     // LDS bases of the loads) in scalar registers
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wi = wave / WVN, wj = wave % WVN;
-    // load side: lane l of a chunk fills slot (row l / 4, quarter l % 4), so it fetches
-    // global quarter (l % 4) ^ (l / 16) of that row (rows of a chunk start at a multiple
-    // of 16, so (r >> 2) & 3 is l / 16)
-    const int lrow = lane >> 2, lq = (lane & 3) ^ (lane >> 4);
-    // read side: fragment lane l reads row l % 32 of a 32-row block at quarter
-    // 2 kk + l / 32, found in slot quarter (2 kk + l / 32) ^ ((l >> 2) & 3)
-    const int rrow = lane & 31, rx = (lane >> 2) & 3, rhalf = lane >> 5;
+    // load side: lane l of a chunk fills (quarter l / 16, row l % 16); read side: fragment
+    // lane l reads row l % 32 of a 32-row block at quarter 2 kk + l / 32, at
+    // (rows / 16) 512 + (16 quarter + row % 16) 8 halves from the block
+    const int rrow = lane & 31, rhalf = lane >> 5;
+    const int rd_off = (rrow >> 4) * 512 + (16 * rhalf + (rrow & 15)) * 8;
 
     for (long slot = blockIdx.x; slot < nslots; slot += gridDim.x) {
     Work w;
     if (!decode_work(slot, nti, ntj, args.nlags, args.group, w)) continue;
     __syncthreads();  // LDS (bins) of the previous work item fully consumed
     const int i0 = w.ti * BM, j0 = w.tj * BN;
-    const long ld = args.ld;
+    const long ld = args.ld;  // rows per k-block
     const long koff = 3L * w.lag, shift = koff & 7;
-    const long acopy = shift * args.copy_stride + (long)i0 * ld + (koff - shift);
+    const long acopy = shift * args.copy_stride + ((koff - shift) / 8 * ld + i0) * 8;
     const uint16_t *Ah = static_cast<const uint16_t *>(args.A) + acopy;
     const uint16_t *Al = static_cast<const uint16_t *>(args.A2) + acopy;
-    const uint16_t *Bh = static_cast<const uint16_t *>(args.B) + (long)j0 * ld;
-    const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (long)j0 * ld;
+    const uint16_t *Bh = static_cast<const uint16_t *>(args.B) + (long)j0 * 8;
+    const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (long)j0 * 8;
     // The loads are buffer loads into LDS (MUBUF), not global ones: the compiler then counts
     // them in vmcnt only, and can wait for single LDS fragment reads (behind a pending
-    // global_load_lds it waits for every LDS read in flight).  One resource per operand
-    // array with the tile's first row as its base (the host keeps BM rows of the operand
-    // below 2 GiB); a chunk's source is that base + a wave-uniform row / k offset (SGPR) +
-    // this lane's byte offset (row lrow, quarter lq: the same in every chunk; VGPR).
-    const __amdgpu_buffer_rsrc_t rAh = __builtin_amdgcn_make_buffer_rsrc((void *)Ah, 0, 0x7fffffff, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rAl = __builtin_amdgcn_make_buffer_rsrc((void *)Al, 0, 0x7fffffff, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rBh = __builtin_amdgcn_make_buffer_rsrc((void *)Bh, 0, 0x7fffffff, 0x00020000);
-    const __amdgpu_buffer_rsrc_t rBl = __builtin_amdgcn_make_buffer_rsrc((void *)Bl, 0, 0x7fffffff, 0x00020000);
-    const int lane_off = (int)(2 * (lrow * ld + 8 * lq));
+    // global_load_lds it waits for every LDS read in flight).  Per k step, one resource per
+    // operand array based at the step's first k-block of the tile's first row (64-bit
+    // address arithmetic, scalar); a chunk's source is that base + the chunk's row offset
+    // (SGPR) + this lane's (quarter, row) offset (the same in every chunk; VGPR, below
+    // 4 ld 16 B).
+    const int lane_off = (int)(((lane >> 4) * ld + (lane & 15)) * 16);
+    const long step_bytes = 4 * ld * 16;  // one 32-deep step: four k-blocks
 
     auto issue = [&](int kt, int st) {
-        const int k0 = 2 * kt * BK;  // bytes
+        const long kb = (long)kt * step_bytes;
+        auto rsrc = [&](const uint16_t *p) {
+            return __builtin_amdgcn_make_buffer_rsrc((void *)((const char *)p + kb), 0, 0x7fffffff, 0x00020000);
+        };
+        const __amdgpu_buffer_rsrc_t rAh = rsrc(Ah), rAl = rsrc(Al), rBh = rsrc(Bh), rBl = rsrc(Bl);
         uint16_t *base = lds + st * STAGE;
 #pragma unroll
         for (int u = 0; u < CA; u++) {
             const int c = (wave * CA + u) * 512;
-            const int so = (int)(2 * 16 * (wave * CA + u) * ld) + k0;
+            const int so = 256 * (wave * CA + u);  // 16 rows of 16 B
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rAh, (__attribute__((address_space(3))) void *)(base + c), 16,
                                                      lane_off, so, 0, 0);
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rAl, (__attribute__((address_space(3))) void *)(base + OFF_AL + c),
@@ -896,7 +901,7 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
 #pragma unroll
         for (int u = 0; u < CB; u++) {
             const int c = (wave * CB + u) * 512;
-            const int so = (int)(2 * 16 * (wave * CB + u) * ld) + k0;
+            const int so = 256 * (wave * CB + u);
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rBh, (__attribute__((address_space(3))) void *)(base + OFF_BH + c),
                                                      16, lane_off, so, 0, 0);
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rBl, (__attribute__((address_space(3))) void *)(base + OFF_BL + c),
@@ -952,21 +957,22 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
         // reads in the order the MFMAs use them: per 16-deep half A hi, B hi (the hh
         // products), B lo (hl), A lo (lh)
         v_t ah[KH][WM], al[KH][WM], bh[KH][WN], bl[KH][WN];
+        const uint16_t *sr = sb + rd_off;
 #pragma unroll
         for (int kk = 0; kk < KH; kk++) {
-            const int qo = 8 * ((2 * kk + rhalf) ^ rx);
+            const int qo = 2 * kk * 16 * 8;  // quarter 2 kk (+ l / 32 in rd_off)
 #pragma unroll
             for (int m = 0; m < WM; m++)
-                ah[kk][m] = *reinterpret_cast<const v_t *>(sb + (wi * WM * 32 + m * 32 + rrow) * BK + qo);
+                ah[kk][m] = *reinterpret_cast<const v_t *>(sr + (wi * WM + m) * 1024 + qo);
 #pragma unroll
             for (int n = 0; n < WN; n++)
-                bh[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BH + (wj * WN * 32 + n * 32 + rrow) * BK + qo);
+                bh[kk][n] = *reinterpret_cast<const v_t *>(sr + OFF_BH + (wj * WN + n) * 1024 + qo);
 #pragma unroll
             for (int n = 0; n < WN; n++)
-                bl[kk][n] = *reinterpret_cast<const v_t *>(sb + OFF_BL + (wj * WN * 32 + n * 32 + rrow) * BK + qo);
+                bl[kk][n] = *reinterpret_cast<const v_t *>(sr + OFF_BL + (wj * WN + n) * 1024 + qo);
 #pragma unroll
             for (int m = 0; m < WM; m++)
-                al[kk][m] = *reinterpret_cast<const v_t *>(sb + OFF_AL + (wi * WM * 32 + m * 32 + rrow) * BK + qo);
+                al[kk][m] = *reinterpret_cast<const v_t *>(sr + OFF_AL + (wi * WM + m) * 1024 + qo);
         }
         // per half: every hh product, then every hl, then every lh (each accumulator still
         // takes hh, hl, lh of a half in that order), so the first MFMAs wait only for the
@@ -1438,26 +1444,26 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         hipLaunchKernelGGL(k, dim3(nblocks), dim3(256), 0, st, ga);
     } else {
         const long Kpad = round_up(K, 64);  // an even number of 32-wide k steps
-        const long ldk = round_up(3L * (nlags - 1) + Kpad + 8, 8);  // halves per i-row
-        // the direct-to-LDS kernels address a tile's operand rows through 32-bit byte offsets
-        ASD_CHECK(2L * 256 * ldk + 2L * Kpad < (1L << 31), "too many lags for the split GEMM (%d)", nlags);
+        const long ldk = round_up(3L * (nlags - 1) + Kpad + 8, 8);  // k extent (halves) of the A copies
+        const long nkb = ldk / 8;                                    // k-blocks of 16 B
+        ASD_CHECK(nkb <= 65535, "too many lags for the split GEMM (%d)", nlags);
         const long cs = ldk * ldi;                                   // one shifted copy
         ASD_TRY(ctx->opA.ensure(2 * 8 * cs));
         ASD_TRY(ctx->opA2.ensure(2 * 8 * cs));
         ASD_TRY(ctx->opB.ensure(2 * cs));
         ASD_TRY(ctx->opB2.ensure(2 * cs));
-        const dim3 pg((unsigned)ldi, (unsigned)((ldk + 255) / 256)), pb(256);
+        const dim3 pb(256);
         auto packer = prec == ASD_BF16X3 ? pack_split_kernel<ASD_BF16X3> : pack_split_kernel<ASD_FP16X2>;
-        hipLaunchKernelGGL(packer, pg, pb, 0, st, d_snaps_a, tstride, nt_a, Ns, ctx->opA.as<uint16_t>(),
-                           ctx->opA2.as<uint16_t>(), ldk, cs, 8);
-        hipLaunchKernelGGL(packer, pg, pb, 0, st, d_snaps_b, tstride, T0, Ns, ctx->opB.as<uint16_t>(),
-                           ctx->opB2.as<uint16_t>(), ldk, cs, 1);
+        hipLaunchKernelGGL(packer, dim3((unsigned)(ldi / 32), (unsigned)nkb), pb, 0, st, d_snaps_a, tstride, nt_a,
+                           Ns, ctx->opA.as<uint16_t>(), ctx->opA2.as<uint16_t>(), ldi, cs, 8);
+        hipLaunchKernelGGL(packer, dim3((unsigned)(ldi / 32), (unsigned)(Kpad / 8)), pb, 0, st, d_snaps_b, tstride,
+                           T0, Ns, ctx->opB.as<uint16_t>(), ctx->opB2.as<uint16_t>(), ldi, cs, 1);
         ga.copy_stride = cs;
         ga.A = ctx->opA.ptr;
         ga.A2 = ctx->opA2.ptr;
         ga.B = ctx->opB.ptr;
         ga.B2 = ctx->opB2.ptr;
-        ga.ld = ldk;
+        ga.ld = ldi;  // rows per k-block
         ga.Kpad = (int)Kpad;
         ASD_TRY(mark(ctx, 1));
         if (prec == ASD_BF16X3)

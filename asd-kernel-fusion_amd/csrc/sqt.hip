@@ -163,13 +163,18 @@ __global__ void pack_kmajor_kernel(const double *__restrict__ snaps, long tstrid
     out[(long)k * ld + i] = (T)v;
 }
 
+// unused rows after each k-block of the split operands (see the host code)
+constexpr long kKbPadRows = 16;
+
 // snapshots -> hi/lo split operands, k = 3 t + c, in k-blocks of 8 halves (16 B):
 // element (i, k) at ((k / 8) ldi + i) 8 + k % 8, so one k-block of consecutive rows is
 // contiguous and a 16-row, 32-deep chunk is four runs of 256 B.
 // BF16X3: hi = bf16(v), lo = bf16(v - hi).  FP16X2: hi = f16(v), lo = f16((v - hi) * 2^11).
 // `nshift` shifted copies are written, copy r holding value(i, k + r) at (i, k), so a
 // window starting at any k offset o is read from copy o % 8 at the k-block (o - o % 8) / 8.
-// grid: (ldi / 32 row groups, nkb k-blocks); thread t: row 32 blockIdx.x + t / 8, k % 8 = t % 8.
+// ldi: rows per k-block (rows >= Ns up to the grid's extent are zeros; rows beyond it are
+// never read).  grid: (row groups of 32, k-blocks); thread t: row 32 blockIdx.x + t / 8,
+// k % 8 = t % 8.
 template <int PREC>
 __global__ __launch_bounds__(256) void pack_split_kernel(const double *__restrict__ snaps, long tstride, int nt,
                                                          int Ns, uint16_t *__restrict__ hi,
@@ -1447,7 +1452,11 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         const long ldk = round_up(3L * (nlags - 1) + Kpad + 8, 8);  // k extent (halves) of the A copies
         const long nkb = ldk / 8;                                    // k-blocks of 16 B
         ASD_CHECK(nkb <= 65535, "too many lags for the split GEMM (%d)", nlags);
-        const long cs = ldk * ldi;                                   // one shifted copy
+        // rows per k-block: the padded site count plus kKbPadRows unused rows, so that
+        // consecutive k-blocks of one row are not a power of two apart (at Ns = 2^14, 256 KiB:
+        // the four k-blocks of a chunk then land in the same L2 sets)
+        const long ldb = ldi + kKbPadRows;
+        const long cs = ldk * ldb;                                   // one shifted copy
         ASD_TRY(ctx->opA.ensure(2 * 8 * cs));
         ASD_TRY(ctx->opA2.ensure(2 * 8 * cs));
         ASD_TRY(ctx->opB.ensure(2 * cs));
@@ -1455,15 +1464,15 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         const dim3 pb(256);
         auto packer = prec == ASD_BF16X3 ? pack_split_kernel<ASD_BF16X3> : pack_split_kernel<ASD_FP16X2>;
         hipLaunchKernelGGL(packer, dim3((unsigned)(ldi / 32), (unsigned)nkb), pb, 0, st, d_snaps_a, tstride, nt_a,
-                           Ns, ctx->opA.as<uint16_t>(), ctx->opA2.as<uint16_t>(), ldi, cs, 8);
+                           Ns, ctx->opA.as<uint16_t>(), ctx->opA2.as<uint16_t>(), ldb, cs, 8);
         hipLaunchKernelGGL(packer, dim3((unsigned)(ldi / 32), (unsigned)(Kpad / 8)), pb, 0, st, d_snaps_b, tstride,
-                           T0, Ns, ctx->opB.as<uint16_t>(), ctx->opB2.as<uint16_t>(), ldi, cs, 1);
+                           T0, Ns, ctx->opB.as<uint16_t>(), ctx->opB2.as<uint16_t>(), ldb, cs, 1);
         ga.copy_stride = cs;
         ga.A = ctx->opA.ptr;
         ga.A2 = ctx->opA2.ptr;
         ga.B = ctx->opB.ptr;
         ga.B2 = ctx->opB2.ptr;
-        ga.ld = ldi;  // rows per k-block
+        ga.ld = ldb;  // rows per k-block
         ga.Kpad = (int)Kpad;
         ASD_TRY(mark(ctx, 1));
         if (prec == ASD_BF16X3)

@@ -419,6 +419,7 @@ STREAM2_CASES = [
     (64, 256, dict(D=0.1, alpha=0.05)),                                # J == 1, damped
     (64, 256, dict(J=0.8, D=0.1, Bz=0.02)),                            # alpha == 0
     (1024, 8, dict(D=0.1)),                                            # the large lattice
+    (64, 1100, dict(D=0.1, Bz=0.05)),                                  # shares of several columns
 ]
 
 

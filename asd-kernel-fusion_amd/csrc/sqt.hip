@@ -401,6 +401,76 @@ __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
     const BinGeom g = bin_geom(i0, j0, Ns, L);
     const double m0 = args.fix->m0, m1 = args.fix->m1;
     unsigned long long *sb = reinterpret_cast<unsigned long long *>(smem_raw);
+    // FP64: unwrapped bins where the LDS holds them (L <= 512; the split kernels' layout:
+    // word (ri - rj - drmin) 2L + L + ci - cj, the flush adds the words of dc and dc - L):
+    // a wave whose 64 rows and 64 columns each stay within one lattice row adds its products
+    // at one per-lane base plus compile-time offsets, the 4 - |m - n| blocks of one
+    // displacement diagonal d = m - n summed in registers first (28 words per lane and word
+    // plane instead of 64).  FP64 at L = 128: 3852 -> 3814 ms (K = 300: 262 -> 255 ms); the
+    // FP32 kernel measured 4.7% slower with it at K = 1200 (123 VGPRs, a different
+    // occupancy) and keeps the wrapped bins.
+    if (std::is_same<T, double>::value && 2L * NW * g.nbins * 8 <= (long)sizeof(smem_raw)) {
+        const int UP = 2 * g.nbins;  // words per word plane
+        for (int b = tid; b < NW * UP; b += 256) sb[b] = 0;
+        __syncthreads();
+        const int wi0 = i0 + wi * 64, wj0 = j0 + wj * 64, ciw = wi0 % L, cjw = wj0 % L;
+        if (i0 + TILE <= Ns && j0 + TILE <= Ns && ciw + 64 <= L && cjw + 64 <= L) {
+            // lane l holds rows 16 m + KMajorMfma<T>::row(l, v) and columns 16 n + l % 16 of the
+            // wave's 64 x 64 block; row(l, v) = l / 16 + 4 v (FP64) or 4 (l / 16) + v (FP32)
+            const int base = (wi0 / L - wj0 / L - g.drmin) * 2 * L + L + ciw - cjw - lane % 16 - 16 * 3;
+            unsigned long long *sw = sb + base;
+#pragma unroll
+            for (int d = -3; d <= 3; d++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    unsigned long long w0 = 0, w1 = 0;
+#pragma unroll
+                    for (int m = 0; m < 4; m++) {
+                        const int n = m - d;
+                        if (n < 0 || n > 3) continue;
+                        const double x = (double)acc[m][n][v];
+                        const double t = __builtin_fma(x, m0, kMagic);
+                        w0 += (unsigned long long)__double_as_longlong(t);
+                        if constexpr (NW == 2) {
+                            const double r = __builtin_fma(x, m0, kMagic - t);  // exact residual
+                            w1 += (unsigned long long)__double_as_longlong(__builtin_fma(r, m1, kMagic));
+                        }
+                    }
+                    const int o = 16 * (d + 3) + KMajorMfma<T>::row(lane, v);
+                    atomicAdd(&sw[o], w0);
+                    if constexpr (NW == 2) atomicAdd(&sw[UP + o], w1);
+                }
+        } else {
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+                for (int v = 0; v < 4; v++) {
+                    const int i = wi0 + m * 16 + KMajorMfma<T>::row(lane, v);
+                    if (i >= Ns) continue;
+                    const int ri = i / L, ci = i - ri * L;
+#pragma unroll
+                    for (int n = 0; n < 4; n++) {
+                        const int j = wj0 + n * 16 + lane % 16;
+                        if (j >= Ns) continue;
+                        const int rj = j / L, cj = j - rj * L;
+                        fix_add<NW>(sb, UP, (ri - rj - g.drmin) * 2 * L + L + ci - cj, (double)acc[m][n][v], m0, m1);
+                    }
+                }
+        }
+        __syncthreads();
+        for (int b = tid; b < g.nbins; b += 256) {
+            const int row = b / L, dc = b - row * L;
+            int dr = (g.drmin + row) % L;
+            if (dr < 0) dr += L;
+            const long dst = (long)dr * L + dc;
+#pragma unroll
+            for (int wd = 0; wd < NW; wd++) {
+                const unsigned long long v = sb[wd * UP + 2 * L * row + dc] + sb[wd * UP + 2 * L * row + L + dc];
+                if (v != 0) atomicAdd(&args.bins[(long)w.lag * Ns + wd * args.wstride + dst], v);
+            }
+        }
+        continue;
+    }
     for (int b = tid; b < NW * g.nbins; b += 256) sb[b] = 0;
     int rj[4], cj[4];
 #pragma unroll

@@ -170,6 +170,26 @@ def test_large_work_list_all_lags_computed(gctx, orc, prec, tol):
     assert err.max() <= tol
 
 
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,prec,tol", [(600, "fp64", 1e-10), (1024, "fp64", 1e-10), (600, "fp32", 1e-5),
+                                        (1024, "fp16x2", 1e-5), (600, "bf16x3", 2e-4)])
+def test_large_lattices_match_separable(gctx, orc, L, prec, tol):
+    """Lattices past L = 512, where the FP64 kernel's LDS no longer holds a tile's unwrapped
+    bins (its wrapped bins take over), with L = 600 not a multiple of any tile edge (waves
+    whose rows cross a lattice row) and L = 1024, the largest lattice; every lag against the
+    separable oracle."""
+    torch = need_gpu()
+    T0, nlags = 4, 2
+    say(f"sqt {prec} L={L} T0={T0} nlags={nlags}")
+    snaps = np.stack([orc.init_spins(2000 + t, L * L) for t in range(T0 + nlags - 1)])
+    q = [(0, 0), (1, 0), (3, 5), (L // 2, 0), (L // 2, L // 2), (17, L - 3), (L - 1, 1)]
+    ref = orc.sqt_separable(L, snaps, T0, nlags, q)
+    got = gctx.sqt(L, torch.tensor(snaps, device="cuda"), T0, nlags, q, prec)
+    err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+    say(f"  per-lag max rel err {err}")
+    assert err.max() <= tol
+
+
 # Oracle bounds at these small samples (5 to 22 time origins), relative to
 # max |S| over all (q, tau).  BF16X3 keeps about 17 significant bits per operand, so
 # with few origins to average over its error reaches 3e-5 to 1.2e-4 (5 origins); a

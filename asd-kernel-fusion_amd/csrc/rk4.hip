@@ -394,6 +394,27 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
     }
 }
 
+// The tile of a block, XCD-aware: blocks are dispatched round-robin over the 8 XCDs (block b
+// to XCD b % 8), each with its own L2, and a tile's halo comes from its neighbours' previous
+// outputs.  Where the tile grid splits into 8 patches (gx even, gy a multiple of 4), XCD x
+// takes patch x (2 x 4 patches of gx / 2 x gy / 4 tiles), so most halo sites were written
+// through the same L2; otherwise block (x, y) takes tile (y, x).  A bijection either way.
+// Used for tiles of 16 x 16 and up (L = 256 / 512 single lattices 2.6% / 5% faster, 16
+// replicas of L = 64 4.4%); the 8 x 8 tiles at L = 128 measured 2.7% slower with it.
+template <bool XCD>
+__device__ __forceinline__ void xcd_tile(int &tr, int &tc)
+{
+    const int gx = gridDim.x, gy = gridDim.y;
+    if (!XCD || gx % 2 || gy % 4) {
+        tr = blockIdx.y;
+        tc = blockIdx.x;
+        return;
+    }
+    const int lin = blockIdx.x + gx * blockIdx.y, x = lin % 8, m = lin / 8, pw = gx / 2, ph = gy / 4;
+    tr = (x / 2) * ph + m / pw;
+    tc = (x % 2) * pw + m % pw;
+}
+
 // single lattice (nrep replicas along blockIdx.z); snapshot slot from *snap_counter
 // when given (graph replay), else slot 0 of snap
 template <int TR, int TC, bool UJ, bool UD, int NT = TR * TC, int NS = 1, bool FAST = false>
@@ -408,8 +429,9 @@ __global__ __launch_bounds__(NT) void rk4_step_kernel(LLGParams p, int L,
     const long base = (long)blockIdx.z * L * L * 3;
     double *sn = nullptr;
     if (snap) sn = snap + (snap_counter ? (long)*snap_counter : 0) * snap_stride + base;
-    rk4_tile<TR, TC, UJ, UD, NT, NS, FAST>(sm, p, L, blockIdx.y, blockIdx.x, in + base, out + base, sn, gd,
-                                           blockIdx.z);
+    int tr, tc;
+    xcd_tile<(TR >= 16)>(tr, tc);
+    rk4_tile<TR, TC, UJ, UD, NT, NS, FAST>(sm, p, L, tr, tc, in + base, out + base, sn, gd, blockIdx.z);
 }
 
 // ------------------------------------------------------------------ batching

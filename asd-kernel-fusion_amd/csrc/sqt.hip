@@ -163,27 +163,24 @@ __global__ void pack_kmajor_kernel(const double *__restrict__ snaps, long tstrid
     out[(long)k * ld + i] = (T)v;
 }
 
-// unused rows after each k-block of the split operands (see the host code)
-constexpr long kKbPadRows = 16;
-
-// snapshots -> hi/lo split operands, k = 3 t + c, in k-blocks of 8 halves (16 B):
-// element (i, k) at ((k / 8) ldi + i) 8 + k % 8, so one k-block of consecutive rows is
-// contiguous and a 16-row, 32-deep chunk is four runs of 256 B.
+// snapshots -> hi/lo split operands, k = 3 t + c, in row groups of 16 rows and k-blocks of
+// 8 halves (16 B): element (i, k) at ((i / 16) nkb + k / 8) 128 + (i % 16) 8 + k % 8 halves,
+// so the 16 rows of one k-block are 256 contiguous bytes and a 16-row, 32-deep chunk (four
+// consecutive k-blocks) is one contiguous KiB from any k-block.
 // BF16X3: hi = bf16(v), lo = bf16(v - hi).  FP16X2: hi = f16(v), lo = f16((v - hi) * 2^11).
 // `nshift` shifted copies are written, copy r holding value(i, k + r) at (i, k), so a
 // window starting at any k offset o is read from copy o % 8 at the k-block (o - o % 8) / 8.
-// ldi: rows per k-block (rows >= Ns up to the grid's extent are zeros; rows beyond it are
-// never read).  grid: (row groups of 32, k-blocks); thread t: row 32 blockIdx.x + t / 8,
-// k % 8 = t % 8.
+// nkb: k-blocks per row group (rows >= Ns are zeros).  grid: (row groups of 32 rows,
+// k-blocks); thread t: row 32 blockIdx.x + t / 8, k % 8 = t % 8.
 template <int PREC>
 __global__ __launch_bounds__(256) void pack_split_kernel(const double *__restrict__ snaps, long tstride, int nt,
                                                          int Ns, uint16_t *__restrict__ hi,
-                                                         uint16_t *__restrict__ lo, long ldi, long copy_stride,
+                                                         uint16_t *__restrict__ lo, long nkb, long copy_stride,
                                                          int nshift)
 {
     const int i = blockIdx.x * 32 + threadIdx.x / 8, e = threadIdx.x % 8;
     const long kb = blockIdx.y;
-    const long o = (kb * ldi + i) * 8 + e;
+    const long o = ((long)(i / 16) * nkb + kb) * 128 + (i % 16) * 8 + e;
     for (int r = 0; r < nshift; r++) {
         const long kk = kb * 8 + e + r;
         const long t = kk / 3;
@@ -281,7 +278,7 @@ struct GemmArgs {
     const void *B;   // B operand
     const void *A2;  // lo parts (split precisions)
     const void *B2;
-    long ld;         // k-major: row stride in elements; split: rows per k-block of 8 halves
+    long ld;         // k-major: row stride in elements; split: k-blocks per row group
     long copy_stride;  // i-major A: elements between the 8 shifted copies
     int L, Ns, Kpad, nt, nlags;
     int K;           // k extent without the zero padding (3 T0)
@@ -754,19 +751,19 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
     if (!decode_work(slot, nti, ntj, args.nlags, args.group, w)) continue;
     __syncthreads();  // LDS of the previous work item fully consumed
     const int i0 = w.ti * BM, j0 = w.tj * BN;
-    const long ld = args.ld;  // rows per k-block
+    const long ld = args.ld;  // k-blocks per row group
     const long koff = 3L * w.lag, shift = koff & 7;
-    const long acopy = shift * args.copy_stride + ((koff - shift) / 8 * ld + i0) * 8;
+    const long acopy = shift * args.copy_stride + ((i0 / 16) * ld + (koff - shift) / 8) * 128;
     const uint16_t *Ah = static_cast<const uint16_t *>(args.A) + acopy;
     const uint16_t *Al = static_cast<const uint16_t *>(args.A2) + acopy;
-    const uint16_t *Bh = static_cast<const uint16_t *>(args.B) + (long)j0 * 8;
-    const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (long)j0 * 8;
+    const uint16_t *Bh = static_cast<const uint16_t *>(args.B) + (j0 / 16) * ld * 128;
+    const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (j0 / 16) * ld * 128;
 
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wi = wave / WVN, wj = wave % WVN;
     // global -> LDS: a row of one operand tile is 32 halves = 4 x 16 B (k-blocks); piece e of
-    // an operand is (row e % rows, quarter e / rows), so consecutive threads read consecutive
-    // rows of one k-block (contiguous 16 B each); thread t moves pieces t, t+NT, ...
+    // an operand is (row 16 (e / 64) + e % 16, quarter (e / 16) % 4), so consecutive threads
+    // read consecutive 16 B of a row group's contiguous KiB; thread t moves pieces t, t+NT, ...
     constexpr int PA = BM * 4 / NT, PB = BN * 4 / NT;  // pieces per thread per array
     static_assert(PA * NT == BM * 4 && PB * NT == BN * 4, "tile rows must split evenly over threads");
 
@@ -796,14 +793,14 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
     do {                                                                                 \
         const long kb_ = (long)(KT) * (BK / 8);                                          \
         _Pragma("unroll") for (int u = 0; u < PA; u++) {                                 \
-            const int e = tid + NT * u, row = e % BM, qq = e / BM;                       \
-            const long off = ((kb_ + qq) * ld + row) * 8;                                \
+            const int e = tid + NT * u, row = 16 * (e >> 6) + (e & 15), qq = (e >> 4) & 3; \
+            const long off = ((e >> 6) * ld + kb_ + qq) * 128 + (e & 15) * 8;           \
             ra[ST][0][u] = *reinterpret_cast<const u32x4 *>(Ah + off);                   \
             ra[ST][1][u] = *reinterpret_cast<const u32x4 *>(Al + off);                   \
         }                                                                                \
         _Pragma("unroll") for (int u = 0; u < PB; u++) {                                 \
-            const int e = tid + NT * u, row = e % BN, qq = e / BN;                       \
-            const long off = ((kb_ + qq) * ld + row) * 8;                                \
+            const int e = tid + NT * u, row = 16 * (e >> 6) + (e & 15), qq = (e >> 4) & 3; \
+            const long off = ((e >> 6) * ld + kb_ + qq) * 128 + (e & 15) * 8;           \
             rb[ST][0][u] = *reinterpret_cast<const u32x4 *>(Bh + off);                   \
             rb[ST][1][u] = *reinterpret_cast<const u32x4 *>(Bl + off);                   \
         }                                                                                \
@@ -811,12 +808,12 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
 #define ASD_SPLIT_SSTORE(BUF, ST)                                                        \
     do {                                                                                 \
         _Pragma("unroll") for (int u = 0; u < PA; u++) {                                 \
-            const int e = tid + NT * u, row = e % BM, qq = e / BM;                       \
+            const int e = tid + NT * u, row = 16 * (e >> 6) + (e & 15), qq = (e >> 4) & 3; \
             *reinterpret_cast<u32x4 *>(&sA[BUF][0][row][qq * 8]) = ra[ST][0][u];         \
             *reinterpret_cast<u32x4 *>(&sA[BUF][1][row][qq * 8]) = ra[ST][1][u];         \
         }                                                                                \
         _Pragma("unroll") for (int u = 0; u < PB; u++) {                                 \
-            const int e = tid + NT * u, row = e % BN, qq = e / BN;                       \
+            const int e = tid + NT * u, row = 16 * (e >> 6) + (e & 15), qq = (e >> 4) & 3; \
             *reinterpret_cast<u32x4 *>(&sB[BUF][0][row][qq * 8]) = rb[ST][0][u];         \
             *reinterpret_cast<u32x4 *>(&sB[BUF][1][row][qq * 8]) = rb[ST][1][u];         \
         }                                                                                \
@@ -940,22 +937,22 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
     if (!decode_work(slot, nti, ntj, args.nlags, args.group, w)) continue;
     __syncthreads();  // LDS (bins) of the previous work item fully consumed
     const int i0 = w.ti * BM, j0 = w.tj * BN;
-    const long ld = args.ld;  // rows per k-block
+    const long ld = args.ld;  // k-blocks per row group
     const long koff = 3L * w.lag, shift = koff & 7;
-    const long acopy = shift * args.copy_stride + ((koff - shift) / 8 * ld + i0) * 8;
+    const long acopy = shift * args.copy_stride + ((i0 / 16) * ld + (koff - shift) / 8) * 128;
     const uint16_t *Ah = static_cast<const uint16_t *>(args.A) + acopy;
     const uint16_t *Al = static_cast<const uint16_t *>(args.A2) + acopy;
-    const uint16_t *Bh = static_cast<const uint16_t *>(args.B) + (long)j0 * 8;
-    const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (long)j0 * 8;
+    const uint16_t *Bh = static_cast<const uint16_t *>(args.B) + (j0 / 16) * ld * 128;
+    const uint16_t *Bl = static_cast<const uint16_t *>(args.B2) + (j0 / 16) * ld * 128;
     // The loads are buffer loads into LDS (MUBUF), not global ones: the compiler then counts
     // them in vmcnt only, and can wait for single LDS fragment reads (behind a pending
     // global_load_lds it waits for every LDS read in flight).  Per k step, one resource per
-    // operand array based at the step's first k-block of the tile's first row (64-bit
-    // address arithmetic, scalar); a chunk's source is that base + the chunk's row offset
-    // (SGPR) + this lane's (quarter, row) offset (the same in every chunk; VGPR, below
-    // 4 ld 16 B).
-    const int lane_off = (int)(((lane >> 4) * ld + (lane & 15)) * 16);
-    const long step_bytes = 4 * ld * 16;  // one 32-deep step: four k-blocks
+    // operand array based at the step's first k-block of the tile's first row group (64-bit
+    // address arithmetic, scalar); a chunk's source is that base + the chunk's row-group
+    // offset (SGPR, below 16 ld 256 B) + 16 B per lane: one contiguous KiB, lane l filling
+    // (quarter l / 16, row l % 16).
+    const int lane_off = 16 * lane;
+    const long step_bytes = 4 * 256;  // one 32-deep step: four k-blocks of a row group
 
     auto issue = [&](int kt, int st) {
         const long kb = (long)kt * step_bytes;
@@ -967,7 +964,7 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
 #pragma unroll
         for (int u = 0; u < CA; u++) {
             const int c = (wave * CA + u) * 512;
-            const int so = 256 * (wave * CA + u);  // 16 rows of 16 B
+            const int so = (int)((wave * CA + u) * ld * 256);  // the chunk's row group
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rAh, (__attribute__((address_space(3))) void *)(base + c), 16,
                                                      lane_off, so, 0, 0);
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rAl, (__attribute__((address_space(3))) void *)(base + OFF_AL + c),
@@ -976,7 +973,7 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
 #pragma unroll
         for (int u = 0; u < CB; u++) {
             const int c = (wave * CB + u) * 512;
-            const int so = 256 * (wave * CB + u);
+            const int so = (int)((wave * CB + u) * ld * 256);
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rBh, (__attribute__((address_space(3))) void *)(base + OFF_BH + c),
                                                      16, lane_off, so, 0, 0);
             __builtin_amdgcn_raw_ptr_buffer_load_lds(rBl, (__attribute__((address_space(3))) void *)(base + OFF_BL + c),
@@ -1522,11 +1519,9 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         const long ldk = round_up(3L * (nlags - 1) + Kpad + 8, 8);  // k extent (halves) of the A copies
         const long nkb = ldk / 8;                                    // k-blocks of 16 B
         ASD_CHECK(nkb <= 65535, "too many lags for the split GEMM (%d)", nlags);
-        // rows per k-block: the padded site count plus kKbPadRows unused rows, so that
-        // consecutive k-blocks of one row are not a power of two apart (at Ns = 2^14, 256 KiB:
-        // the four k-blocks of a chunk then land in the same L2 sets)
-        const long ldb = ldi + kKbPadRows;
-        const long cs = ldk * ldb;                                   // one shifted copy
+        // k-blocks per row group, odd so that row groups are not a power of two apart
+        const long nkbs = nkb | 1;
+        const long cs = nkbs * 8 * ldi;                              // one shifted copy
         ASD_TRY(ctx->opA.ensure(2 * 8 * cs));
         ASD_TRY(ctx->opA2.ensure(2 * 8 * cs));
         ASD_TRY(ctx->opB.ensure(2 * cs));
@@ -1534,15 +1529,15 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         const dim3 pb(256);
         auto packer = prec == ASD_BF16X3 ? pack_split_kernel<ASD_BF16X3> : pack_split_kernel<ASD_FP16X2>;
         hipLaunchKernelGGL(packer, dim3((unsigned)(ldi / 32), (unsigned)nkb), pb, 0, st, d_snaps_a, tstride, nt_a,
-                           Ns, ctx->opA.as<uint16_t>(), ctx->opA2.as<uint16_t>(), ldb, cs, 8);
+                           Ns, ctx->opA.as<uint16_t>(), ctx->opA2.as<uint16_t>(), nkbs, cs, 8);
         hipLaunchKernelGGL(packer, dim3((unsigned)(ldi / 32), (unsigned)(Kpad / 8)), pb, 0, st, d_snaps_b, tstride,
-                           T0, Ns, ctx->opB.as<uint16_t>(), ctx->opB2.as<uint16_t>(), ldb, cs, 1);
+                           T0, Ns, ctx->opB.as<uint16_t>(), ctx->opB2.as<uint16_t>(), nkbs, cs, 1);
         ga.copy_stride = cs;
         ga.A = ctx->opA.ptr;
         ga.A2 = ctx->opA2.ptr;
         ga.B = ctx->opB.ptr;
         ga.B2 = ctx->opB2.ptr;
-        ga.ld = ldb;  // rows per k-block
+        ga.ld = nkbs;  // k-blocks per row group
         ga.Kpad = (int)Kpad;
         ASD_TRY(mark(ctx, 1));
         if (prec == ASD_BF16X3)

@@ -793,13 +793,13 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
     do {                                                                                 \
         const long kb_ = (long)(KT) * (BK / 8);                                          \
         _Pragma("unroll") for (int u = 0; u < PA; u++) {                                 \
-            const int e = tid + NT * u, row = 16 * (e >> 6) + (e & 15), qq = (e >> 4) & 3; \
+            const int e = tid + NT * u, qq = (e >> 4) & 3;                                 \
             const long off = ((e >> 6) * ld + kb_ + qq) * 128 + (e & 15) * 8;           \
             ra[ST][0][u] = *reinterpret_cast<const u32x4 *>(Ah + off);                   \
             ra[ST][1][u] = *reinterpret_cast<const u32x4 *>(Al + off);                   \
         }                                                                                \
         _Pragma("unroll") for (int u = 0; u < PB; u++) {                                 \
-            const int e = tid + NT * u, row = 16 * (e >> 6) + (e & 15), qq = (e >> 4) & 3; \
+            const int e = tid + NT * u, qq = (e >> 4) & 3;                                 \
             const long off = ((e >> 6) * ld + kb_ + qq) * 128 + (e & 15) * 8;           \
             rb[ST][0][u] = *reinterpret_cast<const u32x4 *>(Bh + off);                   \
             rb[ST][1][u] = *reinterpret_cast<const u32x4 *>(Bl + off);                   \

@@ -610,10 +610,12 @@ static int check_batch(int ntraj, const int *Ls, const asd_params *params, long 
     return ASD_OK;
 }
 
-static std::vector<int> batch_key(int ntraj, const int *Ls)
+// a batch composition (the multiset of L) and the RK4 arithmetic mode it was timed in
+static std::vector<int> batch_key(const asd_ctx *c, int ntraj, const int *Ls)
 {
     std::vector<int> k(Ls, Ls + ntraj);
     std::sort(k.begin(), k.end());
+    k.push_back(c->rk4_exact ? -1 : -2);
     return k;
 }
 
@@ -644,7 +646,7 @@ int asd_batch_variant_for(asd_ctx *c, int ntraj, const int *Ls)
 {
     ASD_CHECK(c && ntraj >= 1 && Ls, "bad arguments");
     if (c->batch_variant >= 0) return c->batch_variant;
-    auto it = c->batch_tuned.find(batch_key(ntraj, Ls));
+    auto it = c->batch_tuned.find(batch_key(c, ntraj, Ls));
     return it == c->batch_tuned.end() ? -1 : it->second;
 }
 
@@ -704,7 +706,7 @@ int asd_autotune_batch(asd_ctx *c, int ntraj, const int *Ls, const asd_params *p
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc != ASD_OK) return rc;
-    c->batch_tuned[batch_key(ntraj, Ls)] = bi;
+    c->batch_tuned[batch_key(c, ntraj, Ls)] = bi;
     if (best) *best = bi;
     return ASD_OK;
 }
@@ -722,7 +724,7 @@ int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *pa
     // the kernel variant: forced, recorded for this batch composition, or autotuned now
     int v = c->batch_variant;
     if (v < 0) {
-        auto it = c->batch_tuned.find(batch_key(ntraj, Ls));
+        auto it = c->batch_tuned.find(batch_key(c, ntraj, Ls));
         if (it == c->batch_tuned.end()) {
             ASD_TRY(asd_autotune_batch(c, ntraj, Ls, params, &v, nullptr));
         } else {

@@ -979,6 +979,7 @@ static const BatchVariant kBatchVariants[] = {
     {8, 512, 2, "8x8 tiles, 512 threads, two steps per launch"},
     {32, 1024, 2, "32x32 tiles, 1024 threads, two steps per launch"},
     {8, 256, 1, "8x8 tiles, 256 threads, one step per launch"},
+    {32, 1024, 1, "32x32 tiles, 1024 threads, one step per launch"},
 };
 constexpr int kNumBatchVariants = sizeof(kBatchVariants) / sizeof(kBatchVariants[0]);
 int rk4_batch_num_variants() { return kNumBatchVariants; }
@@ -996,6 +997,7 @@ static BatchKernel batch_kernel(int v, int ns)
     case 2: return ns == 2 ? rk4_batch_kernel<8, 8, 512, 2, F> : rk4_batch_kernel<8, 8, 512, 1, F>;
     case 3: return ns == 2 ? rk4_batch_kernel<32, 32, 1024, 2, F> : rk4_batch_kernel<32, 32, 1024, 1, F>;
     case 4: return rk4_batch_kernel<8, 8, 256, 1, F>;
+    case 5: return rk4_batch_kernel<32, 32, 1024, 1, F>;
     default: return rk4_batch_kernel<16, 16, 256, 1, F>;
     }
 }

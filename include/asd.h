@@ -192,7 +192,8 @@ int asd_stage_times(asd_ctx *ctx, double t[5]);
  * d_snaps[ntraj] (device [nsnap][L*L][3] each, or NULL array).  The kernel variant (tile
  * shape, threads per tile, one or two fused steps per launch) is the one forced with
  * asd_batch_set_variant, else the one the autotuner recorded for this batch composition
- * (the multiset of L), else asd_autotune_batch runs first (a few ms on scratch states).
+ * (the multiset of L) in the current RK4 arithmetic mode (asd_rk4_set_exact), else
+ * asd_autotune_batch runs first (a few ms on scratch states).
  * Every variant gives the same bits.  Returns after the work has completed, with the
  * numeric guard of asd_rk4_run (ASD_ENOTFINITE names the trajectory and the first step).
  * The counterpart of the reference's fixed blocking (gemm_kernel.h:293-328) is the tuned
@@ -202,8 +203,8 @@ int asd_batch_rk4_run(asd_ctx *ctx, int ntraj, const int *Ls, const asd_params *
                       double *const *d_snaps);
 
 /* Times every batch kernel variant for this composition (20 steps on scratch states of the
- * same sizes and parameters, best of two runs after a warm-up) and records the fastest
- * for the multiset Ls; best (may be NULL) receives its index, times_ms (may be NULL,
+ * same sizes and parameters, best of two runs after a warm-up, in the current RK4
+ * arithmetic mode) and records the fastest for the multiset Ls and that mode; best (may be NULL) receives its index, times_ms (may be NULL,
  * asd_batch_num_variants() entries) the times. */
 int asd_autotune_batch(asd_ctx *ctx, int ntraj, const int *Ls, const asd_params *params, int *best,
                        double *times_ms);

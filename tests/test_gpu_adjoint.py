@@ -65,6 +65,38 @@ def test_loss_grad_against_finite_differences(gctx, orc):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("theta", [(1.0, 0.1, 0.1, 0.01), (1.1, 0.0, 0.3, 0.2)])
+def test_gradient_exactly_zero_at_a_fixed_point(gctx, orc, theta):
+    """A uniform ferromagnet along z is a fixed point for every (J, D, B_z, alpha): the
+    exchange, anisotropy and field terms are all parallel to S, so f and its derivatives with
+    respect to the parameters vanish exactly (h x S has only products with a zero factor).
+    The trajectory is static, S(q, tau) = 0 up to the rounding of the Fourier sums (the
+    connected and disconnected terms cancel) and dL/dtheta = 0 exactly for any target,
+    although the adjoint state, driven by S - S_target, is not zero.  The oracle's
+    tangent-linear model agrees."""
+    torch = need_gpu()
+    import asd_amd
+    L, every, T0, nlags = 8, 5, 6, 4
+    nsnap = T0 + nlags - 1
+    nsteps = (nsnap - 1) * every
+    q = gxmg(L)
+    S0 = np.zeros((L * L, 3))
+    S0[:, 2] = 1.0
+    rng = np.random.default_rng(4)
+    target = rng.standard_normal((nlags, len(q))) + 1j * rng.standard_normal((nlags, len(q)))
+    p = asd_amd.params(*theta, 0.01)
+    loss, grad = gctx.sqt_loss_grad(p, L, torch.tensor(S0, device="cuda"), nsteps, every, T0, nlags, q, target,
+                                    checkpoint_every=10)
+    s, _ = gctx.sqt_vjp(p, L, torch.tensor(S0, device="cuda"), nsteps, every, T0, nlags, q)
+    say(f"  fixed point theta={theta}: loss {loss:.6e}, grad {grad}")
+    assert np.abs(s).max() <= 1e-13
+    assert loss == pytest.approx(float(np.sum(np.abs(target) ** 2)), rel=1e-14)
+    assert np.all(grad == 0.0)
+    snaps, dsnaps = orc.tlm_run(orc.params(*theta, 0.01), L, S0, nsteps, every, nsnap)
+    assert np.all(snaps == S0) and np.all(dsnaps == 0.0)
+
+
+@pytest.mark.gpu
 def test_torch_autograd_wrapper(gctx, orc):
     torch = need_gpu()
     import asd_amd

@@ -7,6 +7,7 @@ numpy's FFT in FP64 (size-independent: it holds at any L, T0, lag count).  The t
 come from the GPU RK4 (pinned bit for bit against the oracle at smaller sizes in
 test_gpu_rk4.py) and must keep |S_i| = 1.
 
+* plain small L=32: bit-exact trajectory and S(q,tau) against the oracle itself.
 * plain L=128 (Ns=16384), J=1 D=0.1, seed 1, 8000 steps, snapshot every 10, 400 lags over
   400 origins on Gamma-X-M-Gamma: FP64 within 1e-10 of the identity, FP16X2 within 1e-5
   (relative to max_q |S(q,tau)| of each lag).
@@ -51,6 +52,38 @@ def run_snapshots(gctx, L, seed, nsteps, every, **kw):
     norm = torch.linalg.norm(snaps, dim=-1)
     assert float((norm - 1).abs().max()) <= 1e-12
     return snaps
+
+
+@pytest.mark.gpu
+def test_plain_small_L32_against_oracle(gctx, orc):
+    """The reference's CPU-sized case end to end: L = 32 (Ns = 1024), J = 1, D = 0.1, B = 0,
+    alpha = 0, seed 0, 2000 RK4 steps at dt = 0.01 with a snapshot every 10 (200 snapshots),
+    100 lags over 100 origins on the full 32 x 32 momentum grid.  The GPU trajectory equals
+    the oracle's bit for bit (all 200 snapshots and the final state); S(q, tau) in FP64 is
+    within 1e-10 of the oracle's separable form at every lag and of its GEMM form at lags
+    0 and 1 (the GEMM form costs the oracle ~4 s a lag); FP16X2 within 1e-5."""
+    torch = need_gpu()
+    import asd_amd
+    L, T0, nl, nsteps, every = 32, 100, 100, 2000, 10
+    nsnap = nsteps // every
+    say(f"plain small: L={L}, {nsteps} steps, T0={T0}, lags={nl}, full grid")
+    S0 = asd_amd.init_spins(0, L * L)
+    s = torch.tensor(S0, device="cuda")
+    snaps = torch.empty((nsnap, L * L, 3), dtype=torch.float64, device="cuda")
+    gctx.rk4_run(asd_amd.params(J=1.0, D=0.1, dt=0.01), L, s, nsteps, every, nsnap, snaps)
+    S_ref, snaps_ref = orc.run(orc.params(J=1.0, D=0.1, dt=0.01), L, S0, nsteps, every, nsnap)
+    assert np.array_equal(s.cpu().numpy(), S_ref)
+    assert np.array_equal(snaps.cpu().numpy(), snaps_ref)
+    q = asd_amd.full_grid(L)
+    ref = orc.sqt_separable(L, snaps_ref, T0, nl, q)
+    got = gctx.sqt(L, snaps, T0, nl, q, "fp64")
+    scale = np.abs(ref).max(axis=1, keepdims=True)
+    e64 = float((np.abs(got - ref) / scale).max())
+    ref_g = orc.sqt_gemm(L, snaps_ref[:T0 + 1], T0, 2, q)
+    e_gemm = float((np.abs(got[:2] - ref_g) / scale[:2]).max())
+    e16, r16 = err_stats(gctx.sqt(L, snaps, T0, nl, q, "fp16x2"), ref)
+    say(f"  fp64 vs separable {e64:.2e}, vs GEMM form {e_gemm:.2e}; fp16x2 max {e16:.2e} rms {r16:.2e}")
+    assert e64 <= 1e-10 and e_gemm <= 1e-10 and e16 <= 1e-5
 
 
 @pytest.mark.gpu

@@ -79,6 +79,9 @@ __global__ void set_int_kernel(int *p, int v) { *p = v; }
 
 // longest step loop captured as one graph (longer snapshot intervals run eagerly)
 constexpr long kMaxGraphSteps = 4096;
+// snapshot intervals per captured RK4 chunk (up to; see rk4_drive)
+constexpr int kMaxSnapsPerChunk = 16;
+constexpr long kMaxChunkSteps = 512;
 
 // ---------------------------------------------- scratch of asd_rk4_run_noalloc
 // The allocation-free RK4 entry point keeps its second state buffer (the ping-pong
@@ -119,16 +122,15 @@ static int guard_verdict(unsigned long long h, bool batch)
                      batch ? "trajectory" : "replica", who, step);
 }
 
-// the guard of a launch at offset k of a captured chunk whose snapshot launches (k % every
-// == 0) bump *ctr after they run: its step is every * *ctr + k % every, less every when it
-// is not a snapshot launch (the counter was bumped once more in its interval)
-static Guard chunk_guard(unsigned long long *first, const int *ctr, int k, int every)
+// the guard of a launch at offset k of a captured chunk of `chunk` steps after which one
+// node bumps *ctr (the counter counts the chunks run): its step is chunk * *ctr + k
+static Guard chunk_guard(unsigned long long *first, const int *ctr, int k, int chunk)
 {
     Guard g;
     g.first = first;
     g.ctr = ctr;
-    g.every = every;
-    g.step = (k % every) - (k % every ? every : 0);
+    g.every = chunk;
+    g.step = k;
     return g;
 }
 static Guard eager_guard(unsigned long long *first, long step)
@@ -341,13 +343,30 @@ static int rk4_drive(asd_ctx *c, const LLGParams &p, int L, int nrep, double *d_
     // between the two steps of a launch (snapshots are of a launch's input state)
     const int fs = rk4_fusable_steps(c, L, nrep) == 2 && (!d_snaps || snap_every % 2 == 0) ? 2 : 1;
 
-    // graph: one chunk = `every` steps, doubled where needed so that it holds an even number
-    // of launches (the ping-pong parity repeats); the first launch of each snapshot interval
-    // writes snapshot *counter and a tiny node bumps the counter
-    long chunk = snap_every;
-    if ((chunk / fs) % 2) chunk *= 2;
-    if (use_graph && d_snaps && nsnap_in_loop >= 2 && counter && chunk <= kMaxGraphSteps) {
-        const long nchunks = std::min<long>(nsteps / chunk, nsnap_in_loop / (chunk / snap_every));
+    // graph: one chunk = spc snapshot intervals, an even number of launches (the ping-pong
+    // parity repeats).  The launch at offset k of a chunk with k % every == 0 writes snapshot
+    // slot spc * *counter + k / every (its snapshot pointer is offset by k / every slots and
+    // its slot stride is spc slots) and one node at the end of the chunk bumps the counter, so
+    // the counter counts chunks.  spc is the largest count up to kMaxSnapsPerChunk (and
+    // kMaxChunkSteps steps) that divides the snapshots due in the loop: one bump per chunk
+    // instead of one per snapshot (a bump node costs about a step launch: 3 us at L = 128,
+    // where a step is 2.9 us).
+    auto chunk_of = [&](long m) {
+        long ch = m * snap_every;
+        if ((ch / fs) % 2) ch *= 2;
+        return ch;
+    };
+    if (use_graph && d_snaps && snap_every >= 1 && nsnap_in_loop >= 2 && counter && chunk_of(1) <= kMaxGraphSteps) {
+        long chunk = chunk_of(1);
+        for (long m = std::min<long>(kMaxSnapsPerChunk, nsnap_in_loop); m > 1; m--) {
+            const long ch = chunk_of(m);
+            if (ch <= kMaxChunkSteps && nsnap_in_loop % (ch / snap_every) == 0 && nsteps >= ch) {
+                chunk = ch;
+                break;
+            }
+        }
+        const long spc = chunk / snap_every;  // snapshot intervals per chunk
+        const long nchunks = std::min<long>(nsteps / chunk, nsnap_in_loop / spc);
         if (nchunks >= 1) {
             RK4Graph &g = c->graph;
             const bool same = g.exec && g.L == L && g.nrep == nrep && g.every == snap_every && g.chunk == chunk &&
@@ -368,9 +387,10 @@ static int rk4_drive(asd_ctx *c, const LLGParams &p, int L, int nrep, double *d_
                 for (int k = 0; k < chunk && r == ASD_OK; k += fs) {
                     const bool snap_here = k % snap_every == 0;
                     const int j = k / fs;  // launch index: its input buffer alternates
-                    const Guard gd = first ? chunk_guard(first, counter, k, snap_every) : Guard();
-                    r = rk4_launch_step(c, p, L, nrep, bufs[j & 1], bufs[(j + 1) & 1], snap_here ? d_snaps : nullptr,
-                                        counter, snap_stride, snap_here ? 1 : 0, fs, gd);
+                    const Guard gd = first ? chunk_guard(first, counter, k, (int)chunk) : Guard();
+                    r = rk4_launch_step(c, p, L, nrep, bufs[j & 1], bufs[(j + 1) & 1],
+                                        snap_here ? d_snaps + (k / snap_every) * snap_stride : nullptr, counter,
+                                        spc * snap_stride, k + fs >= chunk ? 1 : 0, fs, gd);
                 }
                 c->stream = saved;
                 hipError_t e1 = hipStreamEndCapture(cap, &graph);
@@ -389,7 +409,7 @@ static int rk4_drive(asd_ctx *c, const LLGParams &p, int L, int nrep, double *d_
             for (long ch = 0; ch < nchunks; ch++) ASD_HIP(hipGraphLaunch(g.exec, st));
             s = nchunks * chunk;
             cur = 0;  // an even number of launches per chunk: state back in bufs[0]
-            // the counter now equals s / every; continue eagerly with explicit slots
+            // continue eagerly with explicit slots
         }
     }
     while (s < nsteps) {
@@ -689,7 +709,7 @@ int asd_autotune_batch(asd_ctx *c, int ntraj, const int *Ls, const asd_params *p
             if (hipEventRecord(e0, st) != hipSuccess) rc = set_error(ASD_EHIP, "hipEventRecord failed");
             for (int k = 0; k < kSteps && rc == ASD_OK; k += fs)
                 rc = rk4_batch_launch(c, v, fs, c->tabs.as<BatchTraj>(), c->work.as<int2>(), nitems, (k / fs) & 1,
-                                      nullptr, 0, 0);
+                                      nullptr, 0, -1, 0);
             float t = 0.f;
             if (rc == ASD_OK && (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
                                  hipEventElapsedTime(&t, e0, e1) != hipSuccess))
@@ -757,11 +777,28 @@ int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *pa
     // two fused steps per launch where the variant has them and no snapshot falls between them
     const int fs = rk4_batch_variant_steps(v) == 2 && every % 2 == 0 ? 2 : 1;
     const long nsnap_in_loop = d_snaps ? std::min<long>(nsnap, (nsteps + every - 1) / every) : 0;
-    // a graph chunk holds an even number of launches, so the ping-pong parity repeats
-    long chunk = every;
-    if ((chunk / fs) % 2) chunk *= 2;
+    // a graph chunk is spc intervals of `every` steps, an even number of launches (the
+    // ping-pong parity repeats), with one counter bump at its end (the counter counts
+    // chunks); the launch at offset k with k % every == 0 writes snapshot slot
+    // spc * *counter + k / every.  spc as in rk4_drive: the largest count up to
+    // kMaxSnapsPerChunk (and kMaxChunkSteps steps) that divides the intervals to run.
+    auto chunk_of = [&](long m) {
+        long ch = m * every;
+        if ((ch / fs) % 2) ch *= 2;
+        return ch;
+    };
+    const long nint = d_snaps ? nsnap_in_loop : nsteps / every;
+    long chunk = chunk_of(1);
+    for (long m = std::min<long>(kMaxSnapsPerChunk, nint); m > 1; m--) {
+        const long ch = chunk_of(m);
+        if (ch <= kMaxChunkSteps && nint % (ch / every) == 0 && nsteps >= ch) {
+            chunk = ch;
+            break;
+        }
+    }
+    const long spc = chunk / every;
     long nchunks = nsteps / chunk;
-    if (d_snaps) nchunks = std::min<long>(nchunks, nsnap_in_loop / (chunk / every));
+    if (d_snaps) nchunks = std::min<long>(nchunks, nsnap_in_loop / spc);
     if (chunk > kMaxGraphSteps) nchunks = 0;
     long s = 0;
     if (nchunks >= 1) {
@@ -780,15 +817,10 @@ int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *pa
             c->stream = cap;
             int r = ASD_OK;
             for (int k = 0; k < chunk && r == ASD_OK; k += fs) {
-                const int snap_here = d_snaps && k % every == 0;
-                // with snapshots the counter is bumped after each snapshot launch; without,
-                // once at the end of the chunk, so it counts chunks
-                Guard gk = d_snaps ? chunk_guard(first, ctr, k, every) : chunk_guard(first, ctr, 0, (int)chunk);
-                if (!d_snaps) gk.step = k;
-                r = rk4_batch_launch(c, v, fs, dt, dw, nitems, (k / fs) & 1, ctr, snap_here, snap_here, gk);
+                const int slot = d_snaps && k % every == 0 ? k / every : -1;
+                r = rk4_batch_launch(c, v, fs, dt, dw, nitems, (k / fs) & 1, ctr, (int)spc, slot, k + fs >= chunk,
+                                     chunk_guard(first, ctr, k, (int)chunk));
             }
-            if (r == ASD_OK && !d_snaps)
-                hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(1), 0, c->stream, ctr);
             c->stream = saved;
             hipError_t e1 = hipStreamEndCapture(cap, &graph);
             (void)hipStreamDestroy(cap);
@@ -808,14 +840,14 @@ int asd_batch_rk4_run(asd_ctx *c, int ntraj, const int *Ls, const asd_params *pa
         for (long ch = 0; ch < nchunks; ch++) ASD_HIP(hipGraphLaunch(g.exec, st));
         s = nchunks * chunk;
     }
-    // remainder, eagerly: the counter already holds the next snapshot slot; the state is
-    // back in the a buffers after the graph's even number of launches
+    // remainder, eagerly, with explicit snapshot slots; the state is back in the a buffers
+    // after the graph's even number of launches
     int cur = 0;
     while (s < nsteps) {
-        const int snap_here = d_snaps && s % every == 0 && s / every < nsnap;
+        const int slot = d_snaps && s % every == 0 && s / every < nsnap ? (int)(s / every) : -1;
         const bool mid_snap = d_snaps && (s + 1) % every == 0 && (s + 1) / every < nsnap;
         const int k = fs == 2 && s + 2 <= nsteps && !mid_snap ? 2 : 1;
-        ASD_TRY(rk4_batch_launch(c, v, k, dt, dw, nitems, cur, ctr, snap_here, snap_here, eager_guard(first, s)));
+        ASD_TRY(rk4_batch_launch(c, v, k, dt, dw, nitems, cur, nullptr, 0, slot, 0, eager_guard(first, s)));
         cur ^= 1;
         s += k;
     }

@@ -140,10 +140,13 @@ namespace asd {
 // record stage event k on the context stream when timing is on
 int mark(asd_ctx *ctx, int k);
 // internal entry points shared between translation units
-// one launch of batch kernel variant `variant` doing `steps` (1, or 2 for a two-step variant)
+// one launch of batch kernel variant `variant` doing `steps` (1, or 2 for a two-step variant);
+// it writes its input states as snapshot slot *snap_counter * snap_mul + snap_add (the counter
+// may be NULL: slot snap_add), none where snap_add < 0, and bumps *snap_counter afterwards when
+// bump_counter is set
 int rk4_batch_launch(asd_ctx *ctx, int variant, int steps, const BatchTraj *d_trajs, const int2 *d_items,
-                     int nitems, int parity, const int *snap_counter, int write_snap, int bump_counter,
-                     const Guard &guard = Guard());
+                     int nitems, int parity, const int *snap_counter, int snap_mul, int snap_add,
+                     int bump_counter, const Guard &guard = Guard());
 int rk4_batch_num_variants();
 const char *rk4_batch_variant_name(int v);
 int rk4_batch_variant_tile(int v);

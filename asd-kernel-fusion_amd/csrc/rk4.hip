@@ -444,7 +444,8 @@ __global__ __launch_bounds__(NT) void rk4_step_kernel(LLGParams p, int L,
 template <int TR, int TC, int NT = TR * TC, int NS = 1, bool FAST = false>
 __global__ __launch_bounds__(NT) void rk4_batch_kernel(const BatchTraj *__restrict__ trajs,
                                                        const int2 *__restrict__ items, int nitems, int parity,
-                                                       const int *__restrict__ snap_counter, int write_snap, Guard gd)
+                                                       const int *__restrict__ snap_counter, int snap_mul,
+                                                       int snap_add, Guard gd)
 {
     __shared__ TileLDS<TR, TC, NS> sm;
     for (int it = blockIdx.x; it < nitems; it += gridDim.x) {
@@ -453,7 +454,10 @@ __global__ __launch_bounds__(NT) void rk4_batch_kernel(const BatchTraj *__restri
         const int ntc = (tr.L + TC - 1) / TC;
         const double *in = parity ? tr.b : tr.a;
         double *out = parity ? tr.a : tr.b;
-        double *sn = (write_snap && tr.snaps) ? tr.snaps + (long)*snap_counter * 3L * tr.L * tr.L : nullptr;
+        // snapshot slot *snap_counter * snap_mul + snap_add (none where snap_add < 0)
+        double *sn = nullptr;
+        if (snap_add >= 0 && tr.snaps)
+            sn = tr.snaps + ((snap_counter ? (long)*snap_counter * snap_mul : 0L) + snap_add) * 3L * tr.L * tr.L;
         __syncthreads();  // LDS of the previous item fully consumed
         if (tr.p.J == 1.0 && tr.p.alpha == 0.0)
             rk4_tile<TR, TC, true, true, NT, NS, FAST>(sm, tr.p, tr.L, w.y / ntc, w.y % ntc, in, out, sn, gd, w.x);
@@ -987,7 +991,7 @@ const char *rk4_batch_variant_name(int v) { return v >= 0 && v < kNumBatchVarian
 int rk4_batch_variant_tile(int v) { return kBatchVariants[v].tile; }
 int rk4_batch_variant_steps(int v) { return kBatchVariants[v].steps; }
 
-using BatchKernel = void (*)(const BatchTraj *, const int2 *, int, int, const int *, int, Guard);
+using BatchKernel = void (*)(const BatchTraj *, const int2 *, int, int, const int *, int, int, Guard);
 
 template <bool F>
 static BatchKernel batch_kernel(int v, int ns)
@@ -1025,8 +1029,8 @@ static int batch_grid(BatchKernel k, int threads, int nitems, int num_cu)
 }
 
 int rk4_batch_launch(asd_ctx *ctx, int variant, int steps, const BatchTraj *d_trajs, const int2 *d_items,
-                     int nitems, int parity, const int *snap_counter, int write_snap, int bump_counter,
-                     const Guard &guard)
+                     int nitems, int parity, const int *snap_counter, int snap_mul, int snap_add,
+                     int bump_counter, const Guard &guard)
 {
     ASD_CHECK(variant >= 0 && variant < kNumBatchVariants, "batch variant %d out of range", variant);
     ASD_CHECK(steps == 1 || (steps == 2 && kBatchVariants[variant].steps == 2), "variant %d cannot fuse %d steps",
@@ -1034,7 +1038,7 @@ int rk4_batch_launch(asd_ctx *ctx, int variant, int steps, const BatchTraj *d_tr
     const BatchKernel k = ctx->rk4_exact ? batch_kernel<false>(variant, steps) : batch_kernel<true>(variant, steps);
     const int threads = kBatchVariants[variant].threads;
     hipLaunchKernelGGL(k, dim3(batch_grid(k, threads, nitems, ctx->num_cu)), dim3(threads), 0, ctx->stream, d_trajs,
-                       d_items, nitems, parity, snap_counter, write_snap, guard);
+                       d_items, nitems, parity, snap_counter, snap_mul, snap_add, guard);
     if (bump_counter)
         hipLaunchKernelGGL(bump_counter_kernel, dim3(1), dim3(1), 0, ctx->stream, (int *)snap_counter);
     ASD_HIP(hipGetLastError());

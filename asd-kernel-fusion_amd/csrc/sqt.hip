@@ -374,6 +374,12 @@ __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
     for (int kt = 0; kt < nk; kt++) {
         const int cur = kt & 1;
         if (kt + 1 < nk) gload(kt + 1);
+        // the stage's fragment reads and MFMAs at raised wave priority, its LDS stores and
+        // barrier at the normal one: the SIMD's other wave (the other workgroup's) then gets
+        // its stores and barrier issued in the gaps of this wave's MFMAs instead of
+        // competing with them (FP64 at K = 1200: 950 -> 926 ms per 100 lags; priority
+        // around each 4-deep step's MFMAs only, 998 ms, or over the stores too, 993 ms, lose)
+        __builtin_amdgcn_s_setprio(1);
 #pragma unroll
         for (int kk = 0; kk < BK / 4; kk++) {
             const int kr = kk * 4 + lane / 16;
@@ -387,6 +393,7 @@ __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
 #pragma unroll
                 for (int n = 0; n < 4; n++) acc[m][n] = KMajorMfma<T>::mfma(fa[m], fb[n], acc[m][n]);
         }
+        __builtin_amdgcn_s_setprio(0);
         if (kt + 1 < nk) sstore(cur ^ 1);
         __syncthreads();
     }
@@ -838,6 +845,10 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
                 // keep the prefetch ahead of this step's MFMAs (the scheduler would
                 // otherwise sink it below the LDS stores to reuse their registers)
                 __builtin_amdgcn_sched_barrier(0);
+                // fragment reads and MFMAs at raised wave priority, the LDS stores and the
+                // barrier at the normal one (8 waves of 32 x 64 at K = 1200: 233 -> 213 ms per
+                // 100 lags; no change with 4 waves, one per SIMD)
+                __builtin_amdgcn_s_setprio(1);
 #pragma unroll
                 for (int kk = 0; kk < BK / 16; kk++) {
                     const int kc = kk * 16 + 8 * (lane / 32);
@@ -868,6 +879,7 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
                             }
                         }
                 }
+                __builtin_amdgcn_s_setprio(0);
                 ASD_SPLIT_SSTORE(h ^ 1, h ^ 1);
                 __syncthreads();
             }
@@ -1025,6 +1037,10 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
         __builtin_amdgcn_s_barrier();
         asm volatile("" ::: "memory");
         if (k + NS - 1 < nk) issue(k + NS - 1, (h + NS - 1) % NS);
+        // fragment reads and MFMAs at raised wave priority, the waits, barrier and load
+        // requests at the normal one (as in the k-major kernel; FP16X2 at K = 1200:
+        // 320.5 -> 310.5 ms per 200 lags)
+        __builtin_amdgcn_s_setprio(1);
         const uint16_t *sb = lds + h * STAGE;
         // reads in the order the MFMAs use them: per 16-deep half A hi, B hi (the hh
         // products), B lo (hl), A lo (lh)
@@ -1088,6 +1104,7 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
             }
             if constexpr (MT > RT - R0) __builtin_amdgcn_sched_group_barrier(0x008, MT - (RT - R0), 0);
         }
+        __builtin_amdgcn_s_setprio(0);
         asm volatile("" ::: "memory");
     };
 

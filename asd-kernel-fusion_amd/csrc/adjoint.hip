@@ -235,7 +235,7 @@ using namespace asd;
 
 static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *d_S0, long nsteps, int every,
                          int T0, int nlags, const int *h_q, int nq, const double *h_g, int checkpoint_every,
-                         double grad[4], double *h_sqt);
+                         double grad[4], double *h_sqt, const double *h_target = nullptr, double *loss = nullptr);
 
 // The reverse sweep linearises the reference's operation order (rk4_stages_store), so the
 // forward and recomputed trajectories take the exact RK4 arithmetic whatever the
@@ -252,11 +252,13 @@ extern "C" int asd_sqt_vjp(asd_ctx *c, const asd_params *pp, int L, const double
     return r;
 }
 
+// h_target (with loss): the misfit L = sum |S - S_target|^2 is formed on the host from this
+// forward pass and its gradient taken with g = 2 (S - S_target), in one forward pass.
 static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *d_S0, long nsteps, int every,
                          int T0, int nlags, const int *h_q, int nq, const double *h_g, int checkpoint_every,
-                         double grad[4], double *h_sqt)
+                         double grad[4], double *h_sqt, const double *h_target, double *loss)
 {
-    ASD_CHECK(c && pp && d_S0 && h_q && (grad || !h_g), "NULL argument");
+    ASD_CHECK(c && pp && d_S0 && h_q && (grad || (!h_g && !h_target)) && (!h_target || loss), "NULL argument");
     ASD_CHECK(L >= 2 && L <= 1024, "L out of range");
     ASD_CHECK(every >= 1 && T0 >= 1 && nlags >= 1 && nq >= 1, "bad sizes");
     ASD_CHECK(checkpoint_every >= 1, "checkpoint_every must be >= 1");
@@ -302,15 +304,33 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
         ASD_TRY(asd_rk4_run(c, pp, L, 1, S, len, every, nsn, nsn > 0 ? snaps + s0 * n : nullptr, 0));
         if (a + len >= nsteps) break;
     }
-    // ---- S(q,tau) in FP64 (leaves F_q(t) of all snapshots in c->fq)
-    ASD_TRY(c->outdev.ensure(sizeof(double) * 2 * (long)nlags * nq));
-    c->sqt_generic_only = 1;
-    const int rs = sqt_run(c, L, snaps, snaps, n, nsnap, T0, nlags, h_q, nq, ASD_FP64, c->outdev.as<double>(), -1);
-    c->sqt_generic_only = 0;
-    ASD_TRY(rs);
-    if (h_sqt)
-        ASD_HIP(hipMemcpyAsync(h_sqt, c->outdev.ptr, sizeof(double) * 2 * (long)nlags * nq, hipMemcpyDeviceToHost,
-                               st));
+    // ---- S(q,tau) in FP64, which leaves F_q(t) of all snapshots in c->fq; the reverse sweep
+    // needs only F, so a gradient whose S(q,tau) nobody asked for skips the correlation GEMM
+    const long m2 = 2L * nlags * nq;
+    if (h_sqt || h_target || !h_g) {
+        ASD_TRY(c->outdev.ensure(sizeof(double) * m2));
+        c->sqt_generic_only = 1;
+        const int rs = sqt_run(c, L, snaps, snaps, n, nsnap, T0, nlags, h_q, nq, ASD_FP64, c->outdev.as<double>(), -1);
+        c->sqt_generic_only = 0;
+        ASD_TRY(rs);
+        if (h_sqt) ASD_HIP(hipMemcpyAsync(h_sqt, c->outdev.ptr, sizeof(double) * m2, hipMemcpyDeviceToHost, st));
+    } else {
+        ASD_TRY(sqt_fourier(c, L, snaps, n, nsnap, h_q, nq));
+    }
+    std::vector<double> gmis;
+    if (h_target) {  // the misfit and its upstream gradient g = 2 (S - S_target)
+        gmis.resize(m2);
+        ASD_HIP(hipMemcpyAsync(gmis.data(), c->outdev.ptr, sizeof(double) * m2, hipMemcpyDeviceToHost, st));
+        ASD_HIP(hipStreamSynchronize(st));
+        double l = 0.0;
+        for (long i = 0; i < m2; i++) {
+            const double d = gmis[i] - h_target[i];
+            l += d * d;
+            gmis[i] = 2.0 * d;
+        }
+        *loss = l;
+        h_g = gmis.data();
+    }
     if (!h_g) {  // forward only
         ASD_HIP(hipStreamSynchronize(st));
         return ASD_OK;
@@ -375,18 +395,12 @@ extern "C" int asd_sqt_loss_grad(asd_ctx *c, const asd_params *p, int L, const d
                                  int checkpoint_every, double *loss, double grad[4])
 {
     ASD_CHECK(c && h_target && loss && grad, "NULL argument");
-    // forward once for S, then the VJP with g = 2 (S - S_target)
-    const long m = 2L * nlags * nq;
-    std::vector<double> s(m), g(m);
-    ASD_TRY(asd_sqt_vjp(c, p, L, d_S0, nsteps, snap_every, T0, nlags, h_q, nq, nullptr, checkpoint_every, nullptr,
-                        s.data()));
-    double l = 0.0;
-    for (long i = 0; i < m; i++) {
-        const double d = s[i] - h_target[i];
-        l += d * d;
-        g[i] = 2.0 * d;
-    }
-    *loss = l;
-    return asd_sqt_vjp(c, p, L, d_S0, nsteps, snap_every, T0, nlags, h_q, nq, g.data(), checkpoint_every, grad,
-                       nullptr);
+    // one forward pass (trajectory, checkpoints, S), the misfit and g = 2 (S - S_target) on
+    // the host, then the reverse sweep over the same checkpoints
+    const int saved = c->rk4_exact;
+    c->rk4_exact = 1;
+    const int r = sqt_vjp_exact(c, p, L, d_S0, nsteps, snap_every, T0, nlags, h_q, nq, nullptr, checkpoint_every,
+                                grad, nullptr, h_target, loss);
+    c->rk4_exact = saved;
+    return r;
 }

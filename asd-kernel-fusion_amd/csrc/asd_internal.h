@@ -179,4 +179,7 @@ void blas_destroy(asd_ctx *ctx);
 int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_b, long ta_stride,
             int nsnap_a, int T0, int nlags, const int *h_q, int nq, asd_precision prec,
             double *d_out, int cfg);
+// the Fourier sums F_q(t) of nt snapshots into ctx->fq ([t][q][xyz] complex, the layout sqt_run
+// leaves there on its generic path), with ctx->twiddle and ctx->qdev set up as sqt_run does
+int sqt_fourier(asd_ctx *ctx, int L, const double *d_snaps, long tstride, int nt, const int *h_q, int nq);
 }  // namespace asd

@@ -1439,6 +1439,34 @@ __global__ void zero_kernel(double *p, long n)
 static long round_up(long a, long b) { return (a + b - 1) / b * b; }
 
 
+// the exact twiddle table e^{2 pi i k / L} and the q list on the device (ctx->twiddle, ctx->qdev)
+static int load_phases(asd_ctx *ctx, int L, const int *h_q, int nq)
+{
+    hipStream_t st = ctx->stream;
+    ASD_TRY(ctx->twiddle.ensure(sizeof(double2) * L));
+    ASD_TRY(ctx->qdev.ensure(sizeof(int) * 2 * nq));
+    std::vector<double2> tw(L);
+    for (int k = 0; k < L; k++) {
+        // exact phases: angle 2 pi k / L evaluated in long double
+        const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)L;
+        tw[k] = make_double2((double)cosl(ang), (double)sinl(ang));
+    }
+    ASD_HIP(hipMemcpyAsync(ctx->twiddle.ptr, tw.data(), sizeof(double2) * L, hipMemcpyHostToDevice, st));
+    ASD_HIP(hipMemcpyAsync(ctx->qdev.ptr, h_q, sizeof(int) * 2 * nq, hipMemcpyHostToDevice, st));
+    ASD_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
+    return ASD_OK;
+}
+
+int sqt_fourier(asd_ctx *ctx, int L, const double *d_snaps, long tstride, int nt, const int *h_q, int nq)
+{
+    ASD_TRY(load_phases(ctx, L, h_q, nq));
+    ASD_TRY(ctx->fq.ensure(sizeof(double) * 6 * (long)nt * nq));
+    hipLaunchKernelGGL(fourier_kernel, dim3(nq, nt), dim3(256), 0, ctx->stream, d_snaps, tstride, L,
+                       ctx->qdev.as<int>(), ctx->twiddle.as<double2>(), ctx->fq.as<double>(), nq);
+    ASD_HIP(hipGetLastError());
+    return ASD_OK;
+}
+
 int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_b, long tstride,
             int nsnap_a, int T0, int nlags, const int *h_q, int nq, asd_precision prec, double *d_out,
             int cfg)
@@ -1456,20 +1484,7 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
     ASD_CHECK(nsnap_a >= nt_a, "need nsnap >= T0 + nlags - 1 (%d < %d)", nsnap_a, nt_a);
     hipStream_t st = ctx->stream;
 
-    // twiddles and q list
-    ASD_TRY(ctx->twiddle.ensure(sizeof(double2) * L));
-    ASD_TRY(ctx->qdev.ensure(sizeof(int) * 2 * nq));
-    {
-        std::vector<double2> tw(L);
-        for (int k = 0; k < L; k++) {
-            // exact phases: angle 2 pi k / L evaluated in long double
-            const long double ang = 2.0L * 3.14159265358979323846264338327950288L * (long double)k / (long double)L;
-            tw[k] = make_double2((double)cosl(ang), (double)sinl(ang));
-        }
-        ASD_HIP(hipMemcpyAsync(ctx->twiddle.ptr, tw.data(), sizeof(double2) * L, hipMemcpyHostToDevice, st));
-        ASD_HIP(hipMemcpyAsync(ctx->qdev.ptr, h_q, sizeof(int) * 2 * nq, hipMemcpyHostToDevice, st));
-        ASD_HIP(hipStreamSynchronize(st));  // host vectors go out of scope
-    }
+    ASD_TRY(load_phases(ctx, L, h_q, nq));
     // bins: fixed-point words [word][lag][d] (two words for FP64), then the decoded doubles
     const long nbins = (long)nlags * Ns;
     const int nw = prec == ASD_FP64 ? 2 : 1;

@@ -237,7 +237,8 @@ int asd_sqt_set_config(asd_ctx *ctx, asd_precision prec, int cfg);
  * gradient with respect to theta = (J, D, B_z, alpha), by a hand-written discrete
  * adjoint of RK4-LLG (incl. renormalisation) and of the correlation, with
  * forward states checkpointed every `checkpoint_every` steps and recomputed in
- * reverse.  h_target: host [nlags][nq][2].  FP64 throughout. */
+ * reverse; one forward pass serves the loss and the gradient.  h_target: host
+ * [nlags][nq][2].  FP64 throughout. */
 int asd_sqt_loss_grad(asd_ctx *ctx, const asd_params *p, int L, const double *d_S0,
                       long nsteps, int snap_every, int T0, int nlags, const int *h_q,
                       int nq, const double *h_target, int checkpoint_every,
@@ -246,8 +247,9 @@ int asd_sqt_loss_grad(asd_ctx *ctx, const asd_params *p, int L, const double *d_
 /* Vector-Jacobian product of the same forward map (RK4 from d_S0, snapshots,
  * FP64 S(q,tau)): grad[p] = sum_{lag,q} (g_re dRe S/dtheta_p + g_im dIm S/dtheta_p)
  * for upstream h_g [nlags][nq][2] (host).  h_sqt (host, may be NULL) receives
- * S(q,tau).  With h_g == NULL only the forward runs (grad may be NULL).
- * checkpoint_every is rounded up to a multiple of snap_every. */
+ * S(q,tau).  With h_g == NULL only the forward runs (grad may be NULL); with h_g and
+ * no h_sqt the correlation GEMM is skipped (the reverse sweep needs only the Fourier
+ * sums of the snapshots).  checkpoint_every is rounded up to a multiple of snap_every. */
 int asd_sqt_vjp(asd_ctx *ctx, const asd_params *p, int L, const double *d_S0, long nsteps,
                 int snap_every, int T0, int nlags, const int *h_q, int nq, const double *h_g,
                 int checkpoint_every, double grad[4], double *h_sqt);

@@ -643,6 +643,26 @@ def main():
         del xa, xb, xc
         log(f"  library calibration: {res['library_calibration']}")
 
+    # calibration of the FP64 headline: the vendor library's plain DGEMM (torch.matmul on float64)
+    # at one lag's shape, FP64 output, no epilogue, against the fused FP64 kernel's rate
+    if args.precision == "fp64":
+        xa = torch.randn(ns, 3 * T0, device="cuda", dtype=torch.float64)
+        xb = torch.randn(3 * T0, ns, device="cuda", dtype=torch.float64)
+        for _ in range(2):
+            xc = xa @ xb
+        torch.cuda.synchronize()
+        nmm = 10
+        t_mm = time.perf_counter()
+        for _ in range(nmm):
+            xc = xa @ xb
+        torch.cuda.synchronize()
+        lib64 = 2.0 * ns * ns * 3 * T0 * nmm / (time.perf_counter() - t_mm) / 1e12
+        res["library_calibration_fp64"] = {"library_dgemm_tflops": round(lib64, 2), "shape": f"{ns}x{ns}x{3 * T0}",
+                                           "fused_kernel_tflops": round(gemm_tflops, 2),
+                                           "ratio": round(gemm_tflops / lib64, 3)}
+        del xa, xb, xc
+        log(f"  FP64 library calibration: {res['library_calibration_fp64']}")
+
     # the same RK4 step on a lattice batch far larger than the caches (L = 1024, 64
     # replicas: 1.6 GB of state), the HBM-bound regime; the automatic kernel choice
     # takes the register-streaming strips here

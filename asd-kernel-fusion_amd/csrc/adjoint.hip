@@ -159,6 +159,31 @@ __global__ void add_kernel(long n, double *__restrict__ a, const double *__restr
     if (i < n) a[i] += b[i];
 }
 
+// The reverse sweep's per-segment work reads the segment index from the device (*sg), so one
+// captured graph serves every full segment: dst = src[*sg * n ..] (the segment's checkpoint),
+// a += b[(*sg * mul + add) * n ..] when that snapshot slot is below nslots (dL/dX of a
+// snapshot), dst = src (plain copy), *sg += delta (the next segment).
+__global__ void copy_indexed_kernel(long n, double *__restrict__ dst, const double *__restrict__ src,
+                                    const int *__restrict__ sg)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[(long)*sg * n + i];
+}
+__global__ void add_indexed_kernel(long n, double *__restrict__ a, const double *__restrict__ b,
+                                   const int *__restrict__ sg, int mul, int add, int nslots)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long slot = (long)*sg * mul + add;
+    if (i < n && slot < nslots) a[i] += b[slot * n + i];
+}
+__global__ void copy_kernel(long n, double *__restrict__ dst, const double *__restrict__ src)
+{
+    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[i];
+}
+__global__ void step_index_kernel(int *sg, int delta) { *sg += delta; }
+__global__ void set_index_kernel(int *sg, int v) { *sg = v; }
+
 // Fbar[tau][q][6] = (1/T0) sum_{t0<T0} F[t0+tau][q][6]
 __global__ void fbar_kernel(const double *__restrict__ F, int nq, int T0, int nlags, double *__restrict__ Fbar)
 {
@@ -277,7 +302,7 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
     // workspace: snapshots | checkpoints | segment states | 8 stage buffers | 5 adjoint buffers | dX |
     // per-block gradient partials | grad
     const int nblk = (int)g1(ns).x;
-    const long words = (long)nsnap * n + ncp * n + (long)C * n + 8 * n + 5 * n + (long)nsnap * n + 4L * nblk + 4;
+    const long words = (long)nsnap * n + ncp * n + (long)C * n + 8 * n + 5 * n + (long)nsnap * n + 4L * nblk + 4 + 1;
     ASD_TRY(c->adj0.ensure(sizeof(double) * words));
     double *snaps = c->adj0.as<double>();
     double *cps = snaps + (long)nsnap * n;
@@ -288,6 +313,7 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
     double *dX = gS + n;
     double *gacc = dX + (long)nsnap * n;
     double *dgrad = gacc + 4L * nblk;
+    int *dsg = reinterpret_cast<int *>(dgrad + 4);  // the reverse sweep's segment index
 
     // ---- forward with checkpoints (every C steps) and snapshots (every `every` steps)
     ASD_TRY(c->adj1.ensure(sizeof(double) * n));
@@ -354,34 +380,75 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
     ASD_HIP(hipMemsetAsync(lam, 0, sizeof(double) * n, st));
     if (nsteps % every == 0 && nsteps / every < nsnap)
         ASD_HIP(hipMemcpyAsync(lam, dX + (nsteps / every) * n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-    const long nseg = (nsteps + C - 1) / C;
-    for (long sg = nseg - 1; sg >= 0; sg--) {
-        const long a = sg * C, b = std::min<long>(a + C, nsteps);
-        ASD_HIP(hipMemcpyAsync(seg, cps + sg * n, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
-        for (long m = 1; m < b - a; m++)
+    // One segment's work (the checkpoint at *dsg, the recomputed states, the reverse steps),
+    // enqueued on c->stream; the same launches for every segment of `len` steps, since C is a
+    // multiple of `every` (a snapshot falls on the same offsets in every segment).
+    auto segment = [&](long len) -> int {
+        hipStream_t s2 = c->stream;
+        hipLaunchKernelGGL(copy_indexed_kernel, g1(n), dim3(256), 0, s2, n, seg, (const double *)cps,
+                           (const int *)dsg);
+        for (long m = 1; m < len; m++)
             ASD_TRY(rk4_launch_step(c, p, L, 1, seg + (m - 1) * n, seg + m * n, nullptr, nullptr, 0, 0));
-        for (long m = b - a - 1; m >= 0; m--) {
+        for (long m = len - 1; m >= 0; m--) {
             const double *Sn = seg + m * n;
             ASD_TRY(rk4_stages_store(c, p, L, Sn, Y2, Y3, Y4, cc, k1, k2, k3, k4));
             // lam holds dL/dS_{n+1}; k1 is reused as dL/dS_n accumulator
             double *lamS = k1;
-            hipLaunchKernelGGL(adj_renorm_kernel, g1(ns), dim3(256), 0, st, ns, p.len, (const double *)cc,
+            hipLaunchKernelGGL(adj_renorm_kernel, g1(ns), dim3(256), 0, s2, ns, p.len, (const double *)cc,
                                (const double *)lam, lamc, lamS);
             const double *Ys[4] = {Sn, Y2, Y3, Y4};
             const double ca[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
             const double cb[4] = {0.5, 0.5, 1.0, 0.0};
             for (int sidx = 3; sidx >= 0; sidx--) {
-                hipLaunchKernelGGL(adj_jt1_kernel, g1(ns), dim3(256), 0, st, p, L, Ys[sidx], (const double *)lamc,
+                hipLaunchKernelGGL(adj_jt1_kernel, g1(ns), dim3(256), 0, s2, p, L, Ys[sidx], (const double *)lamc,
                                    ca[sidx], sidx == 3 ? (const double *)nullptr : (const double *)carry, cb[sidx],
                                    gh, gS, gacc);
-                hipLaunchKernelGGL(adj_jt2_kernel, g1(ns), dim3(256), 0, st, p, L, (const double *)gh,
+                hipLaunchKernelGGL(adj_jt2_kernel, g1(ns), dim3(256), 0, s2, p, L, (const double *)gh,
                                    (const double *)gS, lamS, carry);
             }
-            const long nidx = a + m;
-            if (nidx % every == 0 && nidx / every < nsnap)
-                hipLaunchKernelGGL(add_kernel, g1(n), dim3(256), 0, st, n, lamS, (const double *)(dX + (nidx / every) * n));
-            ASD_HIP(hipMemcpyAsync(lam, lamS, sizeof(double) * n, hipMemcpyDeviceToDevice, st));
+            if (m % every == 0)  // step sg C + m is a snapshot step
+                hipLaunchKernelGGL(add_indexed_kernel, g1(n), dim3(256), 0, s2, n, lamS, (const double *)dX,
+                                   (const int *)dsg, C / every, (int)(m / every), nsnap);
+            hipLaunchKernelGGL(copy_kernel, g1(n), dim3(256), 0, s2, n, lam, (const double *)lamS);
         }
+        hipLaunchKernelGGL(step_index_kernel, dim3(1), dim3(1), 0, s2, dsg, -1);
+        ASD_HIP(hipGetLastError());
+        return ASD_OK;
+    };
+    const long nseg = (nsteps + C - 1) / C;
+    hipLaunchKernelGGL(set_index_kernel, dim3(1), dim3(1), 0, st, dsg, (int)(nseg - 1));
+    long nfull = nseg;
+    if (nsteps % C) {  // the last segment is shorter: eagerly, first
+        ASD_TRY(segment(nsteps % C));
+        nfull--;
+    }
+    if (nfull >= 2) {
+        // the full segments replay one captured graph (about 19 launches a step, launch-bound)
+        hipStream_t cap;
+        ASD_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
+        hipGraph_t graph;
+        hipGraphExec_t exec = nullptr;
+        hipError_t e = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+        int r = ASD_OK;
+        if (e == hipSuccess) {
+            c->stream = cap;
+            r = segment(C);
+            c->stream = st;
+            e = hipStreamEndCapture(cap, &graph);
+        }
+        if (e == hipSuccess && r == ASD_OK) {
+            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+        }
+        (void)hipStreamDestroy(cap);
+        ASD_TRY(r);
+        ASD_HIP(e);
+        for (long sg = 0; sg < nfull && e == hipSuccess; sg++) e = hipGraphLaunch(exec, st);
+        (void)hipStreamSynchronize(st);
+        (void)hipGraphExecDestroy(exec);
+        ASD_HIP(e);
+    } else if (nfull == 1) {
+        ASD_TRY(segment(C));
     }
     hipLaunchKernelGGL(grad_reduce_kernel, dim3(1), dim3(64), 0, st, (const double *)gacc, nblk, dgrad);
     ASD_HIP(hipGetLastError());

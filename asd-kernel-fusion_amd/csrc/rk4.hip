@@ -246,15 +246,25 @@ template <int TR, int TC, int NS = 1> struct TileLDS {
     double Y[2][3][NR];
 };
 
-template <int TR, int TC, bool UJ = false, bool UD = false, int NT = TR * TC, int NS = 1, bool FAST = false>
+// STORE (one step, exact arithmetic; the adjoint's recomputation): each output site's stage
+// inputs Y2 = S + k1/2, Y3 = S + k2/2, Y4 = S + k3 and its unnormalised sum c = S + k1/6 + k2/3 +
+// k3/3 + k4/6 are also written to so.Y2 / Y3 / Y4 / c ([site][xyz] of the lattice).
+struct StageOut {
+    double *Y2 = nullptr, *Y3 = nullptr, *Y4 = nullptr, *c = nullptr;
+};
+
+template <int TR, int TC, bool UJ = false, bool UD = false, int NT = TR * TC, int NS = 1, bool FAST = false,
+          bool STORE = false>
 __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParams &p, int L, int tile_r, int tile_c,
                                          const double *__restrict__ in, double *__restrict__ out,
-                                         double *__restrict__ snap, const Guard &gd, int traj)
+                                         double *__restrict__ snap, const Guard &gd, int traj,
+                                         const StageOut &so = StageOut())
 {
     constexpr int R = 4 * NS, RR = TR + 2 * R, RC = TC + 2 * R, NR = RR * RC;
     constexpr int W = RC - 2, NW = (RR - 2) * W;  // the widest stage region
     constexpr int NE = (NW + NT - 1) / NT;        // its sites per thread
     static_assert(NT >= TR * TC, "the snapshot writer takes one tile site per thread");
+    static_assert(!STORE || (NS == 1 && !FAST), "stage values are stored for one exact step");
     double(&sS)[3][NR] = sm.Y[1];
     const int r0 = tile_r * TR - R, c0 = tile_c * TC - R;
     const int t = threadIdx.x;
@@ -284,6 +294,13 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
         }
     }
 
+    // lattice site of this thread's j-th site when it is one of the tile's outputs, else -1
+    auto tile_site = [&](int j) -> long {
+        const int e = t + j * NT, er = 1 + e / W, ec = 1 + e % W;
+        const int grow = tile_r * TR + er - R, gcol = tile_c * TC + ec - R;
+        if (e >= NW || er < R || er >= R + TR || ec < R || ec >= R + TC || grow >= L || gcol >= L) return -1;
+        return (long)grow * L + gcol;
+    };
     // this thread's sites: LDS index, depth, state
     int idx[NE], depth[NE];
     double S[NE][3];
@@ -349,6 +366,13 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
                         y[j][d] = S[j][d] + k[d];
                         Yout[d][i] = y[j][d];
                     }
+                    if constexpr (STORE) {
+                        const long gs = tile_site(j);
+                        if (gs >= 0) {
+                            double *o = (stage == 1 ? so.Y2 : stage == 2 ? so.Y3 : so.Y4) + 3 * gs;
+                            o[0] = y[j][0]; o[1] = y[j][1]; o[2] = y[j][2];
+                        }
+                    }
                 }
                 if (depth[j] >= 4 * m + 4) {
 #pragma unroll
@@ -386,6 +410,10 @@ __device__ __forceinline__ void rk4_tile(TileLDS<TR, TC, NS> &sm, const LLGParam
                 const int grow = tile_r * TR + er - R, gcol = tile_c * TC + ec - R;
                 if (grow >= L || gcol >= L) continue;
                 double *o = out + 3 * ((long)grow * L + gcol);
+                if constexpr (STORE) {
+                    double *oc = so.c + 3 * ((long)grow * L + gcol);
+                    oc[0] = A[j][0]; oc[1] = A[j][1]; oc[2] = A[j][2];
+                }
                 const double x = FAST ? renorm_lean(p.len, A[j][0], A[j][1], A[j][2], o[0], o[1], o[2])
                                       : renorm_fast(p.len, A[j][0], A[j][1], A[j][2], o[0], o[1], o[2]);
                 if (gd.first && out_of_range(x)) guard_report(gd, traj, m);
@@ -949,21 +977,34 @@ int rk4_compat_stages(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const d
 // Stage inputs of one RK4 step for the adjoint: Y2 = S + k1/2, Y3 = S + k2/2,
 // Y4 = S + k3 and the unnormalised c = S + k1/6 + k2/3 + k3/3 + k4/6, with the
 // same rounding steps as the fused kernel (so recomputed stages are bit-identical).
+// The adjoint's stage recomputation in one launch: the one-step exact tile kernel (8 x 8 tiles,
+// 256 threads) with the stage inputs and the unnormalised sum of every site stored.  The values
+// are the ones the separate RHS / axpy / combine passes of the reference's runge_kutta produce
+// (S + k1/2 as S + kh, the partial sums by the FMA-corrected quotients), since rk4_tile's
+// arithmetic is that of the reference (IEEE ==).
+template <bool UJ, bool UD>
+__global__ __launch_bounds__(256) void rk4_stages_kernel(LLGParams p, int L, const double *__restrict__ in,
+                                                         double *__restrict__ scratch, StageOut so)
+{
+    __shared__ TileLDS<8, 8, 1> sm;
+    rk4_tile<8, 8, UJ, UD, 256, 1, false, true>(sm, p, L, blockIdx.y, blockIdx.x, in, scratch, nullptr, Guard(), 0,
+                                                so);
+}
+
+// S -> Y2 = S + k1/2, Y3 = S + k2/2, Y4 = S + k3, c = S + k1/6 + k2/3 + k3/3 + k4/6 (one lattice);
+// k2 serves as the scratch for the renormalised state, k1, k3 and k4 are not written
 int rk4_stages_store(asd_ctx *ctx, const LLGParams &p, int L, const double *S, double *Y2, double *Y3, double *Y4,
                      double *c, double *k1, double *k2, double *k3, double *k4)
 {
-    const long ns = (long)L * L, n = ns * 3;
-    const dim3 gs((unsigned)((ns + 255) / 256)), ge((unsigned)((n + 255) / 256)), b(256);
-    hipStream_t st = ctx->stream;
-    hipLaunchKernelGGL(rhs_kernel, gs, b, 0, st, p, L, ns, S, k1, p.dt);
-    hipLaunchKernelGGL(axpy_stage_kernel, ge, b, 0, st, n, S, (const double *)k1, Y2, 1);
-    hipLaunchKernelGGL(rhs_kernel, gs, b, 0, st, p, L, ns, (const double *)Y2, k2, p.dt);
-    hipLaunchKernelGGL(axpy_stage_kernel, ge, b, 0, st, n, S, (const double *)k2, Y3, 1);
-    hipLaunchKernelGGL(rhs_kernel, gs, b, 0, st, p, L, ns, (const double *)Y3, k3, p.dt);
-    hipLaunchKernelGGL(axpy_stage_kernel, ge, b, 0, st, n, S, (const double *)k3, Y4, 0);
-    hipLaunchKernelGGL(rhs_kernel, gs, b, 0, st, p, L, ns, (const double *)Y4, k4, p.dt);
-    hipLaunchKernelGGL(rk4_combine_kernel, ge, b, 0, st, n, S, (const double *)k1, (const double *)k2,
-                       (const double *)k3, (const double *)k4, c);
+    (void)k1; (void)k3; (void)k4;
+    StageOut so;
+    so.Y2 = Y2; so.Y3 = Y3; so.Y4 = Y4; so.c = c;
+    const bool uj = p.J == 1.0, ud = p.alpha == 0.0;
+    auto k = uj && ud ? rk4_stages_kernel<true, true>
+                      : (uj ? rk4_stages_kernel<true, false>
+                            : (ud ? rk4_stages_kernel<false, true> : rk4_stages_kernel<false, false>));
+    const dim3 grid((unsigned)((L + 7) / 8), (unsigned)((L + 7) / 8));
+    hipLaunchKernelGGL(k, grid, dim3(256), 0, ctx->stream, p, L, S, k2, so);
     ASD_HIP(hipGetLastError());
     return ASD_OK;
 }

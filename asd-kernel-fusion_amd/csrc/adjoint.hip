@@ -52,78 +52,6 @@ __device__ __forceinline__ void cross3(const double *a, const double *b, double 
     o[2] = a[0] * b[1] - a[1] * b[0];
 }
 
-// lambda_c = len (lam' - c^ (c^.lam')) / |c|; lamS = lambda_c
-__global__ void adj_renorm_kernel(long ns, double len, const double *__restrict__ c,
-                                  const double *__restrict__ lam_next, double *__restrict__ lamc,
-                                  double *__restrict__ lamS)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= ns) return;
-    const double *cc = c + 3 * i, *l = lam_next + 3 * i;
-    const double nrm = sqrt(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
-    const double u[3] = {cc[0] / nrm, cc[1] / nrm, cc[2] / nrm};
-    const double d = u[0] * l[0] + u[1] * l[1] + u[2] * l[2];
-    for (int k = 0; k < 3; k++) {
-        const double v = len * (l[k] - u[k] * d) / nrm;
-        lamc[3 * i + k] = v;
-        lamS[3 * i + k] = v;
-    }
-}
-
-// per-site dL/dh (gh) and local dL/dS (gS) for mu = ca*lamc + cb*carry at stage input Y,
-// plus dt-weighted parameter partials added to this block's own slot gacc[block][4] (no
-// atomics: launches are stream-ordered, so the sums repeat bit for bit)
-__global__ __launch_bounds__(256) void adj_jt1_kernel(LLGParams p, int L, const double *__restrict__ Y,
-                                                      const double *__restrict__ lamc, double ca,
-                                                      const double *__restrict__ carry, double cb,
-                                                      double *__restrict__ gh, double *__restrict__ gS,
-                                                      double *__restrict__ gacc)
-{
-    __shared__ double red[4][256];
-    const long ns = (long)L * L;
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    double part[4] = {0, 0, 0, 0};
-    if (i < ns) {
-        const int row = (int)(i / L), col = (int)(i % L);
-        const double *s = Y + 3 * i;
-        const long nb[4] = {(long)wrapi(row + 1, L) * L + col, (long)wrapi(row - 1, L) * L + col,
-                            (long)row * L + wrapi(col - 1, L), (long)row * L + wrapi(col + 1, L)};
-        double sn[3] = {0, 0, 0};
-        for (int j = 0; j < 4; j++)
-            for (int k = 0; k < 3; k++) sn[k] += Y[3 * nb[j] + k];
-        const double h[3] = {p.J * sn[0], p.J * sn[1], p.J * sn[2] + 2.0 * p.D * s[2] + p.Bz};
-        double pp[3], sxp[3], f[3], mu[3], mxs[3], nu[3], ghv[3], t1[3], t2[3];
-        cross3(h, s, pp);
-        cross3(s, pp, sxp);
-        for (int k = 0; k < 3; k++) f[k] = (pp[k] + p.a_over_len * sxp[k]) / p.den;
-        for (int k = 0; k < 3; k++) mu[k] = ca * lamc[3 * i + k] + (carry ? cb * carry[3 * i + k] : 0.0);
-        cross3(mu, s, mxs);
-        for (int k = 0; k < 3; k++) nu[k] = (mu[k] + p.a_over_len * mxs[k]) / p.den;
-        cross3(s, nu, ghv);
-        cross3(nu, h, t1);
-        cross3(pp, mu, t2);
-        for (int k = 0; k < 3; k++) {
-            gh[3 * i + k] = ghv[k];
-            gS[3 * i + k] = t1[k] + (p.a_over_len / p.den) * t2[k];
-        }
-        gS[3 * i + 2] += 2.0 * p.D * ghv[2];
-        part[0] = p.dt * (ghv[0] * sn[0] + ghv[1] * sn[1] + ghv[2] * sn[2]);
-        part[1] = p.dt * 2.0 * ghv[2] * s[2];
-        part[2] = p.dt * ghv[2];
-        double da = 0.0;
-        for (int k = 0; k < 3; k++) da += mu[k] * (sxp[k] / (p.len * p.den) - 2.0 * p.alpha * f[k] / p.den);
-        part[3] = p.dt * da;
-    }
-    for (int k = 0; k < 4; k++) red[k][threadIdx.x] = part[k];
-    __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
-        if ((int)threadIdx.x < st)
-            for (int k = 0; k < 4; k++) red[k][threadIdx.x] += red[k][threadIdx.x + st];
-        __syncthreads();
-    }
-    if (threadIdx.x < 4) gacc[4L * blockIdx.x + threadIdx.x] += red[threadIdx.x][0];
-}
-
 // grad[k] = sum over blocks b (in order) of gacc[b][k]; one thread per parameter
 __global__ void grad_reduce_kernel(const double *__restrict__ gacc, int nblk, double *__restrict__ grad)
 {
@@ -134,47 +62,14 @@ __global__ void grad_reduce_kernel(const double *__restrict__ gacc, int nblk, do
     grad[k] = s;
 }
 
-// lambda_Y(j) = dt (gS(j) + J sum_{i in nn(j)} gh(i)); lamS += lambda_Y; carry = lambda_Y
-__global__ void adj_jt2_kernel(LLGParams p, int L, const double *__restrict__ gh, const double *__restrict__ gS,
-                               double *__restrict__ lamS, double *__restrict__ carry)
-{
-    const long ns = (long)L * L;
-    const long j = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= ns) return;
-    const int row = (int)(j / L), col = (int)(j % L);
-    const long nb[4] = {(long)wrapi(row + 1, L) * L + col, (long)wrapi(row - 1, L) * L + col,
-                        (long)row * L + wrapi(col - 1, L), (long)row * L + wrapi(col + 1, L)};
-    for (int k = 0; k < 3; k++) {
-        double a = 0.0;
-        for (int n = 0; n < 4; n++) a += gh[3 * nb[n] + k];
-        const double v = p.dt * (gS[3 * j + k] + p.J * a);
-        lamS[3 * j + k] += v;
-        carry[3 * j + k] = v;
-    }
-}
-
-__global__ void add_kernel(long n, double *__restrict__ a, const double *__restrict__ b)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) a[i] += b[i];
-}
-
 // The reverse sweep's per-segment work reads the segment index from the device (*sg), so one
 // captured graph serves every full segment: dst = src[*sg * n ..] (the segment's checkpoint),
-// a += b[(*sg * mul + add) * n ..] when that snapshot slot is below nslots (dL/dX of a
-// snapshot), dst = src (plain copy), *sg += delta (the next segment).
+// dst = src (plain copy), *sg += delta (the next segment).
 __global__ void copy_indexed_kernel(long n, double *__restrict__ dst, const double *__restrict__ src,
                                     const int *__restrict__ sg)
 {
     const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) dst[i] = src[(long)*sg * n + i];
-}
-__global__ void add_indexed_kernel(long n, double *__restrict__ a, const double *__restrict__ b,
-                                   const int *__restrict__ sg, int mul, int add, int nslots)
-{
-    const long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    const long slot = (long)*sg * mul + add;
-    if (i < n && slot < nslots) a[i] += b[slot * n + i];
 }
 __global__ void copy_kernel(long n, double *__restrict__ dst, const double *__restrict__ src)
 {
@@ -252,6 +147,123 @@ __global__ void dx_kernel(const double *__restrict__ Lam, const int *__restrict_
     for (int d = 0; d < 3; d++) dX[((long)t * Ns + i) * 3 + d] = 2.0 * a[d];
 }
 
+// One reverse RK4 step in one launch, per T x T tile with a radius-5 halo: the renormalisation
+// adjoint, then for the stages s = 4, 3, 2, 1 the local part of J_f^T (adj_jt1's formulas: dL/dh,
+// the local dL/dS and the parameter partials) and its neighbour gather (adj_jt2's: lambda_Y =
+// dt (gS + J sum_nn dL/dh)), with the stage inputs and dL/dh in LDS.  Each reverse stage widens
+// the dependency by one site (the gather), and the first stage's field needs one more, so the
+// tile's own sites (depth >= 5 in the loaded region) end with the exact lamS of the separate
+// kernels' formulas.  Parameter partials are summed over the tile's own sites only, in a fixed
+// order (thread, then a tree over the block), into the block's slot gacc[block][4].  The snapshot
+// adjoint dX[*sg * mul + add] (add < 0: none) is added to the result, written to lam_out.
+constexpr int kAdjTile = 8, kAdjHalo = 5, kAdjRR = kAdjTile + 2 * kAdjHalo, kAdjNR = kAdjRR * kAdjRR;
+constexpr int kAdjThreads = 512;  // >= kAdjNR: one loaded site per thread
+static_assert(kAdjThreads >= kAdjNR, "one site of the loaded region per thread");
+
+__global__ __launch_bounds__(kAdjThreads) void adj_step_kernel(
+    LLGParams p, int L, const double *__restrict__ S, const double *__restrict__ Y2, const double *__restrict__ Y3,
+    const double *__restrict__ Y4, const double *__restrict__ c, const double *__restrict__ lam,
+    double *__restrict__ lam_out, double *__restrict__ gacc, const double *__restrict__ dX,
+    const int *__restrict__ sg, int mul, int add, int nslots)
+{
+    __shared__ double sY[3][kAdjNR], sG[3][kAdjNR];
+    __shared__ double red[4][kAdjThreads];
+    const int t = threadIdx.x;
+    const int tr = blockIdx.y, tc = blockIdx.x;
+    const bool has = t < kAdjNR;
+    const int er = has ? t / kAdjRR : 0, ec = has ? t % kAdjRR : 0;
+    const int depth = has ? min(min(er, kAdjRR - 1 - er), min(ec, kAdjRR - 1 - ec)) : -1;
+    const int grow = wrapi(tr * kAdjTile - kAdjHalo + er, L), gcol = wrapi(tc * kAdjTile - kAdjHalo + ec, L);
+    const long gi = (long)grow * L + gcol;
+    const bool own = depth >= kAdjHalo && tr * kAdjTile + er - kAdjHalo < L && tc * kAdjTile + ec - kAdjHalo < L;
+    // neighbours in the loaded region (row+1, row-1, col-1, col+1), valid where depth >= 1
+    const int li = er * kAdjRR + ec;
+    const int nb[4] = {li + kAdjRR, li - kAdjRR, li - 1, li + 1};
+
+    // renormalisation adjoint: lamc = len (lam - u (u . lam)) / |c|, lamS = lamc
+    double lamc[3] = {0, 0, 0}, lamS[3] = {0, 0, 0}, carry[3] = {0, 0, 0}, gS[3] = {0, 0, 0};
+    if (has) {
+        const double *cc = c + 3 * gi, *l = lam + 3 * gi;
+        const double nrm = sqrt(cc[0] * cc[0] + cc[1] * cc[1] + cc[2] * cc[2]);
+        const double u[3] = {cc[0] / nrm, cc[1] / nrm, cc[2] / nrm};
+        const double d = u[0] * l[0] + u[1] * l[1] + u[2] * l[2];
+        for (int k = 0; k < 3; k++) {
+            lamc[k] = p.len * (l[k] - u[k] * d) / nrm;
+            lamS[k] = lamc[k];
+        }
+    }
+    double part[4] = {0, 0, 0, 0};
+    const double *Ys[4] = {S, Y2, Y3, Y4};
+    const double ca[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+    const double cb[4] = {0.5, 0.5, 1.0, 0.0};
+    for (int sidx = 3; sidx >= 0; sidx--) {
+        const int level = 3 - sidx;
+        if (has) {
+            const double *y = Ys[sidx] + 3 * gi;
+            sY[0][li] = y[0]; sY[1][li] = y[1]; sY[2][li] = y[2];
+        }
+        __syncthreads();
+        if (depth >= level + 1) {  // adj_jt1's formulas at this site
+            const double s[3] = {sY[0][li], sY[1][li], sY[2][li]};
+            double sn[3] = {0, 0, 0};
+            for (int j = 0; j < 4; j++)
+                for (int k = 0; k < 3; k++) sn[k] += sY[k][nb[j]];
+            const double h[3] = {p.J * sn[0], p.J * sn[1], p.J * sn[2] + 2.0 * p.D * s[2] + p.Bz};
+            double pp[3], sxp[3], f[3], mu[3], mxs[3], nu[3], ghv[3], t1[3], t2[3];
+            cross3(h, s, pp);
+            cross3(s, pp, sxp);
+            for (int k = 0; k < 3; k++) f[k] = (pp[k] + p.a_over_len * sxp[k]) / p.den;
+            for (int k = 0; k < 3; k++) mu[k] = ca[sidx] * lamc[k] + (sidx == 3 ? 0.0 : cb[sidx] * carry[k]);
+            cross3(mu, s, mxs);
+            for (int k = 0; k < 3; k++) nu[k] = (mu[k] + p.a_over_len * mxs[k]) / p.den;
+            cross3(s, nu, ghv);
+            cross3(nu, h, t1);
+            cross3(pp, mu, t2);
+            for (int k = 0; k < 3; k++) {
+                sG[k][li] = ghv[k];
+                gS[k] = t1[k] + (p.a_over_len / p.den) * t2[k];
+            }
+            gS[2] += 2.0 * p.D * ghv[2];
+            if (own) {
+                part[0] += p.dt * (ghv[0] * sn[0] + ghv[1] * sn[1] + ghv[2] * sn[2]);
+                part[1] += p.dt * 2.0 * ghv[2] * s[2];
+                part[2] += p.dt * ghv[2];
+                double da = 0.0;
+                for (int k = 0; k < 3; k++) da += mu[k] * (sxp[k] / (p.len * p.den) - 2.0 * p.alpha * f[k] / p.den);
+                part[3] += p.dt * da;
+            }
+        }
+        __syncthreads();
+        if (depth >= level + 2) {  // adj_jt2's gather
+            for (int k = 0; k < 3; k++) {
+                double a = 0.0;
+                for (int n = 0; n < 4; n++) a += sG[k][nb[n]];
+                const double v = p.dt * (gS[k] + p.J * a);
+                lamS[k] += v;
+                carry[k] = v;
+            }
+        }
+        __syncthreads();  // sY / sG are overwritten by the next stage
+    }
+    if (own) {
+        const long slot = add >= 0 ? (long)*sg * mul + add : -1;
+        double *o = lam_out + 3 * gi;
+        for (int k = 0; k < 3; k++) {
+            double v = lamS[k];
+            if (slot >= 0 && slot < nslots) v += dX[slot * 3L * L * L + 3 * gi + k];
+            o[k] = v;
+        }
+    }
+    for (int k = 0; k < 4; k++) red[k][t] = part[k];
+    __syncthreads();
+    for (int st = kAdjThreads / 2; st > 0; st >>= 1) {
+        if (t < st)
+            for (int k = 0; k < 4; k++) red[k][t] += red[k][t + st];
+        __syncthreads();
+    }
+    if (t < 4) gacc[4L * (blockIdx.y * gridDim.x + blockIdx.x) + t] += red[t][0];
+}
+
 static dim3 g1(long n) { return dim3((unsigned)((n + 255) / 256)); }
 
 }  // namespace asd
@@ -299,18 +311,20 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
     hipStream_t st = c->stream;
     const LLGParams p = make_llg(*pp);
 
-    // workspace: snapshots | checkpoints | segment states | 8 stage buffers | 5 adjoint buffers | dX |
+    // workspace: snapshots | checkpoints | segment states | 8 stage buffers | adjoint buffer | dX |
     // per-block gradient partials | grad
-    const int nblk = (int)g1(ns).x;
-    const long words = (long)nsnap * n + ncp * n + (long)C * n + 8 * n + 5 * n + (long)nsnap * n + 4L * nblk + 4 + 1;
+    // the reverse step's tiles; gacc holds one parameter-partial slot per tile
+    const dim3 tiles((unsigned)((L + kAdjTile - 1) / kAdjTile), (unsigned)((L + kAdjTile - 1) / kAdjTile));
+    const int nblk = (int)(tiles.x * tiles.y);
+    const long words = (long)nsnap * n + ncp * n + (long)C * n + 8 * n + n + (long)nsnap * n + 4L * nblk + 4 + 1;
     ASD_TRY(c->adj0.ensure(sizeof(double) * words));
     double *snaps = c->adj0.as<double>();
     double *cps = snaps + (long)nsnap * n;
     double *seg = cps + ncp * n;
     double *Y2 = seg + (long)C * n, *Y3 = Y2 + n, *Y4 = Y3 + n, *cc = Y4 + n;
     double *k1 = cc + n, *k2 = k1 + n, *k3 = k2 + n, *k4 = k3 + n;
-    double *lam = k4 + n, *lamc = lam + n, *carry = lamc + n, *gh = carry + n, *gS = gh + n;
-    double *dX = gS + n;
+    double *lam = k4 + n;  // dL/dS of the step after the current one
+    double *dX = lam + n;
     double *gacc = dX + (long)nsnap * n;
     double *dgrad = gacc + 4L * nblk;
     int *dsg = reinterpret_cast<int *>(dgrad + 4);  // the reverse sweep's segment index
@@ -392,23 +406,13 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
         for (long m = len - 1; m >= 0; m--) {
             const double *Sn = seg + m * n;
             ASD_TRY(rk4_stages_store(c, p, L, Sn, Y2, Y3, Y4, cc, k1, k2, k3, k4));
-            // lam holds dL/dS_{n+1}; k1 is reused as dL/dS_n accumulator
+            // lam holds dL/dS_{n+1}; dL/dS_n goes to k1, with the snapshot adjoint of step
+            // sg C + m added when that step is a snapshot step
             double *lamS = k1;
-            hipLaunchKernelGGL(adj_renorm_kernel, g1(ns), dim3(256), 0, s2, ns, p.len, (const double *)cc,
-                               (const double *)lam, lamc, lamS);
-            const double *Ys[4] = {Sn, Y2, Y3, Y4};
-            const double ca[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
-            const double cb[4] = {0.5, 0.5, 1.0, 0.0};
-            for (int sidx = 3; sidx >= 0; sidx--) {
-                hipLaunchKernelGGL(adj_jt1_kernel, g1(ns), dim3(256), 0, s2, p, L, Ys[sidx], (const double *)lamc,
-                                   ca[sidx], sidx == 3 ? (const double *)nullptr : (const double *)carry, cb[sidx],
-                                   gh, gS, gacc);
-                hipLaunchKernelGGL(adj_jt2_kernel, g1(ns), dim3(256), 0, s2, p, L, (const double *)gh,
-                                   (const double *)gS, lamS, carry);
-            }
-            if (m % every == 0)  // step sg C + m is a snapshot step
-                hipLaunchKernelGGL(add_indexed_kernel, g1(n), dim3(256), 0, s2, n, lamS, (const double *)dX,
-                                   (const int *)dsg, C / every, (int)(m / every), nsnap);
+            hipLaunchKernelGGL(adj_step_kernel, tiles, dim3(kAdjThreads), 0, s2, p, L, Sn, (const double *)Y2,
+                               (const double *)Y3, (const double *)Y4, (const double *)cc, (const double *)lam, lamS,
+                               gacc, (const double *)dX, (const int *)dsg, C / every,
+                               m % every == 0 ? (int)(m / every) : -1, nsnap);
             hipLaunchKernelGGL(copy_kernel, g1(n), dim3(256), 0, s2, n, lam, (const double *)lamS);
         }
         hipLaunchKernelGGL(step_index_kernel, dim3(1), dim3(1), 0, s2, dsg, -1);

@@ -311,19 +311,20 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
     hipStream_t st = c->stream;
     const LLGParams p = make_llg(*pp);
 
-    // workspace: snapshots | checkpoints | segment states | 8 stage buffers | adjoint buffer | dX |
+    // workspace: snapshots | checkpoints | segment states | 4 stage buffers | lamS | scratch | lam | dX |
     // per-block gradient partials | grad
     // the reverse step's tiles; gacc holds one parameter-partial slot per tile
     const dim3 tiles((unsigned)((L + kAdjTile - 1) / kAdjTile), (unsigned)((L + kAdjTile - 1) / kAdjTile));
     const int nblk = (int)(tiles.x * tiles.y);
-    const long words = (long)nsnap * n + ncp * n + (long)C * n + 8 * n + n + (long)nsnap * n + 4L * nblk + 4 + 1;
+    const long words = (long)nsnap * n + ncp * n + (long)C * n + 6 * n + n + (long)nsnap * n + 4L * nblk + 4 + 1;
     ASD_TRY(c->adj0.ensure(sizeof(double) * words));
     double *snaps = c->adj0.as<double>();
     double *cps = snaps + (long)nsnap * n;
     double *seg = cps + ncp * n;
     double *Y2 = seg + (long)C * n, *Y3 = Y2 + n, *Y4 = Y3 + n, *cc = Y4 + n;
-    double *k1 = cc + n, *k2 = k1 + n, *k3 = k2 + n, *k4 = k3 + n;
-    double *lam = k4 + n;  // dL/dS of the step after the current one
+    double *lamS = cc + n;      // dL/dS of the current step
+    double *scratch = lamS + n;  // the stage kernel's renormalised state (unused)
+    double *lam = scratch + n;   // dL/dS of the step after the current one
     double *dX = lam + n;
     double *gacc = dX + (long)nsnap * n;
     double *dgrad = gacc + 4L * nblk;
@@ -405,10 +406,9 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
             ASD_TRY(rk4_launch_step(c, p, L, 1, seg + (m - 1) * n, seg + m * n, nullptr, nullptr, 0, 0));
         for (long m = len - 1; m >= 0; m--) {
             const double *Sn = seg + m * n;
-            ASD_TRY(rk4_stages_store(c, p, L, Sn, Y2, Y3, Y4, cc, k1, k2, k3, k4));
-            // lam holds dL/dS_{n+1}; dL/dS_n goes to k1, with the snapshot adjoint of step
+            ASD_TRY(rk4_stages_store(c, p, L, Sn, Y2, Y3, Y4, cc, scratch));
+            // lam holds dL/dS_{n+1}; dL/dS_n goes to lamS, with the snapshot adjoint of step
             // sg C + m added when that step is a snapshot step
-            double *lamS = k1;
             hipLaunchKernelGGL(adj_step_kernel, tiles, dim3(kAdjThreads), 0, s2, p, L, Sn, (const double *)Y2,
                                (const double *)Y3, (const double *)Y4, (const double *)cc, (const double *)lam, lamS,
                                gacc, (const double *)dX, (const int *)dsg, C / every,

@@ -165,8 +165,10 @@ __global__ void set_u64_kernel(unsigned long long *p, unsigned long long v);
 __global__ void bump_counter_kernel(int *c);
 int rk4_compat_stages(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *spin,
                       double *cur, double *k1, double *k2, double *k3, double *k4);
+// one exact RK4 step's stage inputs Y2 = S + k1/2, Y3 = S + k2/2, Y4 = S + k3 and unnormalised sum
+// c of a lattice, in one launch (the renormalised state goes to `scratch`, [L*L][3])
 int rk4_stages_store(asd_ctx *ctx, const LLGParams &p, int L, const double *S, double *Y2, double *Y3, double *Y4,
-                     double *c, double *k1, double *k2, double *k3, double *k4);
+                     double *c, double *scratch);
 int rhs_launch(asd_ctx *ctx, const LLGParams &p, int L, int nrep, const double *S, double *dS);
 int sqt_num_configs(asd_precision prec);
 const char *sqt_config_name(asd_precision prec, int cfg);

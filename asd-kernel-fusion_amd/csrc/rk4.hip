@@ -991,12 +991,10 @@ __global__ __launch_bounds__(256) void rk4_stages_kernel(LLGParams p, int L, con
                                                 so);
 }
 
-// S -> Y2 = S + k1/2, Y3 = S + k2/2, Y4 = S + k3, c = S + k1/6 + k2/3 + k3/3 + k4/6 (one lattice);
-// k2 serves as the scratch for the renormalised state, k1, k3 and k4 are not written
+// S -> Y2 = S + k1/2, Y3 = S + k2/2, Y4 = S + k3, c = S + k1/6 + k2/3 + k3/3 + k4/6 (one lattice)
 int rk4_stages_store(asd_ctx *ctx, const LLGParams &p, int L, const double *S, double *Y2, double *Y3, double *Y4,
-                     double *c, double *k1, double *k2, double *k3, double *k4)
+                     double *c, double *scratch)
 {
-    (void)k1; (void)k3; (void)k4;
     StageOut so;
     so.Y2 = Y2; so.Y3 = Y3; so.Y4 = Y4; so.c = c;
     const bool uj = p.J == 1.0, ud = p.alpha == 0.0;
@@ -1004,7 +1002,7 @@ int rk4_stages_store(asd_ctx *ctx, const LLGParams &p, int L, const double *S, d
                       : (uj ? rk4_stages_kernel<true, false>
                             : (ud ? rk4_stages_kernel<false, true> : rk4_stages_kernel<false, false>));
     const dim3 grid((unsigned)((L + 7) / 8), (unsigned)((L + 7) / 8));
-    hipLaunchKernelGGL(k, grid, dim3(256), 0, ctx->stream, p, L, S, k2, so);
+    hipLaunchKernelGGL(k, grid, dim3(256), 0, ctx->stream, p, L, S, scratch, so);
     ASD_HIP(hipGetLastError());
     return ASD_OK;
 }

@@ -427,7 +427,8 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
         nfull--;
     }
     if (nfull >= 2) {
-        // the full segments replay one captured graph (about 19 launches a step, launch-bound)
+        // the full segments replay one captured graph (four small launches a step: the
+        // recomputed state, the stage values, the reverse step, the copy; launch-bound)
         hipStream_t cap;
         ASD_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         hipGraph_t graph;

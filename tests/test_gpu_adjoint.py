@@ -39,6 +39,33 @@ def test_vjp_matches_tangent_linear(gctx, orc, theta, ckpt):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("L", [5, 12, 20])
+def test_vjp_ragged_lattices_match_tangent_linear(gctx, orc, L):
+    """Lattices that are not a multiple of the reverse step's 8 x 8 tiles (L = 12, 20: partial
+    tiles at the far edges) or smaller than one tile with its halo (L = 5: the halo wraps
+    around the lattice more than once), against the oracle's tangent-linear model."""
+    torch = need_gpu()
+    import asd_amd
+    every, T0, nlags = 5, 6, 4
+    nsnap = T0 + nlags - 1
+    nsteps = (nsnap - 1) * every + 7
+    q = [(k % L, (2 * k) % L) for k in range(5)]
+    S0 = orc.init_spins(8 + L, L * L)
+    theta = (0.9, 0.07, 0.12, 0.03)
+    rng = np.random.default_rng(L)
+    g = rng.standard_normal((nlags, len(q), 2))
+    s_gpu, grad = gctx.sqt_vjp(asd_amd.params(*theta, 0.01), L, torch.tensor(S0, device="cuda"), nsteps, every, T0,
+                               nlags, q, g=g, checkpoint_every=10)
+    snaps, dsnaps = orc.tlm_run(orc.params(*theta, 0.01), L, S0, nsteps, every, nsnap)
+    s_ref = orc.sqt_gemm(L, snaps, T0, nlags, q)
+    assert np.abs(s_gpu - s_ref).max() <= 1e-10 * np.abs(s_ref).max()
+    tlm = np.array([np.sum(g[..., 0] * d.real + g[..., 1] * d.imag)
+                    for d in (orc.tlm_sqt(L, snaps, dsnaps[i], T0, nlags, q) for i in range(4))])
+    say(f"  L={L}: adjoint {grad}\n  tangent {tlm}")
+    assert np.abs(grad - tlm).max() <= 1e-9 * max(1.0, np.abs(tlm).max())
+
+
+@pytest.mark.gpu
 def test_loss_grad_against_finite_differences(gctx, orc):
     torch = need_gpu()
     import asd_amd

@@ -42,7 +42,7 @@ const char *asd_version(void);
 
 /* --------------------------------------------------------- parameters --- */
 /* Replaces the reference globals J1 and dt (llg_sqt_fusion.c:15, :17) and adds
- * the terms the north-star physics needs.  H_eff = J sum_nn S_j + (2 D S_z + B_z) z,
+ * easy-axis anisotropy, a field along z and Gilbert damping.  H_eff = J sum_nn S_j + (2 D S_z + B_z) z,
  * dS/dt = [H x S + (alpha/spin_length) S x (H x S)] * (1/(1 + alpha^2)) (reciprocal
  * formed once), which is the reference's h x S (llg_sqt_fusion.c:82) exactly when
  * D = B_z = alpha = 0. */

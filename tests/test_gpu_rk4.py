@@ -1,6 +1,6 @@
 """RK4-LLG on the GPU (asd_rk4_run) against the FP64 oracle.
 
-Bar (north star): trajectories within 1e-10 of the oracle over the first 100
+Bar: trajectories within 1e-10 of the oracle over the first 100
 steps and |S_i| = spin_length within 1e-12.  The kernel keeps the reference's
 rounding steps, so the comparison is also reported as bit-exact or not.
 """

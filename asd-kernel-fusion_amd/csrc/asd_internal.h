@@ -109,6 +109,10 @@ struct asd_ctx {
     asd::RK4Graph graph;
     // S(q,t) workspace
     asd::DevBuf opA, opB, opA2, opB2, bins, fq, twiddle, qdev, outdev, fix;
+    // what twiddle and qdev hold (lattice side, q list, buffers), so repeated calls skip the upload
+    int phases_L = 0;
+    std::vector<int> phases_q;
+    const void *phases_tw = nullptr, *phases_qd = nullptr;
     // compat (host-pointer) staging
     asd::DevBuf c0, c1, c2, c3, c4, c5, c6;
     // batching: work list, trajectory table, second state buffer

@@ -441,8 +441,8 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
             c->stream = st;
             e = hipStreamEndCapture(cap, &graph);
         }
-        if (e == hipSuccess && r == ASD_OK) {
-            e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) {  // the captured graph exists: instantiate it unless the capture failed
+            if (r == ASD_OK) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
             (void)hipGraphDestroy(graph);
         }
         (void)hipStreamDestroy(cap);

@@ -288,6 +288,11 @@ struct GemmArgs {
     const FixScale *fix;
 };
 
+// s_waitcnt immediates (gfx9 encoding: vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt[5:4]
+// in [15:14]) that wait on one counter only
+constexpr int wait_vmcnt(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14); }
+constexpr int kWaitLgkm0 = 15 | (7 << 4) | (3 << 14);
+
 // ------------------------------------------------------------ FP64 / FP32
 // k-major operands; v_mfma_{f64_16x16x4f64, f32_16x16x4f32}: lane l holds A[i=l%16][k=l/16],
 // B[k=l/16][j=l%16].  The 4 outputs of lane l are D[4(l/16)+v][l%16] for f32 but
@@ -313,12 +318,18 @@ template <> struct KMajorMfma<float> {
     static __device__ __forceinline__ int row(int lane, int v) { return 4 * (lane / 16) + v; }
 };
 
-template <class T>
+// DMA = 0: operand rows staged through registers, two LDS stages of 16 k rows.
+// DMA = 1 (FP64): rows moved by direct global -> LDS loads (buffer_load_dwordx4 ... lds, one
+// 64-lane load fills one 128-element row) through a ring of 4 stages of 8 k rows; each step
+// reads the next stage's first fragments after its MFMAs, so the MFMAs after a barrier start
+// on operands already in registers.
+template <class T, int DMA = 0>
 __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
 {
     // rows padded by 16 elements: the four row groups of a fragment read (lane / 16) then fall on
     // disjoint banks for FP32 and pairwise disjoint for FP64 (FP32 0.7% faster, FP64 unchanged)
     constexpr int BK = 16, PAD = 16, LDS_LD = TILE + PAD;
+    static_assert(DMA == 0 || std::is_same<T, double>::value, "the direct-to-LDS ring moves FP64 rows");
     typedef typename KMajorMfma<T>::acc_t acc_t;
     __shared__ __attribute__((aligned(16))) unsigned char smem_raw[2 * 2 * BK * LDS_LD * sizeof(T) > 8 * 4096
                                                                        ? 2 * 2 * BK * LDS_LD * sizeof(T)
@@ -349,6 +360,87 @@ __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
 #pragma unroll
         for (int b = 0; b < 4; b++) acc[a][b] = acc_t{0, 0, 0, 0};
 
+    if constexpr (DMA != 0) {
+        // ring slot s, array a (0: A, 1: B), k row r at ((2 s + a) DK + r) LDS_LD elements; the
+        // padded row stride keeps the fragment reads' bank pattern of the register-staged form
+        constexpr int DK = 8, NS = 4, PER_STAGE = 4;  // k rows per stage, stages, loads per wave and stage
+        static_assert(NS * 2 * DK * LDS_LD * sizeof(T) <= sizeof(smem_raw), "ring exceeds the LDS array");
+        static_assert(DK == 2 * 4, "each of the 4 waves loads 2 A rows and 2 B rows per stage");
+        T *ring = reinterpret_cast<T *>(smem_raw);
+        const int nkd = args.Kpad / DK;
+        const int wv = __builtin_amdgcn_readfirstlane(wave);
+        // one buffer resource per array and stage, based at the stage's first k row (64-bit
+        // address arithmetic, scalar); a row's source is that base + r ld 8 B (SGPR) + 16 B per lane
+        auto issue = [&](int kt, int st) {
+            const long kb = (long)kt * DK * ld;
+            const __amdgpu_buffer_rsrc_t rA =
+                __builtin_amdgcn_make_buffer_rsrc((void *)(Ap + kb), 0, 0x7fffffff, 0x00020000);
+            const __amdgpu_buffer_rsrc_t rB =
+                __builtin_amdgcn_make_buffer_rsrc((void *)(Bp + kb), 0, 0x7fffffff, 0x00020000);
+#pragma unroll
+            for (int u = 0; u < 2; u++) {
+                const int r = 2 * wv + u;
+                const int so = (int)(r * ld * (long)sizeof(T));
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rA, (__attribute__((address_space(3))) void *)(ring + ((2 * st + 0) * DK + r) * LDS_LD), 16,
+                    16 * lane, so, 0, 0);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rB, (__attribute__((address_space(3))) void *)(ring + ((2 * st + 1) * DK + r) * LDS_LD), 16,
+                    16 * lane, so, 0, 0);
+            }
+        };
+        auto frag = [&](int st, int kk, T(&fa)[4], T(&fb)[4]) {
+            const int kr = kk * 4 + lane / 16;
+            const T *sa = ring + ((2 * st + 0) * DK + kr) * LDS_LD + wi * 64 + lane % 16;
+            const T *sb = ring + ((2 * st + 1) * DK + kr) * LDS_LD + wj * 64 + lane % 16;
+#pragma unroll
+            for (int m = 0; m < 4; m++) fa[m] = sa[m * 16];
+#pragma unroll
+            for (int n = 0; n < 4; n++) fb[n] = sb[n * 16];
+        };
+        auto mfmas = [&](const T(&fa)[4], const T(&fb)[4]) {
+#pragma unroll
+            for (int m = 0; m < 4; m++)
+#pragma unroll
+                for (int n = 0; n < 4; n++) acc[m][n] = KMajorMfma<T>::mfma(fa[m], fb[n], acc[m][n]);
+        };
+        // prologue: stages 0 .. NS - 2 requested (a K shorter than that re-reads its last stage,
+        // so the counts waited for stay exact), stage 0 landed everywhere, its first fragments read
+#pragma unroll
+        for (int st = 0; st < NS - 1; st++) issue(min(st, nkd - 1), st);
+        __builtin_amdgcn_s_waitcnt(wait_vmcnt(PER_STAGE * (NS - 2)));
+        __builtin_amdgcn_s_barrier();
+        asm volatile("" ::: "memory");
+        T pa[4], pb[4];
+        frag(0, 0, pa, pb);
+        // step k: wait for this wave's loads of stage k + 1 (stage k + 2's may stay in flight),
+        // barrier (stage k + 1 is then in LDS for every wave, and every wave has finished reading
+        // the slot of step k - 1), refill that slot with stage k + 3, then the MFMAs of the two
+        // 4-deep steps of stage k (the first on the fragments read in step k - 1), and read stage
+        // k + 1's first fragments.  Reads of slot k - 1 all fed MFMAs of step k - 1, so they are
+        // complete by the barrier; only the prefetch of slot k + 1 is in flight across it.
+        for (int k = 0; k < nkd; k++) {
+            const int h = k & (NS - 1);
+            if (k + 2 < nkd)
+                __builtin_amdgcn_s_waitcnt(wait_vmcnt(PER_STAGE));
+            else
+                __builtin_amdgcn_s_waitcnt(wait_vmcnt(0));
+            __builtin_amdgcn_s_barrier();
+            asm volatile("" ::: "memory");
+            if (k + NS - 1 < nkd) issue(k + NS - 1, (k + NS - 1) & (NS - 1));
+            __builtin_amdgcn_s_setprio(1);
+            T fa[4], fb[4];
+            frag(h, 1, fa, fb);
+            mfmas(pa, pb);
+            mfmas(fa, fb);
+            if (k + 1 < nkd) frag((h + 1) & (NS - 1), 0, pa, pb);
+            __builtin_amdgcn_s_setprio(0);
+            asm volatile("" ::: "memory");
+        }
+        // every load in flight (a short K's re-reads included) lands before the bins reuse the LDS
+        __builtin_amdgcn_s_waitcnt(wait_vmcnt(0));
+        __syncthreads();
+    } else {
     T ra[VEC], rb[VEC];
     auto gload = [&](int kt) {
         const T *a = Ap + (long)(kt * BK + lk) * ld + lc;
@@ -396,6 +488,7 @@ __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
         __builtin_amdgcn_s_setprio(0);
         if (kt + 1 < nk) sstore(cur ^ 1);
         __syncthreads();
+    }
     }
 
     // ---- epilogue: displacement binning (FP64 keeps the residual word)
@@ -894,11 +987,6 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
     }
 }
 
-// s_waitcnt immediates (gfx9 encoding: vmcnt[3:0], expcnt[6:4], lgkmcnt[11:8], vmcnt[5:4]
-// in [15:14]) that wait on one counter only
-constexpr int wait_vmcnt(int n) { return (n & 15) | (7 << 4) | (15 << 8) | ((n >> 4) << 14); }
-constexpr int kWaitLgkm0 = 15 | (7 << 4) | (3 << 14);
-
 // The split-precision GEMM with operand tiles moved by direct global -> LDS loads
 // (buffer_load_dwordx4 ... lds: 64 lanes x 16 B into 1 KiB of LDS at a wave-uniform base)
 // through a ring of NS stages, so no staging registers and no LDS stores are needed
@@ -1154,13 +1242,26 @@ static const SqtCfg kKmajorCfgs[] = {
     {2, 2, 8, "128x128 tiles, 64x64 per wave, group 8"},
     {2, 2, 16, "128x128 tiles, 64x64 per wave, group 16"},
     {2, 2, 4, "128x128 tiles, 64x64 per wave, group 4"},
+    {2, 2, 8, "128x128 tiles, 64x64 per wave, 4-stage direct-to-LDS ring, group 8", 2, 2, 4},
+    {2, 2, 16, "128x128 tiles, 64x64 per wave, 4-stage direct-to-LDS ring, group 16", 2, 2, 4},
 };
 constexpr int kNumSplit = sizeof(kSplitCfgs) / sizeof(kSplitCfgs[0]);
 constexpr int kNumKmajor = sizeof(kKmajorCfgs) / sizeof(kKmajorCfgs[0]);
+// the direct-to-LDS ring moves FP64 rows only: FP32 is offered the first three
+constexpr int kNumKmajorFp32 = 3;
+
+// the configuration an untuned (L, K) takes: the one the autotuner picks on most lattices (the
+// direct-to-LDS rings; FP32 has none)
+static_assert(kNumKmajor > 3 && kNumSplitFp16 > 5, "the default configurations exist for every precision");
+static int default_config(asd_precision prec)
+{
+    return prec == ASD_FP64 ? 3 : prec == ASD_FP32 ? 0 : 5;
+}
 
 int sqt_num_configs(asd_precision prec)
 {
-    return prec <= ASD_FP32 ? kNumKmajor : (prec == ASD_FP16X2 ? kNumSplitFp16 : kNumSplit);
+    return prec == ASD_FP64 ? kNumKmajor
+                            : prec == ASD_FP32 ? kNumKmajorFp32 : (prec == ASD_FP16X2 ? kNumSplitFp16 : kNumSplit);
 }
 const char *sqt_config_name(asd_precision prec, int cfg)
 {
@@ -1480,7 +1581,7 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
 {
     if (cfg < 0) {
         auto it = ctx->tuned[prec].find(std::make_pair(L, 3 * T0));
-        cfg = it != ctx->tuned[prec].end() ? it->second : 0;
+        cfg = it != ctx->tuned[prec].end() ? it->second : default_config(prec);
     }
     ASD_CHECK(cfg < sqt_num_configs(prec), "configuration %d out of range", cfg);
     const int Ns = L * L;
@@ -1548,7 +1649,9 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ga.ld = ldi;
         ga.Kpad = (int)Kpad;
         ASD_TRY(mark(ctx, 1));
-        GemmKernel k = prec == ASD_FP64 ? sqt_gemm_kmajor_kernel<double> : sqt_gemm_kmajor_kernel<float>;
+        GemmKernel k = prec == ASD_FP32 ? sqt_gemm_kmajor_kernel<float>
+                       : kKmajorCfgs[cfg].dma ? sqt_gemm_kmajor_kernel<double, 1>
+                                              : sqt_gemm_kmajor_kernel<double>;
         // a capped grid of many more blocks than fit at once measured 2-4% faster than the
         // persistent grid for this kernel (FP32 / FP64 at L = 256)
         const int nblocks = (int)std::min(num_slots((long)nt_tiles * nt_tiles, nlags), kMaxGrid);

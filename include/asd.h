@@ -227,7 +227,9 @@ int asd_autotune_sqt(asd_ctx *ctx, int L, int T0, asd_precision prec, int *best_
 /* number of tile configurations for a precision, and a printable name */
 int asd_sqt_num_configs(asd_precision prec);
 const char *asd_sqt_config_name(asd_precision prec, int cfg);
-/* force a configuration for a precision (-1 = automatic) */
+/* force a configuration for a precision (-1 = automatic: the one the autotuner recorded
+ * for (L, K), else the precision's default, the direct-to-LDS ring kernel the autotuner
+ * picks on most lattices: FP64 config 3, BF16X3 / FP16X2 config 5, FP32 config 0) */
 int asd_sqt_set_config(asd_ctx *ctx, asd_precision prec, int cfg);
 
 /* ------------------------------------------------------------ adjoint --- */

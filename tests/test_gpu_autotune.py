@@ -51,6 +51,6 @@ def test_autotuned_choice_is_recorded_and_used(gctx, orc, prec, bound):
     mx, rms = err_stats(auto, ref)
     say(f"  against the FP64 oracle: max {mx:.3e} rms {rms:.3e}")
     assert mx <= bound
-    # another (L, K) is not affected by this choice: it still runs (configuration 0 or its own)
+    # another (L, K) is not affected by this choice: it still runs (its default configuration or its own)
     other = gctx.sqt(L, snaps, T0 - 1, nl, q, prec)
     assert np.all(np.isfinite(other))

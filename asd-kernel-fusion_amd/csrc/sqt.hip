@@ -319,8 +319,9 @@ template <> struct KMajorMfma<float> {
 };
 
 // DMA = 0: operand rows staged through registers, two LDS stages of 16 k rows.
-// DMA = 1 (FP64): rows moved by direct global -> LDS loads (buffer_load_dwordx4 ... lds, one
-// 64-lane load fills one 128-element row) through a ring of 4 stages of 8 k rows; each step
+// DMA = 1: rows moved by direct global -> LDS loads (buffer_load_dwordx4 ... lds: 16 B per lane,
+// one load fills one 128-element row, FP64 with 64 lanes, FP32 with 32) through a ring of 4 stages
+// of 8 k rows; each step
 // reads the next stage's first fragments after its MFMAs, so the MFMAs after a barrier start
 // on operands already in registers.
 template <class T, int DMA = 0>
@@ -329,7 +330,6 @@ __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
     // rows padded by 16 elements: the four row groups of a fragment read (lane / 16) then fall on
     // disjoint banks for FP32 and pairwise disjoint for FP64 (FP32 0.7% faster, FP64 unchanged)
     constexpr int BK = 16, PAD = 16, LDS_LD = TILE + PAD;
-    static_assert(DMA == 0 || std::is_same<T, double>::value, "the direct-to-LDS ring moves FP64 rows");
     typedef typename KMajorMfma<T>::acc_t acc_t;
     __shared__ __attribute__((aligned(16))) unsigned char smem_raw[2 * 2 * BK * LDS_LD * sizeof(T) > 8 * 4096
                                                                        ? 2 * 2 * BK * LDS_LD * sizeof(T)
@@ -366,6 +366,8 @@ __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
         constexpr int DK = 8, NS = 4, PER_STAGE = 4;  // k rows per stage, stages, loads per wave and stage
         static_assert(NS * 2 * DK * LDS_LD * sizeof(T) <= sizeof(smem_raw), "ring exceeds the LDS array");
         static_assert(DK == 2 * 4, "each of the 4 waves loads 2 A rows and 2 B rows per stage");
+        constexpr int LPR = TILE * (int)sizeof(T) / 16;  // lanes that fill one row (16 B each)
+        static_assert(LPR == 64 || LPR == 32, "a row is one load of 32 or 64 lanes");
         T *ring = reinterpret_cast<T *>(smem_raw);
         const int nkd = args.Kpad / DK;
         const int wv = __builtin_amdgcn_readfirstlane(wave);
@@ -377,16 +379,18 @@ __global__ __launch_bounds__(256, 2) void sqt_gemm_kmajor_kernel(GemmArgs args)
                 __builtin_amdgcn_make_buffer_rsrc((void *)(Ap + kb), 0, 0x7fffffff, 0x00020000);
             const __amdgpu_buffer_rsrc_t rB =
                 __builtin_amdgcn_make_buffer_rsrc((void *)(Bp + kb), 0, 0x7fffffff, 0x00020000);
+            if (LPR == 64 || lane < LPR) {  // FP32: the upper half-wave is masked off (it writes nothing)
 #pragma unroll
-            for (int u = 0; u < 2; u++) {
-                const int r = 2 * wv + u;
-                const int so = (int)(r * ld * (long)sizeof(T));
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    rA, (__attribute__((address_space(3))) void *)(ring + ((2 * st + 0) * DK + r) * LDS_LD), 16,
-                    16 * lane, so, 0, 0);
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    rB, (__attribute__((address_space(3))) void *)(ring + ((2 * st + 1) * DK + r) * LDS_LD), 16,
-                    16 * lane, so, 0, 0);
+                for (int u = 0; u < 2; u++) {
+                    const int r = 2 * wv + u;
+                    const int so = (int)(r * ld * (long)sizeof(T));
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        rA, (__attribute__((address_space(3))) void *)(ring + ((2 * st + 0) * DK + r) * LDS_LD), 16,
+                        16 * lane, so, 0, 0);
+                    __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                        rB, (__attribute__((address_space(3))) void *)(ring + ((2 * st + 1) * DK + r) * LDS_LD), 16,
+                        16 * lane, so, 0, 0);
+                }
             }
         };
         auto frag = [&](int st, int kk, T(&fa)[4], T(&fb)[4]) {
@@ -1247,21 +1251,18 @@ static const SqtCfg kKmajorCfgs[] = {
 };
 constexpr int kNumSplit = sizeof(kSplitCfgs) / sizeof(kSplitCfgs[0]);
 constexpr int kNumKmajor = sizeof(kKmajorCfgs) / sizeof(kKmajorCfgs[0]);
-// the direct-to-LDS ring moves FP64 rows only: FP32 is offered the first three
-constexpr int kNumKmajorFp32 = 3;
 
 // the configuration an untuned (L, K) takes: the one the autotuner picks on most lattices (the
-// direct-to-LDS rings; FP32 has none)
+// direct-to-LDS rings)
 static_assert(kNumKmajor > 3 && kNumSplitFp16 > 5, "the default configurations exist for every precision");
 static int default_config(asd_precision prec)
 {
-    return prec == ASD_FP64 ? 3 : prec == ASD_FP32 ? 0 : 5;
+    return prec <= ASD_FP32 ? 3 : 5;
 }
 
 int sqt_num_configs(asd_precision prec)
 {
-    return prec == ASD_FP64 ? kNumKmajor
-                            : prec == ASD_FP32 ? kNumKmajorFp32 : (prec == ASD_FP16X2 ? kNumSplitFp16 : kNumSplit);
+    return prec <= ASD_FP32 ? kNumKmajor : (prec == ASD_FP16X2 ? kNumSplitFp16 : kNumSplit);
 }
 const char *sqt_config_name(asd_precision prec, int cfg)
 {
@@ -1649,9 +1650,9 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ga.ld = ldi;
         ga.Kpad = (int)Kpad;
         ASD_TRY(mark(ctx, 1));
-        GemmKernel k = prec == ASD_FP32 ? sqt_gemm_kmajor_kernel<float>
-                       : kKmajorCfgs[cfg].dma ? sqt_gemm_kmajor_kernel<double, 1>
-                                              : sqt_gemm_kmajor_kernel<double>;
+        const bool dma = kKmajorCfgs[cfg].dma != 0;
+        GemmKernel k = prec == ASD_FP32 ? (dma ? sqt_gemm_kmajor_kernel<float, 1> : sqt_gemm_kmajor_kernel<float>)
+                                        : (dma ? sqt_gemm_kmajor_kernel<double, 1> : sqt_gemm_kmajor_kernel<double>);
         // a capped grid of many more blocks than fit at once measured 2-4% faster than the
         // persistent grid for this kernel (FP32 / FP64 at L = 256)
         const int nblocks = (int)std::min(num_slots((long)nt_tiles * nt_tiles, nlags), kMaxGrid);

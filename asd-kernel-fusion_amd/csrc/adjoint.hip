@@ -32,6 +32,7 @@
 // factorisation e^{-iq.(r_i - r_j)} = e^{-iq.r_i} e^{iq.r_j} that the forward
 // epilogue's displacement bins evaluate.
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <vector>
 
@@ -322,9 +323,9 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
     double *cps = snaps + (long)nsnap * n;
     double *seg = cps + ncp * n;
     double *Y2 = seg + (long)C * n, *Y3 = Y2 + n, *Y4 = Y3 + n, *cc = Y4 + n;
-    double *lamS = cc + n;      // dL/dS of the current step
+    double *lamS = cc + n;      // dL/dS of one step; lam and lamS alternate within a segment
     double *scratch = lamS + n;  // the stage kernel's renormalised state (unused)
-    double *lam = scratch + n;   // dL/dS of the step after the current one
+    double *lam = scratch + n;   // dL/dS at the start and end of every segment
     double *dX = lam + n;
     double *gacc = dX + (long)nsnap * n;
     double *dgrad = gacc + 4L * nblk;
@@ -404,17 +405,21 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
                            (const int *)dsg);
         for (long m = 1; m < len; m++)
             ASD_TRY(rk4_launch_step(c, p, L, 1, seg + (m - 1) * n, seg + m * n, nullptr, nullptr, 0, 0));
+        // lam and lamS alternate as the step's input and output; an odd segment ends with a copy,
+        // so every segment starts and ends with dL/dS in lam
+        double *lin = lam, *lout = lamS;
         for (long m = len - 1; m >= 0; m--) {
             const double *Sn = seg + m * n;
             ASD_TRY(rk4_stages_store(c, p, L, Sn, Y2, Y3, Y4, cc, scratch));
-            // lam holds dL/dS_{n+1}; dL/dS_n goes to lamS, with the snapshot adjoint of step
+            // lin holds dL/dS_{n+1}; dL/dS_n goes to lout, with the snapshot adjoint of step
             // sg C + m added when that step is a snapshot step
             hipLaunchKernelGGL(adj_step_kernel, tiles, dim3(kAdjThreads), 0, s2, p, L, Sn, (const double *)Y2,
-                               (const double *)Y3, (const double *)Y4, (const double *)cc, (const double *)lam, lamS,
+                               (const double *)Y3, (const double *)Y4, (const double *)cc, (const double *)lin, lout,
                                gacc, (const double *)dX, (const int *)dsg, C / every,
                                m % every == 0 ? (int)(m / every) : -1, nsnap);
-            hipLaunchKernelGGL(copy_kernel, g1(n), dim3(256), 0, s2, n, lam, (const double *)lamS);
+            std::swap(lin, lout);
         }
+        if (lin != lam) hipLaunchKernelGGL(copy_kernel, g1(n), dim3(256), 0, s2, n, lam, (const double *)lin);
         hipLaunchKernelGGL(step_index_kernel, dim3(1), dim3(1), 0, s2, dsg, -1);
         ASD_HIP(hipGetLastError());
         return ASD_OK;
@@ -427,8 +432,8 @@ static int sqt_vjp_exact(asd_ctx *c, const asd_params *pp, int L, const double *
         nfull--;
     }
     if (nfull >= 2) {
-        // the full segments replay one captured graph (four small launches a step: the
-        // recomputed state, the stage values, the reverse step, the copy; launch-bound)
+        // the full segments replay one captured graph (three small launches a step: the
+        // recomputed state, the stage values, the reverse step; launch-bound)
         hipStream_t cap;
         ASD_HIP(hipStreamCreateWithFlags(&cap, hipStreamNonBlocking));
         hipGraph_t graph;

@@ -205,7 +205,8 @@ def test_every_kernel_config_matches_oracle(gctx, orc, prec):
     one smaller than a tile (144), so partial tiles and masked rows are covered.
     K = 3 T0 covers one whole 32-deep k step (18, 21, 27), a last step whose second
     16-deep half lies in the zero padding after zero, one and two whole steps (15, 48,
-    66; the direct-to-LDS kernels do only its first half).
+    66; the direct-to-LDS kernels do only its first half); the FP64 / FP32 ring of 8-deep
+    stages sees 2, 4, 6 and 10 stages, the first fewer than the 3 it requests up front.
     Each output element is accumulated over k in the same order by every layout,
     so the layouts agree with each other up to the order of the FP64 bin sums."""
     torch = need_gpu()
@@ -231,6 +232,32 @@ def test_every_kernel_config_matches_oracle(gctx, orc, prec):
                 assert ec <= 1e-12, (L, cfg, name, ec)
     finally:
         gctx.set_config(prec, -1)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_direct_to_lds_ring_equals_register_staged_at_full_depth(gctx, prec):
+    """At the plain L = 128 workload's shape (K = 1200: 150 stages of the direct-to-LDS ring,
+    16384 x 16384 per lag), the ring kernel (configuration 3, the default) and the
+    register-staged kernel (configuration 0) give the same bits: the k order of every
+    product is the same and the bins are integer sums."""
+    torch = need_gpu()
+    L, T0, nl = 128, 400, 3
+    g = torch.Generator(device="cuda")
+    g.manual_seed(5)
+    snaps = torch.randn((T0 + nl - 1, L * L, 3), dtype=torch.float64, device="cuda", generator=g)
+    snaps /= torch.linalg.norm(snaps, dim=-1, keepdim=True)
+    q = gxmg(L)
+    try:
+        out = []
+        for cfg in (0, 3):
+            gctx.set_config(prec, cfg)
+            out.append(gctx.sqt(L, snaps, T0, nl, q, prec))
+            say(f"sqt {prec} L={L} K={3 * T0} config {cfg}: sum |S| {np.abs(out[-1]).sum():.15e}")
+    finally:
+        gctx.set_config(prec, -1)
+    assert np.all(np.isfinite(out[0])) and np.abs(out[0]).max() > 0
+    assert np.array_equal(out[0], out[1])
 
 
 @pytest.mark.gpu

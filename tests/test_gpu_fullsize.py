@@ -110,7 +110,7 @@ PRECISION_BOUNDS = {"fp32": 1e-5, "fp16x2": 1e-5, "bf16x3": 1.5e-4}
 @pytest.mark.gpu
 def test_precision_L256_full_grid_against_separable_identity(gctx):
     L, T0, nl = 256, 200, 200
-    say(f"full size: L={L}, 4000 steps, T0={T0}, lags={nl}, full grid, fp32 / bf16x3 / fp16x2")
+    say(f"full size: L={L}, 4000 steps, T0={T0}, lags={nl}, full grid, fp32 / bf16x3 / fp16x2 / fp64")
     snaps = run_snapshots(gctx, L, 2, 4000, 10, J=1.0, D=0.05, Bz=0.2)
     q = [(a, b) for a in range(L) for b in range(L)]
     lags = [0, 1, 100, 199]
@@ -120,3 +120,8 @@ def test_precision_L256_full_grid_against_separable_identity(gctx):
         mx, rms = err_stats(got, ref)
         say(f"  {prec}: max {mx:.3e} rms {rms:.3e} at lags {lags} vs the separable identity (bound {bound:g})")
         assert mx <= bound
+    # FP64 at the same depth (K = 600), lags 0 and 1 (all 200 lags take 14 s)
+    got = gctx.sqt(L, snaps, T0, 2, q, "fp64")
+    mx, rms = err_stats(got, ref[:2])
+    say(f"  fp64: max {mx:.3e} rms {rms:.3e} at lags [0, 1] vs the separable identity (bound 1e-10)")
+    assert mx <= 1e-10

@@ -24,8 +24,8 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_DIR = os.path.dirname(_HERE)
 LIB_PATH = os.path.join(PKG_DIR, "libasd.so")
 
-FP64, FP32, BF16X3, FP16X2 = 0, 1, 2, 3
-PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "fp16x2": FP16X2}
+FP64, FP32, BF16X3, FP16X2, FP16F8 = 0, 1, 2, 3, 4
+PRECISIONS = {"fp64": FP64, "fp32": FP32, "bf16x3": BF16X3, "fp16x2": FP16X2, "fp16f8": FP16F8}
 
 ASD_OK = 0
 ASD_EINVAL, ASD_ENOMEM, ASD_EHIP, ASD_ENOTSUP, ASD_ENOTFINITE = -1, -2, -3, -4, -5

@@ -495,7 +495,7 @@ int asd_sqt(asd_ctx *c, int L, const double *d_snaps, int nsnap, int T0, int nla
     ASD_CHECK(c && d_snaps && out, "NULL argument");
     ASD_TRY(check_L(L));
     ASD_CHECK(T0 >= 1 && nlags >= 1, "T0 and nlags must be >= 1");
-    ASD_CHECK(prec >= ASD_FP64 && prec <= ASD_FP16X2, "unknown precision");
+    ASD_CHECK(prec >= ASD_FP64 && prec <= ASD_FP16F8, "unknown precision");
     ASD_TRY(check_q(L, h_q, nq));
     ASD_HIP(hipSetDevice(c->device));
     double *dout = out;
@@ -556,7 +556,7 @@ int asd_sqt_replicas(asd_ctx *c, int L, int nrep, const double *d_now, const dou
     ASD_CHECK(c && d_now && d_zero && out, "NULL argument");
     ASD_TRY(check_L(L));
     ASD_CHECK(nrep >= 1, "nrep must be >= 1");
-    ASD_CHECK(prec >= ASD_FP64 && prec <= ASD_FP16X2, "unknown precision");
+    ASD_CHECK(prec >= ASD_FP64 && prec <= ASD_FP16F8, "unknown precision");
     ASD_TRY(check_q(L, h_q, nq));
     ASD_HIP(hipSetDevice(c->device));
     ASD_TRY(c->outdev.ensure(sizeof(double) * 2 * (long)nq));
@@ -569,15 +569,15 @@ int asd_sqt_replicas(asd_ctx *c, int L, int nrep, const double *d_now, const dou
 
 int asd_sqt_num_configs(asd_precision prec)
 {
-    return (prec >= ASD_FP64 && prec <= ASD_FP16X2) ? sqt_num_configs(prec) : 0;
+    return (prec >= ASD_FP64 && prec <= ASD_FP16F8) ? sqt_num_configs(prec) : 0;
 }
 const char *asd_sqt_config_name(asd_precision prec, int cfg)
 {
-    return (prec >= ASD_FP64 && prec <= ASD_FP16X2) ? sqt_config_name(prec, cfg) : "invalid";
+    return (prec >= ASD_FP64 && prec <= ASD_FP16F8) ? sqt_config_name(prec, cfg) : "invalid";
 }
 int asd_sqt_set_config(asd_ctx *c, asd_precision prec, int cfg)
 {
-    ASD_CHECK(c && prec >= ASD_FP64 && prec <= ASD_FP16X2, "bad arguments");
+    ASD_CHECK(c && prec >= ASD_FP64 && prec <= ASD_FP16F8, "bad arguments");
     ASD_CHECK(cfg >= -1 && cfg < asd_sqt_num_configs(prec), "config out of range");
     c->forced_cfg[prec] = cfg;
     return ASD_OK;
@@ -586,7 +586,7 @@ int asd_autotune_sqt(asd_ctx *c, int L, int T0, asd_precision prec, int *best, d
 {
     ASD_CHECK(c, "ctx NULL");
     ASD_TRY(check_L(L));
-    ASD_CHECK(T0 >= 1 && prec >= ASD_FP64 && prec <= ASD_FP16X2, "bad arguments");
+    ASD_CHECK(T0 >= 1 && prec >= ASD_FP64 && prec <= ASD_FP16F8, "bad arguments");
     ASD_HIP(hipSetDevice(c->device));
     return sqt_autotune(c, L, T0, prec, best, times_ms);
 }

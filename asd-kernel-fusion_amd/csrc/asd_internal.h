@@ -122,8 +122,8 @@ struct asd_ctx {
     int batch_variant = -1;
     std::map<std::vector<int>, int> batch_tuned;
     // autotuner choice per precision (-1 = automatic); tuned per (L, K)
-    int forced_cfg[4] = {-1, -1, -1, -1};
-    std::map<std::pair<int, int>, int> tuned[4];
+    int forced_cfg[5] = {-1, -1, -1, -1, -1};
+    std::map<std::pair<int, int>, int> tuned[5];
     // RK4 tile edge: 0 = automatic; 8, 16 or 32 (LDS tiles) or 64 (register streaming)
     int rk4_tile = 0;
     // RK4 arithmetic: 1 = the reference's operation order (IEEE-equal values), 0 = the tolerance mode

@@ -36,6 +36,8 @@ typedef float f4 __attribute__((ext_vector_type(4)));
 typedef float f16v __attribute__((ext_vector_type(16)));
 typedef __bf16 bf8 __attribute__((ext_vector_type(8)));
 typedef _Float16 h8 __attribute__((ext_vector_type(8)));
+typedef int v8i __attribute__((ext_vector_type(8)));
+typedef int i32x4 __attribute__((ext_vector_type(4)));
 // 16-byte staging vector: a native clang vector (HIP's uint4 is a struct with union
 // members, which arrays of do not get promoted to registers and end up in scratch)
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
@@ -172,11 +174,28 @@ __global__ void pack_kmajor_kernel(const double *__restrict__ snaps, long tstrid
 // window starting at any k offset o is read from copy o % 8 at the k-block (o - o % 8) / 8.
 // nkb: k-blocks per row group (rows >= Ns are zeros).  grid: (row groups of 32 rows,
 // k-blocks); thread t: row 32 blockIdx.x + t / 8, k % 8 = t % 8.
+// e4m3 (OCP FP8: 4 exponent bits, bias 7, 3 mantissa bits) of a float with |x| <= 448, rounded to
+// nearest even: subnormals (|x| < 2^-6) are multiples of 2^-9, whose count rint(|x| 2^9) is also
+// the code (8 lands on the smallest normal, code 0x08); normals keep the top 3 mantissa bits
+// after a round-to-nearest-even add on the float's bits, a carry moving into the exponent.
+__device__ __forceinline__ uint32_t to_e4m3(float x)
+{
+    const uint32_t sgn = (__float_as_uint(x) >> 24) & 0x80u;
+    const float ax = __builtin_fabsf(x);
+    if (ax < 0x1p-6f) return sgn | (uint32_t)__builtin_rintf(ax * 512.0f);
+    uint32_t a = __float_as_uint(ax);
+    a += 0x7FFFFu + ((a >> 20) & 1u);
+    return sgn | ((a >> 20) - (120u << 3));
+}
+
+// FP16F8 writes, in place of the lo half, the cross-product bytes of the element: A (bside 0)
+// e4m3(hi) in the low byte and e4m3(lo 2^11) in the high byte, B (bside 1) the other way round,
+// so that A's hi byte meets B's lo byte and A's lo byte B's hi byte in the fp8 MFMA.
 template <int PREC>
 __global__ __launch_bounds__(256) void pack_split_kernel(const double *__restrict__ snaps, long tstride, int nt,
                                                          int Ns, uint16_t *__restrict__ hi,
                                                          uint16_t *__restrict__ lo, long nkb, long copy_stride,
-                                                         int nshift)
+                                                         int nshift, int bside)
 {
     const int i = blockIdx.x * 32 + threadIdx.x / 8, e = threadIdx.x % 8;
     const long kb = blockIdx.y;
@@ -193,12 +212,18 @@ __global__ __launch_bounds__(256) void pack_split_kernel(const double *__restric
             __hip_bfloat16 bl = __float2bfloat16(rr);
             h = *reinterpret_cast<uint16_t *>(&bh);
             l = *reinterpret_cast<uint16_t *>(&bl);
-        } else {
+        } else if (PREC == ASD_FP16X2) {
             __half fh = __float2half(v);
             const float rr = (v - __half2float(fh)) * 2048.0f;
             __half fl = __float2half(rr);
             h = *reinterpret_cast<uint16_t *>(&fh);
             l = *reinterpret_cast<uint16_t *>(&fl);
+        } else {
+            __half fh = __float2half(v);
+            const float hv = __half2float(fh);
+            const uint32_t xh = to_e4m3(hv), xl = to_e4m3((v - hv) * 2048.0f);
+            h = *reinterpret_cast<uint16_t *>(&fh);
+            l = (uint16_t)(bside ? (xl | xh << 8) : (xh | xl << 8));
         }
         hi[r * copy_stride + o] = h;
         lo[r * copy_stride + o] = l;
@@ -654,6 +679,17 @@ template <> struct SplitMfma<ASD_FP16X2> {
         return __builtin_amdgcn_mfma_f32_32x32x16_f16(a, b, c, 0, 0, 0);
     }
     static constexpr bool dual = true;
+};
+// FP16F8: the hi products as FP16X2; the cross products on v_mfma_f32_32x32x64_f8f6f4 with
+// e4m3 operands (format 0 for A and B, no block scales).  Lane l supplies 32 bytes of row
+// (A) or column (B) l % 32; byte e of lane l in A meets byte e of lane l' in B when l / 32 ==
+// l' / 32 (checked with a lane-layout probe on gfx950), so any assignment of k slots to bytes
+// that A and B share gives the same sum.
+template <> struct SplitMfma<ASD_FP16F8> : SplitMfma<ASD_FP16X2> {
+    static __device__ __forceinline__ f16v mfma8(v8i a, v8i b, f16v c)
+    {
+        return __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(a, b, c, 0, 0, 0, 0, 0, 0);
+    }
 };
 
 // Epilogue of the 32x32x16 split kernels: bins one wave's accumulators (rows
@@ -1134,10 +1170,65 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
         // 320.5 -> 310.5 ms per 200 lags)
         __builtin_amdgcn_s_setprio(1);
         const uint16_t *sb = lds + h * STAGE;
+        const uint16_t *sr = sb + rd_off;
+        if constexpr (PREC == ASD_FP16F8) {
+            // the hi halves of the step's 16-deep halves (the hh products on the fp16 MFMA), then
+            // the cross bytes: lane l takes k-blocks l / 32 and 2 + l / 32 of its row, 32 bytes,
+            // for one fp8 MFMA over the whole 32-deep stage.  In a half step (KH = 1) the second
+            // half's cross bytes are read too: B is zero there, so those products vanish.
+            v_t ah[KH][WM], bh[KH][WN];
+            i32x4 ax[2][WM], bx[2][WN];
+#pragma unroll
+            for (int kk = 0; kk < KH; kk++) {
+                const int qo = 2 * kk * 16 * 8;
+#pragma unroll
+                for (int m = 0; m < WM; m++)
+                    ah[kk][m] = *reinterpret_cast<const v_t *>(sr + (wi * WM + m) * 1024 + qo);
+#pragma unroll
+                for (int n = 0; n < WN; n++)
+                    bh[kk][n] = *reinterpret_cast<const v_t *>(sr + OFF_BH + (wj * WN + n) * 1024 + qo);
+            }
+#pragma unroll
+            for (int kk = 0; kk < 2; kk++) {
+                const int qo = 2 * kk * 16 * 8;
+#pragma unroll
+                for (int n = 0; n < WN; n++)
+                    bx[kk][n] = *reinterpret_cast<const i32x4 *>(sr + OFF_BL + (wj * WN + n) * 1024 + qo);
+#pragma unroll
+                for (int m = 0; m < WM; m++)
+                    ax[kk][m] = *reinterpret_cast<const i32x4 *>(sr + OFF_AL + (wi * WM + m) * 1024 + qo);
+            }
+#pragma unroll
+            for (int kk = 0; kk < KH; kk++)
+#pragma unroll
+                for (int m = 0; m < WM; m++)
+#pragma unroll
+                    for (int n = 0; n < WN; n++)
+                        acc[m][n] = SplitMfma<PREC>::mfma(ah[kk][m], bh[kk][n], acc[m][n]);
+#pragma unroll
+            for (int m = 0; m < WM; m++)
+#pragma unroll
+                for (int n = 0; n < WN; n++) {
+                    const v8i a8 = __builtin_shufflevector(ax[0][m], ax[1][m], 0, 1, 2, 3, 4, 5, 6, 7);
+                    const v8i b8 = __builtin_shufflevector(bx[0][n], bx[1][n], 0, 1, 2, 3, 4, 5, 6, 7);
+                    acc2[m][n] = SplitMfma<PREC>::mfma8(a8, b8, acc2[m][n]);
+                }
+            {
+                constexpr int R0 = WM + WN, RT = (KH + 2) * (WM + WN), MT = (KH + 1) * WM * WN;
+                constexpr int NI = MT < RT - R0 ? MT : RT - R0;
+                __builtin_amdgcn_sched_group_barrier(0x100, R0, 0);
+#pragma unroll
+                for (int i = 0; i < NI; i++) {
+                    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);
+                    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);
+                }
+                if constexpr (RT - R0 > NI) __builtin_amdgcn_sched_group_barrier(0x100, RT - R0 - NI, 0);
+                if constexpr (MT > NI) __builtin_amdgcn_sched_group_barrier(0x008, MT - NI, 0);
+            }
+        } else {
         // reads in the order the MFMAs use them: per 16-deep half A hi, B hi (the hh
         // products), B lo (hl), A lo (lh)
         v_t ah[KH][WM], al[KH][WM], bh[KH][WN], bl[KH][WN];
-        const uint16_t *sr = sb + rd_off;
 #pragma unroll
         for (int kk = 0; kk < KH; kk++) {
             const int qo = 2 * kk * 16 * 8;  // quarter 2 kk (+ l / 32 in rd_off)
@@ -1196,6 +1287,7 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
             }
             if constexpr (MT > RT - R0) __builtin_amdgcn_sched_group_barrier(0x008, MT - (RT - R0), 0);
         }
+        }
         __builtin_amdgcn_s_setprio(0);
         asm volatile("" ::: "memory");
     };
@@ -1242,6 +1334,9 @@ static const SqtCfg kSplitCfgs[] = {
 // FP16X2 keeps two accumulators: the 64x128 / 128x64 register-staged wave tiles do not
 // fit the register file, so only the first six are offered
 constexpr int kNumSplitFp16 = 6;
+// FP16F8 runs on the direct-to-LDS kernels only: its configurations, as rows of kSplitCfgs
+static const int kF8Cfgs[] = {5, 3, 4};
+constexpr int kNumF8 = sizeof(kF8Cfgs) / sizeof(kF8Cfgs[0]);
 static const SqtCfg kKmajorCfgs[] = {
     {2, 2, 8, "128x128 tiles, 64x64 per wave, group 8"},
     {2, 2, 16, "128x128 tiles, 64x64 per wave, group 16"},
@@ -1257,17 +1352,19 @@ constexpr int kNumKmajor = sizeof(kKmajorCfgs) / sizeof(kKmajorCfgs[0]);
 static_assert(kNumKmajor > 3 && kNumSplitFp16 > 5, "the default configurations exist for every precision");
 static int default_config(asd_precision prec)
 {
-    return prec <= ASD_FP32 ? 3 : 5;
+    return prec <= ASD_FP32 ? 3 : prec == ASD_FP16F8 ? 0 : 5;
 }
 
 int sqt_num_configs(asd_precision prec)
 {
-    return prec <= ASD_FP32 ? kNumKmajor : (prec == ASD_FP16X2 ? kNumSplitFp16 : kNumSplit);
+    return prec <= ASD_FP32 ? kNumKmajor
+                            : prec == ASD_FP16X2 ? kNumSplitFp16 : prec == ASD_FP16F8 ? kNumF8 : kNumSplit;
 }
 const char *sqt_config_name(asd_precision prec, int cfg)
 {
     if (cfg < 0 || cfg >= sqt_num_configs(prec)) return "invalid";
-    return prec <= ASD_FP32 ? kKmajorCfgs[cfg].name : kSplitCfgs[cfg].name;
+    return prec <= ASD_FP32 ? kKmajorCfgs[cfg].name
+                            : prec == ASD_FP16F8 ? kSplitCfgs[kF8Cfgs[cfg]].name : kSplitCfgs[cfg].name;
 }
 
 using GemmKernel = void (*)(GemmArgs);
@@ -1299,7 +1396,14 @@ static void launch_split(const SqtCfg &c, GemmArgs ga, hipStream_t st, int devic
     const int threads = 64 * c.wvm * c.wvn;
     ga.group = c.group;
     GemmKernel k;
-    if (c.dma == 4 && c.wvm == 4)
+    if constexpr (PREC == ASD_FP16F8) {  // the direct-to-LDS kernels only
+        if (c.dma == 4 && c.wvm == 4)
+            k = sqt_gemm_dma_kernel<PREC, 1, 2, 4, 2, 4>;
+        else if (c.dma == 4)
+            k = sqt_gemm_dma_kernel<PREC, 2, 1, 2, 4, 4>;
+        else
+            k = sqt_gemm_dma_kernel<PREC, 2, 2, 4, 2, 3>;
+    } else if (c.dma == 4 && c.wvm == 4)
         k = sqt_gemm_dma_kernel<PREC, 1, 2, 4, 2, 4>;
     else if (c.dma == 4)
         k = sqt_gemm_dma_kernel<PREC, 2, 1, 2, 4, 4>;
@@ -1670,11 +1774,13 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ASD_TRY(ctx->opB.ensure(2 * cs));
         ASD_TRY(ctx->opB2.ensure(2 * cs));
         const dim3 pb(256);
-        auto packer = prec == ASD_BF16X3 ? pack_split_kernel<ASD_BF16X3> : pack_split_kernel<ASD_FP16X2>;
+        auto packer = prec == ASD_BF16X3   ? pack_split_kernel<ASD_BF16X3>
+                      : prec == ASD_FP16X2 ? pack_split_kernel<ASD_FP16X2>
+                                           : pack_split_kernel<ASD_FP16F8>;
         hipLaunchKernelGGL(packer, dim3((unsigned)(ldi / 32), (unsigned)nkb), pb, 0, st, d_snaps_a, tstride, nt_a,
-                           Ns, ctx->opA.as<uint16_t>(), ctx->opA2.as<uint16_t>(), nkbs, cs, 8);
+                           Ns, ctx->opA.as<uint16_t>(), ctx->opA2.as<uint16_t>(), nkbs, cs, 8, 0);
         hipLaunchKernelGGL(packer, dim3((unsigned)(ldi / 32), (unsigned)(Kpad / 8)), pb, 0, st, d_snaps_b, tstride,
-                           T0, Ns, ctx->opB.as<uint16_t>(), ctx->opB2.as<uint16_t>(), nkbs, cs, 1);
+                           T0, Ns, ctx->opB.as<uint16_t>(), ctx->opB2.as<uint16_t>(), nkbs, cs, 1, 1);
         ga.copy_stride = cs;
         ga.A = ctx->opA.ptr;
         ga.A2 = ctx->opA2.ptr;
@@ -1685,8 +1791,10 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ASD_TRY(mark(ctx, 1));
         if (prec == ASD_BF16X3)
             launch_split<ASD_BF16X3>(kSplitCfgs[cfg], ga, st, ctx->device);
-        else
+        else if (prec == ASD_FP16X2)
             launch_split<ASD_FP16X2>(kSplitCfgs[cfg], ga, st, ctx->device);
+        else
+            launch_split<ASD_FP16F8>(kSplitCfgs[kF8Cfgs[cfg]], ga, st, ctx->device);
     }
     ASD_HIP(hipGetLastError());
     ASD_TRY(mark(ctx, 2));

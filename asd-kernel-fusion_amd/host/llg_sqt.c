@@ -53,13 +53,14 @@ static int parse_prec(const char *s)
     if (!strcmp(s, "fp32")) return ASD_FP32;
     if (!strcmp(s, "bf16x3")) return ASD_BF16X3;
     if (!strcmp(s, "fp16x2")) return ASD_FP16X2;
+    if (!strcmp(s, "fp16f8")) return ASD_FP16F8;
     fprintf(stderr, "unknown precision %s\n", s);
     exit(1);
 }
 
 static const char *prec_name(int p)
 {
-    static const char *n[] = {"fp64", "fp32", "bf16x3", "fp16x2"};
+    static const char *n[] = {"fp64", "fp32", "bf16x3", "fp16x2", "fp16f8"};
     return n[p];
 }
 
@@ -106,7 +107,7 @@ int main(int argc, char **argv)
         default:
             fprintf(stderr,
                     "usage: %s [-n L -m MCsweep -f Nfile -e dt -t trd -x vx -y vy -Q]  (reference mode)\n"
-                    "       %s -P [-n L -S steps -E every -T T0 -G lags -p fp64|fp32|bf16x3|fp16x2 -F\n"
+                    "       %s -P [-n L -S steps -E every -T T0 -G lags -p fp64|fp32|bf16x3|fp16x2|fp16f8 -F\n"
                     "              -W snapshot_file | -R snapshot_file]\n"
                     "       common: -J J -D D -B Bz -a alpha -l spin_length -s seed -o timing_file -O sqt_file\n",
                     argv[0], argv[0]);

@@ -13,10 +13,12 @@ reference's own arithmetic (its fused_dgemm is an FP64 DGEMM).  `value` is the
 S(q,tau) throughput in algorithmic GEMM flops (sum over lags of 2 Ns^2 K = 2.577e14
 per pass) per second of S(q,tau) time, and `roofline` is the FP64 correlation GEMM
 against the FP64 MFMA peak.  Beside it: RK4 spin-steps/s (and on an HBM-bound batch),
-and the mixed-precision leg -- the same pass with the GEMM in split FP16X2 on MFMA,
-its own roofline (3 MFMA products per useful flop), its max / RMS error against the
-FP64 result on the same snapshots and the end-to-end time ratio FP64 / mixed.  Inputs
-are resident on the GPU before timing.
+and the mixed-precision leg -- the same pass with the GEMM in split FP16F8 on MFMA
+(fp16 hi products, fp8 cross products), its own roofline (3 MFMA products per useful
+flop, costing 2 in fp16 MFMA work), its max / RMS error against the FP64 result on the
+same snapshots and the end-to-end time ratio FP64 / mixed -- with the FP16X2 split
+(3 fp16 products) reported the same way beside it.  Inputs are resident on the GPU
+before timing.
 
 The CPU baseline is the reference's own kernel_fusion code (runge_kutta and
 fused_dgemm from oracle/_ref, built from its sources with its AVX2/FMA flags)
@@ -37,7 +39,10 @@ for _p in (ROOT, PKG):
         sys.path.insert(0, _p)
 
 METRIC = "RK4 spin-steps/s; S(q,t) TFLOP/s at L=128; mixed vs FP64 time @ err≤1e-5"
-PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "fp16x2": 2500.0}
+PEAK_TFLOPS = {"fp64": 78.6, "fp32": 157.3, "bf16x3": 2500.0, "fp16x2": 2500.0, "fp16f8": 2500.0}
+# MFMA products per useful flop, and their cost in dense fp16 MFMA work (fp8 runs at twice the rate)
+PRODUCTS_PER_FLOP = {"fp64": 1, "fp32": 1, "bf16x3": 3, "fp16x2": 3, "fp16f8": {"fp16": 1, "fp8": 2}}
+FP16_WORK_PER_FLOP = {"fp64": 1, "fp32": 1, "bf16x3": 3, "fp16x2": 3, "fp16f8": 2}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -160,7 +165,7 @@ def workload_precision256(args):
     q = asd_amd.full_grid(L)
     flop = nl * 2.0 * ns * ns * 3 * T0
     res, outs = {}, {}
-    for prec in ("fp64", "fp32", "bf16x3", "fp16x2"):
+    for prec in ("fp64", "fp32", "bf16x3", "fp16x2", "fp16f8"):
         ctx.autotune(L, T0, prec)
         out = torch.empty((nl, len(q), 2), dtype=torch.float64, device="cuda")
         ctx.sqt(L, snaps, T0, nl, q, prec, out=out)
@@ -172,7 +177,7 @@ def workload_precision256(args):
         res[prec] = {"sqt_s": round(total / 1e3, 3), "gemm_s": round(st["gemm"] / 1e3, 3),
                      "gemm_tflops": round(flop / st["gemm"] / 1e9, 2)}
         log(f"  {prec}: {res[prec]}")
-    for prec in ("fp32", "bf16x3", "fp16x2"):
+    for prec in ("fp32", "bf16x3", "fp16x2", "fp16f8"):
         res[prec]["err_vs_fp64"] = err_stats(outs[prec], outs["fp64"])
     zbar = snaps[:T0].mean(dim=0).cpu().numpy()
     sum_rule = float(abs(outs["fp64"][0].sum() - (ns - (zbar ** 2).sum())) / ns)
@@ -242,17 +247,19 @@ def workload_batch96(args):
         ctx.rk4_run(plist[t], L, s, nsteps, every, nsnap, snaps[t])
     torch.cuda.synchronize()
     t_single = time.perf_counter() - t0
-    tuned = {}
-    for L in sorted(set(Ls)):
-        best, times = ctx.autotune(L, T0, "fp16x2")
-        tuned[L] = list(times)[best]
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for t, L in enumerate(Ls):
-        ctx.sqt(L, snaps[t], T0, nl, gxmg(L), "fp16x2")
-    torch.cuda.synchronize()
-    t_sqt = time.perf_counter() - t0
     flop = sum(nl * 2.0 * (L * L) ** 2 * 3 * T0 for L in Ls)
+    tuned, t_sqt = {}, {}
+    for prec in ("fp16x2", "fp16f8"):
+        tuned[prec] = {}
+        for L in sorted(set(Ls)):
+            best, times = ctx.autotune(L, T0, prec)
+            tuned[prec][L] = list(times)[best]
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for t, L in enumerate(Ls):
+            ctx.sqt(L, snaps[t], T0, nl, gxmg(L), prec)
+        torch.cuda.synchronize()
+        t_sqt[prec] = time.perf_counter() - t0
     print(json.dumps({"workload": "batch 96 trajectories", "trajectories": len(Ls), "sites_total": total_sites,
                       "rk4_steps": nsteps, "rk4_batched_s": round(t_batch, 4),
                       "rk4_spin_steps_per_s": total_sites * nsteps / t_batch,
@@ -264,7 +271,10 @@ def workload_batch96(args):
                                          "rk4_spin_steps_per_s": total_sites * nsteps / t_fast,
                                          "batch_variant": list(vtimes_f)[vbest_f],
                                          "max_abs_diff_vs_exact_100_steps": fast_diff},
-                      "sqt_fp16x2_s": round(t_sqt, 3), "sqt_tflops": round(flop / t_sqt / 1e12, 2),
+                      "sqt_fp16x2_s": round(t_sqt["fp16x2"], 3),
+                      "sqt_tflops": round(flop / t_sqt["fp16x2"] / 1e12, 2),
+                      "sqt_fp16f8_s": round(t_sqt["fp16f8"], 3),
+                      "sqt_fp16f8_tflops": round(flop / t_sqt["fp16f8"] / 1e12, 2),
                       "autotuned_cfg_per_L": tuned}), flush=True)
 
 
@@ -462,10 +472,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "fp16x2"],
+    ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "bf16x3", "fp16x2", "fp16f8"],
                     help="GEMM precision of the timed (headline) pass; fp64 is the reference's arithmetic")
-    ap.add_argument("--mixed", default="fp16x2", choices=["fp32", "bf16x3", "fp16x2"],
+    ap.add_argument("--mixed", default="fp16f8", choices=["fp32", "bf16x3", "fp16x2", "fp16f8"],
                     help="split precision of the mixed-precision leg")
+    ap.add_argument("--mixed-also", default="fp16x2", choices=["none", "fp32", "bf16x3", "fp16x2", "fp16f8"],
+                    help="a second split precision reported beside the mixed-precision leg")
     ap.add_argument("--L", type=int, default=128)
     ap.add_argument("--nsteps", type=int, default=8000)
     ap.add_argument("--every", type=int, default=10)
@@ -528,8 +540,9 @@ def main():
 
     log(f"bench: L={L} nsteps={args.nsteps} nsnap={nsnap} T0={T0} nlags={nlags} nq={nq} prec={args.precision}")
     mixed = None if args.no_mixed or args.mixed == args.precision else args.mixed
+    also = None if (not mixed or args.mixed_also in ("none", args.precision, mixed)) else args.mixed_also
     tuned = {}
-    for prec in [args.precision] + ([mixed] if mixed else []):
+    for prec in [args.precision] + [m for m in (mixed, also) if m]:
         best, times = ctx.autotune(L, T0, prec)
         tuned[prec] = {"chosen": list(times)[best], "gemm_ms_8_lags": {k: round(v, 3) for k, v in times.items()}}
         log(f"  autotune {prec}: {tuned[prec]}")
@@ -554,7 +567,7 @@ def main():
     value = gemm_flop / (sqt_ms * 1e-3) / 1e12
     rk_rate = args.nsteps * ns / (rk_ms * 1e-3)
     gemm_tflops = gemm_flop / (avg["gemm"] * 1e-3) / 1e12
-    split = args.precision in ("fp16x2", "bf16x3")
+    split = args.precision in ("fp16x2", "bf16x3", "fp16f8")
 
     res = {
         "metric": METRIC, "value": round(value, 3), "unit": "TFLOP/s", "n_gpus": 1,
@@ -571,7 +584,7 @@ def main():
                      "traffic": profiled_traffic(tuned[args.precision]["chosen"], args.precision, L, T0, nlags),
                      "kernel": f"S(q,t) GEMM, {args.precision}, {tuned[args.precision]['chosen']} "
                                f"(all {nlags} lags in one launch)",
-                     "mfma_products_per_flop": 3 if split else 1},
+                     "mfma_products_per_flop": PRODUCTS_PER_FLOP[args.precision]},
         # the reference applies the phase in the GEMM epilogue: 4 flops per G element and q; here the
         # epilogue bins G by displacement (one fixed-point add per element) and the phases act on the L^2
         # real bins per lag (a multiply-add each for cos and sin: 4 flops per bin and q)
@@ -586,12 +599,14 @@ def main():
         "sqt_s": round(sqt_ms / 1e3, 4),
     }
 
-    if mixed:
+    def mixed_leg(prec):
+        """One timed end-to-end pass in split precision `prec` after a warm one: times, the GEMM's
+        roofline and the error against the FP64 result of the timed passes (same snapshots)."""
         out_mix = torch.empty_like(out_main)
-        one_pass(mixed, out_mix)                   # warm the split path (workspace)
+        one_pass(prec, out_mix)                    # warm the split path (workspace)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        stm = one_pass(mixed, out_mix)
+        stm = one_pass(prec, out_mix)
         torch.cuda.synchronize()
         wall_m = time.perf_counter() - t0
         sqtm_ms = stm["pack"] + stm["gemm"] + stm["fourier"] + stm["finish"]
@@ -601,28 +616,39 @@ def main():
         sa, sb = a[..., 0] + 1j * a[..., 1], b[..., 0] + 1j * b[..., 1]
         scale = np.abs(sb).max(axis=1, keepdims=True)
         e = np.abs(sa - sb) / scale
-        res["mixed_precision"] = {
-            "precision": mixed, "sqt_s": round(sqtm_ms / 1e3, 4), "gemm_s": round(stm["gemm"] / 1e3, 4),
+        leg = {
+            "precision": prec, "sqt_s": round(sqtm_ms / 1e3, 4), "gemm_s": round(stm["gemm"] / 1e3, 4),
             "sqt_tflops": round(gemm_flop / sqtm_ms / 1e9, 3), "end_to_end_s": round(wall_m, 4),
-            "roofline": {"bound": "mfma", "achieved": round(gm_tf, 3), "peak": PEAK_TFLOPS[mixed], "unit": "TFLOP/s",
-                         "frac": round(gm_tf / PEAK_TFLOPS[mixed], 4), "mfma_products_per_flop": 3,
-                         "mfma_work_frac": round(3 * gm_tf / PEAK_TFLOPS[mixed], 4),
-                         "traffic": profiled_traffic(tuned[mixed]["chosen"], mixed, L, T0, nlags),
-                         "kernel": f"S(q,t) GEMM, {mixed}, {tuned[mixed]['chosen']}"},
+            "roofline": {"bound": "mfma", "achieved": round(gm_tf, 3), "peak": PEAK_TFLOPS[prec], "unit": "TFLOP/s",
+                         "frac": round(gm_tf / PEAK_TFLOPS[prec], 4),
+                         "mfma_products_per_flop": PRODUCTS_PER_FLOP[prec],
+                         "mfma_work_frac": round(FP16_WORK_PER_FLOP[prec] * gm_tf / PEAK_TFLOPS[prec], 4),
+                         "traffic": profiled_traffic(tuned[prec]["chosen"], prec, L, T0, nlags),
+                         "kernel": f"S(q,t) GEMM, {prec}, {tuned[prec]['chosen']}"},
             "err_vs_fp64": {"max": float(e.max()), "rms": float(np.sqrt((e ** 2).mean())),
                             "floor": "relative to max_q |S(q,tau)| of each lag, same snapshots"}}
         if args.precision == "fp64":
-            res["mixed_vs_fp64"] = round((rk_ms + sqt_ms) / (stm["rk4"] + sqtm_ms), 3)
-        log(f"  mixed pass: {stm}")
+            leg["vs_fp64"] = round((rk_ms + sqt_ms) / (stm["rk4"] + sqtm_ms), 3)
+        log(f"  mixed pass {prec}: {stm}")
+        return leg, stm
+
+    legs = {}
+    if mixed:
+        res["mixed_precision"], legs[mixed] = mixed_leg(mixed)
+        if "vs_fp64" in res["mixed_precision"]:
+            res["mixed_vs_fp64"] = res["mixed_precision"].pop("vs_fp64")
+    if also:
+        res["mixed_precision_" + also], legs[also] = mixed_leg(also)
 
     # calibration: the vendor library's plain fp16 GEMM (torch.matmul -> hipBLASLt) at one lag's
     # shape with K padded to a multiple of 64 (the library's faster shape: 16384 x 16384 x 1200
     # runs ~15% below x 1216), fp16 output, no epilogue, against the split kernel's MFMA work
     # rate (three products per useful flop, over the K its MFMAs cover: K rounded up to the
     # MFMA depth of 16)
-    split_prec = args.precision if split else mixed
-    if split_prec in ("fp16x2", "bf16x3"):
-        split_tf = gemm_tflops if split else gemm_flop / stm["gemm"] / 1e9
+    split_prec = args.precision if args.precision in ("fp16x2", "bf16x3") else next(
+        (m for m in (mixed, also) if m in ("fp16x2", "bf16x3")), None)
+    if split_prec:
+        split_tf = gemm_tflops if split_prec == args.precision else gemm_flop / legs[split_prec]["gemm"] / 1e9
         kpad = (3 * T0 + 63) // 64 * 64
         kexec = (3 * T0 + 15) // 16 * 16
         xa = torch.randn(ns, kpad, device="cuda", dtype=torch.float16)

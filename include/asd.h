@@ -59,7 +59,9 @@ typedef enum asd_precision {
     ASD_FP64 = 0,   /* v_mfma_f64_16x16x4f64, FP64 accumulate                         */
     ASD_FP32 = 1,   /* v_mfma_f32_16x16x4f32, FP32 accumulate                         */
     ASD_BF16X3 = 2, /* a = a_hi + a_lo (bf16); a_hi b_hi + a_hi b_lo + a_lo b_hi, FP32 acc */
-    ASD_FP16X2 = 3  /* a = a_hi + 2^-11 a_lo (fp16); hi and lo products in two FP32 accs */
+    ASD_FP16X2 = 3, /* a = a_hi + 2^-11 a_lo (fp16); hi and lo products in two FP32 accs */
+    ASD_FP16F8 = 4  /* the same split, with the cross products a_hi b_lo + a_lo b_hi formed
+                     * from fp8 (e4m3) roundings of the four parts on the fp8 MFMA            */
 } asd_precision;
 
 /* --------------------------------------------------- host-side helpers --- */
@@ -229,7 +231,8 @@ int asd_sqt_num_configs(asd_precision prec);
 const char *asd_sqt_config_name(asd_precision prec, int cfg);
 /* force a configuration for a precision (-1 = automatic: the one the autotuner recorded
  * for (L, K), else the precision's default, the direct-to-LDS ring kernel the autotuner
- * picks on most lattices: FP64 / FP32 config 3, BF16X3 / FP16X2 config 5) */
+ * picks on most lattices: FP64 / FP32 config 3, BF16X3 / FP16X2 config 5, FP16F8 config 0;
+ * FP16F8 runs on the direct-to-LDS kernels only and has three configurations) */
 int asd_sqt_set_config(asd_ctx *ctx, asd_precision prec, int cfg);
 
 /* ------------------------------------------------------------ adjoint --- */

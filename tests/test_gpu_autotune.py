@@ -16,7 +16,7 @@ def gxmg(L):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec,bound", [("fp16x2", 1e-5), ("fp64", 1e-10), ("bf16x3", 5e-4)])
+@pytest.mark.parametrize("prec,bound", [("fp16x2", 1e-5), ("fp16f8", 1e-5), ("fp64", 1e-10), ("bf16x3", 5e-4)])
 def test_autotuned_choice_is_recorded_and_used(gctx, orc, prec, bound):
     torch = need_gpu()
     import asd_amd

@@ -123,18 +123,21 @@ def test_batching_workload_full_size(gctx):
         assert torch.equal(spins[t], s1), (t, L)
         assert torch.equal(snaps[t], sn1), (t, L)
     say("  batched trajectories equal their single runs")
-    for L in sorted(set(Ls)):
-        gctx.autotune(L, T0, "fp16x2")
-    worst = 0.0
+    for prec in ("fp16x2", "fp16f8"):
+        for L in sorted(set(Ls)):
+            gctx.autotune(L, T0, prec)
+    worst = {"fp16x2": 0.0, "fp16f8": 0.0}
     for t, L in enumerate(Ls):
         q = [(k, 0) for k in range(L // 2)] + [(L // 2, k) for k in range(L // 2)] + \
             [(L // 2 - k, L // 2 - k) for k in range(L // 2)]
-        got = gctx.sqt(L, snaps[t], T0, nl, q, "fp16x2")
         ref = _separable(snaps[t].cpu().numpy(), L, T0, nl, q)
-        e = float((np.abs(got - ref) / np.abs(ref).max(axis=1, keepdims=True)).max())
-        worst = max(worst, e)
-        assert e <= 1e-5, (t, L, e)
-    say(f"  fp16x2 S(q,tau) of all {len(Ls)} trajectories: worst max error {worst:.3e}")
+        for prec in worst:
+            got = gctx.sqt(L, snaps[t], T0, nl, q, prec)
+            e = float((np.abs(got - ref) / np.abs(ref).max(axis=1, keepdims=True)).max())
+            worst[prec] = max(worst[prec], e)
+            assert e <= 1e-5, (t, L, prec, e)
+    say(f"  S(q,tau) of all {len(Ls)} trajectories, worst max error: fp16x2 {worst['fp16x2']:.3e}, "
+        f"fp16f8 {worst['fp16f8']:.3e}")
 
 
 @pytest.mark.gpu

@@ -33,7 +33,7 @@ def test_plain_L128_sqt_repeats_bit_for_bit(gctx):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x2", "fp64"])
+@pytest.mark.parametrize("prec", ["fp32", "bf16x3", "fp16x2", "fp16f8", "fp64"])
 def test_every_tile_configuration_repeats(gctx, prec):
     """Each configuration of the precision twice on the same snapshots (L = 40: ragged tiles)."""
     torch = need_gpu()

@@ -93,7 +93,7 @@ def test_plain_L128_full_size_against_separable_identity(gctx):
     snaps = run_snapshots(gctx, L, 1, 8000, 10, J=1.0, D=0.1)
     q = gxmg(L)
     ref = separable(snaps.cpu().numpy(), L, T0, range(nl), q)
-    for prec, bound in (("fp64", 1e-10), ("fp16x2", 1e-5)):
+    for prec, bound in (("fp64", 1e-10), ("fp16x2", 1e-5), ("fp16f8", 1e-5)):
         got = gctx.sqt(L, snaps, T0, nl, q, prec)
         mx, rms = err_stats(got, ref)
         say(f"  {prec}: max {mx:.3e} rms {rms:.3e} vs the separable identity")
@@ -102,15 +102,15 @@ def test_plain_L128_full_size_against_separable_identity(gctx):
 
 # Bounds at the precision workload (max over q of the error relative to max_q |S(q,tau)| per lag,
 # against the separable identity, which equals FP64 here to ~1e-14): FP32 1e-5 (measured
-# ~1.6e-7), FP16X2 1e-5 (~1.2e-6), BF16X3 1.5e-4 (~7e-5: bf16 keeps 16 bits of each spin
+# ~1.6e-7), FP16X2 and FP16F8 1e-5 (~1.2e-6, ~1.5e-6), BF16X3 1.5e-4 (~7e-5: bf16 keeps 16 bits of each spin
 # component in hi + lo, and its single FP32 accumulator drops the cross terms' low bits).
-PRECISION_BOUNDS = {"fp32": 1e-5, "fp16x2": 1e-5, "bf16x3": 1.5e-4}
+PRECISION_BOUNDS = {"fp32": 1e-5, "fp16x2": 1e-5, "fp16f8": 1e-5, "bf16x3": 1.5e-4}
 
 
 @pytest.mark.gpu
 def test_precision_L256_full_grid_against_separable_identity(gctx):
     L, T0, nl = 256, 200, 200
-    say(f"full size: L={L}, 4000 steps, T0={T0}, lags={nl}, full grid, fp32 / bf16x3 / fp16x2 / fp64")
+    say(f"full size: L={L}, 4000 steps, T0={T0}, lags={nl}, full grid, fp32 / fp16x2 / fp16f8 / bf16x3 / fp64")
     snaps = run_snapshots(gctx, L, 2, 4000, 10, J=1.0, D=0.05, Bz=0.2)
     q = [(a, b) for a in range(L) for b in range(L)]
     lags = [0, 1, 100, 199]

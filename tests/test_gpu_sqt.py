@@ -62,7 +62,7 @@ def test_fp64_matches_separable(gctx, orc, L, T0, nlags):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec,bound", [("fp32", 1e-5), ("bf16x3", 1e-5), ("fp16x2", 1e-5)])
+@pytest.mark.parametrize("prec,bound", [("fp32", 1e-5), ("bf16x3", 1e-5), ("fp16x2", 1e-5), ("fp16f8", 1e-5)])
 def test_reduced_precision_error(gctx, orc, prec, bound):
     """Error of the MFMA variants against the GPU FP64 result on identical snapshots,
     per (q, tau) relative to max_q |S(q, tau)| (the floor)."""
@@ -194,11 +194,11 @@ def test_large_lattices_match_separable(gctx, orc, L, prec, tol):
 # max |S| over all (q, tau).  BF16X3 keeps about 17 significant bits per operand, so
 # with few origins to average over its error reaches 3e-5 to 1.2e-4 (5 origins); a
 # wrong fragment layout or a dropped tile gives errors of order 1.
-SMALL_SAMPLE_TOL = {"fp64": 1e-10, "fp32": 1e-5, "bf16x3": 2e-4, "fp16x2": 1e-5}
+SMALL_SAMPLE_TOL = {"fp64": 1e-10, "fp32": 1e-5, "bf16x3": 2e-4, "fp16x2": 1e-5, "fp16f8": 2e-4}
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3", "fp16x2"])
+@pytest.mark.parametrize("prec", ["fp64", "fp32", "bf16x3", "fp16x2", "fp16f8"])
 def test_every_kernel_config_matches_oracle(gctx, orc, prec):
     """Every tile / wave layout the autotuner may pick gives the oracle's S(q,tau):
     lattices whose site count is not a multiple of any tile edge (400, 1296) and
@@ -258,6 +258,69 @@ def test_direct_to_lds_ring_equals_register_staged_at_full_depth(gctx, prec):
         gctx.set_config(prec, -1)
     assert np.all(np.isfinite(out[0])) and np.abs(out[0]).max() > 0
     assert np.array_equal(out[0], out[1])
+
+
+def e4m3(x):
+    """Nearest OCP FP8 e4m3 value (bias 7, 3 mantissa bits; below 2^-6 the multiples of 2^-9),
+    ties to even, for |x| <= 448: the rounding FP16F8 applies to its cross-product parts."""
+    x = np.asarray(x, dtype=np.float64)
+    ax = np.abs(x)
+    e = np.maximum(np.floor(np.log2(np.where(ax > 0, ax, 1.0))), -6.0)
+    step = 2.0 ** (e - 3)
+    return np.round(x / step) * step
+
+
+def test_e4m3_rounding_known_values():
+    """The e4m3 reference rounding on values whose codes are fixed by the format: the largest
+    finite 448, the smallest normal 2^-6 and subnormal 2^-9, ties to even at both ends of a
+    binade, a carry into the next binade, and a subnormal tie."""
+    cases = {448.0: 448.0, 1.0: 1.0, 1.0625: 1.0, 1.1875: 1.25, 1.9375: 2.0, 2.0 ** -6: 2.0 ** -6,
+             2.0 ** -9: 2.0 ** -9, 1.5 * 2.0 ** -9: 2.0 ** -8, 2.5 * 2.0 ** -9: 2.0 ** -8,
+             0.3: 0.3125, -0.3: -0.3125, 0.0: 0.0, 2.0 ** -11: 0.0}
+    for x, want in cases.items():
+        assert e4m3(x) == want, (x, e4m3(x), want)
+
+
+def sqt_connected(L, X, Y, T0, nlags, q):
+    """(1 / (Ns T0)) sum_t0 sum_c F_X(t0 + tau) conj F_Y(t0): the connected part of S(q,tau)
+    for the operand arrays X (A side) and Y (B side), with the oracle's phase convention
+    e^{-2 pi i (qx (i / L) + qy (i % L)) / L} (orc_sqt_separable, separable form)."""
+    ns = L * L
+    r, c = np.divmod(np.arange(ns), L)
+    qa = np.asarray(q)
+    ph = np.exp(-2j * np.pi * (np.outer(qa[:, 0], r) + np.outer(qa[:, 1], c)) / L)  # [nq][Ns]
+    FX = np.einsum("qi,tic->tqc", ph, X)
+    FY = np.einsum("qi,tic->tqc", ph, Y)
+    out = np.empty((nlags, len(q)), dtype=complex)
+    for tau in range(nlags):
+        out[tau] = np.einsum("tqc,tqc->q", FX[tau:tau + T0], FY[:T0].conj()) / (ns * T0)
+    return out
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("L,T0,nlags", [(20, 30, 4), (36, 25, 3)])
+def test_fp16f8_matches_its_arithmetic_on_the_cpu(gctx, orc, L, T0, nlags):
+    """FP16F8 against a CPU restatement of its own arithmetic: hi = f16(v), the cross parts
+    e4m3(hi) and e4m3((v - hi) 2^11), products hi.hi + 2^-11 (hi8.lo8 + lo8.hi8).  The
+    connected part is linear in the operand products, so the restatement is the oracle's FP64
+    S(q,tau) with the connected part of the exact snapshots replaced by those of the three
+    products (separable form).  The GPU sums in FP32 where this sums in FP64, so the two agree
+    to FP32 accumulation (bound 1e-6 relative to max_q |S| per lag), several times closer than
+    FP16F8 is to FP64; a wrong byte pairing or rounding would give errors of the fp8 size."""
+    torch = need_gpu()
+    snaps = trajectory(orc, L, T0 + nlags - 1, D=0.1, Bz=0.05)
+    q = gxmg(L)
+    v = snaps.astype(np.float32)
+    hi = v.astype(np.float16).astype(np.float32)
+    lo = (v - hi) * np.float32(2048.0)
+    H, H8, R8 = hi.astype(np.float64), e4m3(hi), e4m3(lo) / 2048.0
+    ref64 = orc.sqt_gemm(L, snaps, T0, nlags, q)
+    emul = (ref64 - sqt_connected(L, snaps, snaps, T0, nlags, q) + sqt_connected(L, H, H, T0, nlags, q)
+            + sqt_connected(L, H8, R8, T0, nlags, q) + sqt_connected(L, R8, H8, T0, nlags, q))
+    got = gctx.sqt(L, torch.tensor(snaps, device="cuda"), T0, nlags, q, "fp16f8")
+    e_emul, e_64 = max_rel(got, emul), max_rel(got, ref64)
+    say(f"sqt fp16f8 L={L} K={3 * T0}: max rel err vs its CPU restatement {e_emul:.3e}, vs FP64 {e_64:.3e}")
+    assert e_emul <= 1e-6 and e_emul <= 0.25 * e_64
 
 
 @pytest.mark.gpu
@@ -358,7 +421,7 @@ def test_invalid_sizes_are_rejected(gctx, orc):
         gctx.sqt(L, snaps, 3, 2, [(0, L)], "fp64")       # q index out of [0, L)
     with pytest.raises(ValueError):                      # snapshots of another lattice size
         gctx.sqt(L, snaps[:, : L * L // 2].contiguous(), 3, 2, q, "fp64")
-    for prec in (-1, 4):                                 # no such precision
+    for prec in (-1, 5):                                 # no such precision
         with pytest.raises(asd_amd.AsdError):
             gctx.sqt(L, snaps, 3, 2, q, prec)
         with pytest.raises(asd_amd.AsdError):

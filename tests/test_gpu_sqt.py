@@ -324,6 +324,35 @@ def test_fp16f8_matches_its_arithmetic_on_the_cpu(gctx, orc, L, T0, nlags):
 
 
 @pytest.mark.gpu
+def test_fp16f8_replica_form_matches_its_arithmetic(gctx, orc):
+    """The replica form (A = replicas now, B = replicas at 0: two different arrays) in FP16F8
+    against the CPU restatement of its roundings, built on the GPU's FP64 replica result
+    (which test_replica_form_matches_reference_kernel pins to the reference kernel)."""
+    torch = need_gpu()
+    L, nrep = 16, 40
+    zero = np.stack([orc.init_spins(70 + r, L * L) for r in range(nrep)])
+    now, _ = orc.run(orc.params(dt=0.01, D=0.1), L, zero, 25)
+    q = gxmg(L)
+    dn, dz = torch.tensor(now, device="cuda"), torch.tensor(zero, device="cuda")
+    s64 = gctx.sqt_replicas(L, dn, dz, q, "fp64")
+    got = gctx.sqt_replicas(L, dn, dz, q, "fp16f8")
+
+    def parts(x):
+        v = x.astype(np.float32)
+        hi = v.astype(np.float16).astype(np.float32)
+        return hi.astype(np.float64), e4m3(hi), e4m3((v - hi) * np.float32(2048.0)) / 2048.0
+
+    (Hn, H8n, R8n), (Hz, H8z, R8z) = parts(now), parts(zero)
+    conn = lambda X, Y: sqt_connected(L, X, Y, nrep, 1, q)[0]
+    emul = s64 - conn(now, zero) + conn(Hn, Hz) + conn(H8n, R8z) + conn(R8n, H8z)
+    scale = np.abs(s64).max()
+    e_emul, e_64 = np.abs(got - emul).max() / scale, np.abs(got - s64).max() / scale
+    say(f"sqt_replicas fp16f8 L={L} K={3 * nrep}: max rel err vs its CPU restatement {e_emul:.3e}, "
+        f"vs FP64 {e_64:.3e}")
+    assert e_emul <= 1e-6 and e_emul <= 0.25 * e_64
+
+
+@pytest.mark.gpu
 @pytest.mark.parametrize("L,T0,nlags,qset", [(16, 20, 8, "full"), (20, 12, 6, "gxmg"), (32, 10, 4, "gxmg")])
 def test_unfused_path_matches_fused_and_oracle(gctx, orc, L, T0, nlags, qset):
     """asd_sqt_unfused (rocBLAS GEMM into an explicit Ns x Ns matrix, outer subtract,

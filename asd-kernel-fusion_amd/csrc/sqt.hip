@@ -1330,13 +1330,23 @@ static const SqtCfg kSplitCfgs[] = {
     {2, 2, 16, "128x128 tiles, 4 waves of 64x64, group 16"},
     {2, 4, 8, "128x256 tiles, 4 waves of 64x128, group 8"},
     {4, 2, 8, "256x128 tiles, 4 waves of 128x64, group 8"},
+    // fewer tile-rows per group: FP16F8 at K = 1200, 121-124 ms per 100 lags against 124-129 at group 4
+    {2, 2, 2, "256x128 tiles, 8 waves of 64x64, 3-stage LDS ring, group 2", 4, 2, 3},
+    {2, 2, 1, "256x128 tiles, 8 waves of 64x64, 3-stage LDS ring, group 1", 4, 2, 3},
 };
-// FP16X2 keeps two accumulators: the 64x128 / 128x64 register-staged wave tiles do not
-// fit the register file, so only the first six are offered
-constexpr int kNumSplitFp16 = 6;
-// FP16F8 runs on the direct-to-LDS kernels only: its configurations, as rows of kSplitCfgs
-static const int kF8Cfgs[] = {5, 3, 4};
-constexpr int kNumF8 = sizeof(kF8Cfgs) / sizeof(kF8Cfgs[0]);
+constexpr int kNumSplit = sizeof(kSplitCfgs) / sizeof(kSplitCfgs[0]);
+// The rows each split precision is offered, in its own configuration order.  BF16X3: all.
+// FP16X2 keeps two accumulators, so the 64x128 / 128x64 register-staged wave tiles (rows 6-8)
+// do not fit the register file.  FP16F8 runs on the direct-to-LDS kernels only.
+static const int kFp16Rows[] = {0, 1, 2, 3, 4, 5, 9, 10};
+static const int kF8Rows[] = {5, 3, 4, 9, 10};
+constexpr int kNumFp16 = sizeof(kFp16Rows) / sizeof(kFp16Rows[0]);
+constexpr int kNumF8 = sizeof(kF8Rows) / sizeof(kF8Rows[0]);
+// row of kSplitCfgs for configuration cfg of a split precision
+static int split_row(asd_precision prec, int cfg)
+{
+    return prec == ASD_FP16X2 ? kFp16Rows[cfg] : prec == ASD_FP16F8 ? kF8Rows[cfg] : cfg;
+}
 static const SqtCfg kKmajorCfgs[] = {
     {2, 2, 8, "128x128 tiles, 64x64 per wave, group 8"},
     {2, 2, 16, "128x128 tiles, 64x64 per wave, group 16"},
@@ -1344,12 +1354,11 @@ static const SqtCfg kKmajorCfgs[] = {
     {2, 2, 8, "128x128 tiles, 64x64 per wave, 4-stage direct-to-LDS ring, group 8", 2, 2, 4},
     {2, 2, 16, "128x128 tiles, 64x64 per wave, 4-stage direct-to-LDS ring, group 16", 2, 2, 4},
 };
-constexpr int kNumSplit = sizeof(kSplitCfgs) / sizeof(kSplitCfgs[0]);
 constexpr int kNumKmajor = sizeof(kKmajorCfgs) / sizeof(kKmajorCfgs[0]);
 
 // the configuration an untuned (L, K) takes: the one the autotuner picks on most lattices (the
 // direct-to-LDS rings)
-static_assert(kNumKmajor > 3 && kNumSplitFp16 > 5, "the default configurations exist for every precision");
+static_assert(kNumKmajor > 3 && kNumFp16 > 5 && kNumSplit > 5, "the default configurations exist for every precision");
 static int default_config(asd_precision prec)
 {
     return prec <= ASD_FP32 ? 3 : prec == ASD_FP16F8 ? 0 : 5;
@@ -1357,14 +1366,12 @@ static int default_config(asd_precision prec)
 
 int sqt_num_configs(asd_precision prec)
 {
-    return prec <= ASD_FP32 ? kNumKmajor
-                            : prec == ASD_FP16X2 ? kNumSplitFp16 : prec == ASD_FP16F8 ? kNumF8 : kNumSplit;
+    return prec <= ASD_FP32 ? kNumKmajor : prec == ASD_FP16X2 ? kNumFp16 : prec == ASD_FP16F8 ? kNumF8 : kNumSplit;
 }
 const char *sqt_config_name(asd_precision prec, int cfg)
 {
     if (cfg < 0 || cfg >= sqt_num_configs(prec)) return "invalid";
-    return prec <= ASD_FP32 ? kKmajorCfgs[cfg].name
-                            : prec == ASD_FP16F8 ? kSplitCfgs[kF8Cfgs[cfg]].name : kSplitCfgs[cfg].name;
+    return prec <= ASD_FP32 ? kKmajorCfgs[cfg].name : kSplitCfgs[split_row(prec, cfg)].name;
 }
 
 using GemmKernel = void (*)(GemmArgs);
@@ -1790,11 +1797,11 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
         ga.Kpad = (int)Kpad;
         ASD_TRY(mark(ctx, 1));
         if (prec == ASD_BF16X3)
-            launch_split<ASD_BF16X3>(kSplitCfgs[cfg], ga, st, ctx->device);
+            launch_split<ASD_BF16X3>(kSplitCfgs[split_row(prec, cfg)], ga, st, ctx->device);
         else if (prec == ASD_FP16X2)
-            launch_split<ASD_FP16X2>(kSplitCfgs[cfg], ga, st, ctx->device);
+            launch_split<ASD_FP16X2>(kSplitCfgs[split_row(prec, cfg)], ga, st, ctx->device);
         else
-            launch_split<ASD_FP16F8>(kSplitCfgs[kF8Cfgs[cfg]], ga, st, ctx->device);
+            launch_split<ASD_FP16F8>(kSplitCfgs[split_row(prec, cfg)], ga, st, ctx->device);
     }
     ASD_HIP(hipGetLastError());
     ASD_TRY(mark(ctx, 2));

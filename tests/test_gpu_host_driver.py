@@ -71,6 +71,14 @@ def test_pipeline_snapshot_file_round_trip(orc, tmp_path):
     a = np.fromfile(tmp_path / "a.bin", dtype="<f8")
     b = np.fromfile(tmp_path / "b.bin", dtype="<f8")
     assert a.shape == b.shape and np.abs(a - b).max() <= 1e-13 * np.abs(a).max()
+    # the split precisions through the driver's -p flag, on the same snapshots (K = 36: a small
+    # sample, bound 2e-4 as for the library's small-sample checks)
+    for prec in ("fp16x2", "fp16f8"):
+        r = subprocess.run([BIN, *common[:-1], prec, "-R", "snaps.bin", "-O", f"{prec}.bin"], cwd=tmp_path,
+                           capture_output=True, text=True, timeout=120)
+        assert r.returncode == 0, r.stderr
+        c = np.fromfile(tmp_path / f"{prec}.bin", dtype="<f8")
+        assert c.shape == a.shape and np.abs(c - a).max() <= 2e-4 * np.abs(a).max(), prec
 
     (tmp_path / "short.bin").write_bytes((tmp_path / "snaps.bin").read_bytes()[:-8])
     r = subprocess.run([BIN, *common, "-R", "short.bin"], cwd=tmp_path, capture_output=True, text=True, timeout=120)

@@ -65,6 +65,11 @@ struct FixScale {
     unsigned long long vmax_bits;  // bits of max |snapshot component| (non-negative doubles order as integers)
     double m0, m1;                 // 2^s0 (first word) and 2^s1 (second word, relative to the first)
     double d0, d1;                 // 2^-s0 and 2^-(s0 + s1) for the decoder
+    // the split precisions pack v ps (a power of two putting max |v| in [64, 128), inside the
+    // normal range of fp16 and e4m3 with room below: |v| of any size keeps their relative
+    // precision) and convert their products with ms = 2^s0 / ps^2, so the bin words equal those of
+    // the unscaled products
+    double ps, ms;
     int bad;                       // a snapshot component is NaN or infinite
 };
 
@@ -108,13 +113,15 @@ __global__ void fix_scale_kernel(FixScale *f, int K, int Ns)
     int clog_ns = 0;
     while ((1L << clog_ns) < (long)Ns) clog_ns++;
     int e_el = 0;  // |G_ij| <= K M^2 <= 2^e_el
+    int ep = 0;    // pack scale exponent of the split precisions
     if (!(M <= 1.7976931348623157e308)) {
         f->bad = 1;  // NaN or Inf
     } else if (M > 0.0) {
         int eM = 0, eK = 0;
-        (void)frexp(M, &eM);
+        (void)frexp(M, &eM);  // M < 2^eM <= 2 M
         (void)frexp((double)K, &eK);
         e_el = 2 * eM + eK;
+        ep = max(-480, min(480, 7 - eM));
     }
     int s0 = min(50, 62 - clog_ns) - e_el;
     s0 = max(-1000, min(1000, s0));
@@ -123,6 +130,8 @@ __global__ void fix_scale_kernel(FixScale *f, int K, int Ns)
     f->m1 = ldexp(1.0, s1);
     f->d0 = ldexp(1.0, -s0);
     f->d1 = ldexp(1.0, -s0 - s1);
+    f->ps = ldexp(1.0, ep);
+    f->ms = ldexp(1.0, max(-1000, min(1000, s0 - 2 * ep)));
 }
 
 // one product value into the bin words of LDS (word w of bin b at sb[w * nbins + b])
@@ -195,8 +204,9 @@ template <int PREC>
 __global__ __launch_bounds__(256) void pack_split_kernel(const double *__restrict__ snaps, long tstride, int nt,
                                                          int Ns, uint16_t *__restrict__ hi,
                                                          uint16_t *__restrict__ lo, long nkb, long copy_stride,
-                                                         int nshift, int bside)
+                                                         int nshift, int bside, const FixScale *__restrict__ fix)
 {
+    const double ps = fix->ps;
     const int i = blockIdx.x * 32 + threadIdx.x / 8, e = threadIdx.x % 8;
     const long kb = blockIdx.y;
     const long o = ((long)(i / 16) * nkb + kb) * 128 + (i % 16) * 8 + e;
@@ -204,7 +214,7 @@ __global__ __launch_bounds__(256) void pack_split_kernel(const double *__restric
         const long kk = kb * 8 + e + r;
         const long t = kk / 3;
         const int c = (int)(kk - 3 * t);
-        const float v = (t < nt && i < Ns) ? (float)snaps[t * tstride + 3L * i + c] : 0.0f;
+        const float v = (t < nt && i < Ns) ? (float)(snaps[t * tstride + 3L * i + c] * ps) : 0.0f;
         uint16_t h, l;
         if (PREC == ASD_BF16X3) {
             __hip_bfloat16 bh = __float2bfloat16(v);
@@ -1023,7 +1033,7 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_split_kernel(GemmArgs
 
     bin_split_tile<WM, WN, DUAL>(acc, acc2, reinterpret_cast<unsigned long long *>(&sA[0][0][0][0]), i0, j0,
                                  wi * WM * 32, wj * WN * 32, BM, BN, args.Ns, args.L,
-                                 args.bins + (long)w.lag * args.Ns, args.fix->m0);
+                                 args.bins + (long)w.lag * args.Ns, args.fix->ms);
     }
 }
 
@@ -1305,7 +1315,7 @@ __global__ __launch_bounds__(64 * WVM * WVN) void sqt_gemm_dma_kernel(GemmArgs a
     __syncthreads();
     bin_split_tile_unwrapped<WM, WN, DUAL>(acc, acc2, reinterpret_cast<unsigned long long *>(lds), i0, j0,
                                            wi * WM * 32, wj * WN * 32, BM, BN, args.Ns, args.L,
-                                           args.bins + (long)w.lag * args.Ns, args.fix->m0);
+                                           args.bins + (long)w.lag * args.Ns, args.fix->ms);
     }
 }
 
@@ -1785,9 +1795,11 @@ int sqt_run(asd_ctx *ctx, int L, const double *d_snaps_a, const double *d_snaps_
                       : prec == ASD_FP16X2 ? pack_split_kernel<ASD_FP16X2>
                                            : pack_split_kernel<ASD_FP16F8>;
         hipLaunchKernelGGL(packer, dim3((unsigned)(ldi / 32), (unsigned)nkb), pb, 0, st, d_snaps_a, tstride, nt_a,
-                           Ns, ctx->opA.as<uint16_t>(), ctx->opA2.as<uint16_t>(), nkbs, cs, 8, 0);
+                           Ns, ctx->opA.as<uint16_t>(), ctx->opA2.as<uint16_t>(), nkbs, cs, 8, 0,
+                           (const FixScale *)fix);
         hipLaunchKernelGGL(packer, dim3((unsigned)(ldi / 32), (unsigned)(Kpad / 8)), pb, 0, st, d_snaps_b, tstride,
-                           T0, Ns, ctx->opB.as<uint16_t>(), ctx->opB2.as<uint16_t>(), nkbs, cs, 1, 1);
+                           T0, Ns, ctx->opB.as<uint16_t>(), ctx->opB2.as<uint16_t>(), nkbs, cs, 1, 1,
+                           (const FixScale *)fix);
         ga.copy_stride = cs;
         ga.A = ctx->opA.ptr;
         ga.A2 = ctx->opA2.ptr;

@@ -324,6 +324,29 @@ def test_fp16f8_matches_its_arithmetic_on_the_cpu(gctx, orc, L, T0, nlags):
 
 
 @pytest.mark.gpu
+@pytest.mark.parametrize("prec", ["fp16x2", "fp16f8", "bf16x3"])
+def test_split_precisions_scale_with_the_spin_length(gctx, orc, prec):
+    """The split precisions pack the snapshots scaled by a power of two that puts their largest
+    component in [64, 128), and convert the products back exactly, so spins of any length keep
+    the same relative error: by a power of two the result scales bit for bit (2^12 is beyond fp8's
+    range of 448 and 2^-16 puts every component among fp16's subnormals, unscaled), and by 1000
+    it stays within the precision's bound of the FP64 result."""
+    torch = need_gpu()
+    L, T0, nlags = 32, 100, 6
+    snaps = torch.tensor(trajectory(orc, L, T0 + nlags - 1, D=0.1), device="cuda")
+    q = gxmg(L)
+    base = gctx.sqt(L, snaps, T0, nlags, q, prec)
+    for sc in (2.0 ** 12, 2.0 ** -16):
+        got = gctx.sqt(L, snaps * sc, T0, nlags, q, prec)
+        assert np.array_equal(got, base * sc * sc), (prec, sc)
+    big = snaps * 1000.0
+    e = max_rel(gctx.sqt(L, big, T0, nlags, q, prec), gctx.sqt(L, big, T0, nlags, q, "fp64"))
+    e1 = max_rel(base, gctx.sqt(L, snaps, T0, nlags, q, "fp64"))
+    say(f"sqt {prec} L={L} K={3 * T0}: spins x1000 max rel err vs FP64 {e:.3e} (x1: {e1:.3e})")
+    assert e <= (1.5e-4 if prec == "bf16x3" else 1e-5)
+
+
+@pytest.mark.gpu
 def test_fp16f8_replica_form_matches_its_arithmetic(gctx, orc):
     """The replica form (A = replicas now, B = replicas at 0: two different arrays) in FP16F8
     against the CPU restatement of its roundings, built on the GPU's FP64 replica result

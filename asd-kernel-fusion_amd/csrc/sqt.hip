@@ -1368,10 +1368,10 @@ constexpr int kNumKmajor = sizeof(kKmajorCfgs) / sizeof(kKmajorCfgs[0]);
 
 // the configuration an untuned (L, K) takes: the one the autotuner picks on most lattices (the
 // direct-to-LDS rings)
-static_assert(kNumKmajor > 3 && kNumFp16 > 5 && kNumSplit > 5, "the default configurations exist for every precision");
+static_assert(kNumKmajor > 3 && kNumFp16 > 5 && kNumSplit > 5 && kNumF8 > 3, "the default configurations exist for every precision");
 static int default_config(asd_precision prec)
 {
-    return prec <= ASD_FP32 ? 3 : prec == ASD_FP16F8 ? 0 : 5;
+    return prec <= ASD_FP32 ? 3 : prec == ASD_FP16F8 ? 3 : 5;
 }
 
 int sqt_num_configs(asd_precision prec)

@@ -231,7 +231,7 @@ int asd_sqt_num_configs(asd_precision prec);
 const char *asd_sqt_config_name(asd_precision prec, int cfg);
 /* force a configuration for a precision (-1 = automatic: the one the autotuner recorded
  * for (L, K), else the precision's default, the direct-to-LDS ring kernel the autotuner
- * picks on most lattices: FP64 / FP32 config 3, BF16X3 / FP16X2 config 5, FP16F8 config 0;
+ * picks on most lattices: FP64 / FP32 / FP16F8 config 3, BF16X3 / FP16X2 config 5;
  * FP16F8 runs on the direct-to-LDS kernels only; asd_sqt_num_configs gives each count) */
 int asd_sqt_set_config(asd_ctx *ctx, asd_precision prec, int cfg);
 

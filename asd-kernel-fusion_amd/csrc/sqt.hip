@@ -178,7 +178,9 @@ __global__ void pack_kmajor_kernel(const double *__restrict__ snaps, long tstrid
 // 8 halves (16 B): element (i, k) at ((i / 16) nkb + k / 8) 128 + (i % 16) 8 + k % 8 halves,
 // so the 16 rows of one k-block are 256 contiguous bytes and a 16-row, 32-deep chunk (four
 // consecutive k-blocks) is one contiguous KiB from any k-block.
+// v is the snapshot component times the pack scale fix->ps (a power of two, exact).
 // BF16X3: hi = bf16(v), lo = bf16(v - hi).  FP16X2: hi = f16(v), lo = f16((v - hi) * 2^11).
+// FP16F8: hi as FP16X2, the lo half holds the fp8 cross-product bytes (below).
 // `nshift` shifted copies are written, copy r holding value(i, k + r) at (i, k), so a
 // window starting at any k offset o is read from copy o % 8 at the k-block (o - o % 8) / 8.
 // nkb: k-blocks per row group (rows >= Ns are zeros).  grid: (row groups of 32 rows,

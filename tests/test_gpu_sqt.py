@@ -281,6 +281,20 @@ def test_e4m3_rounding_known_values():
         assert e4m3(x) == want, (x, e4m3(x), want)
 
 
+def pack_scale(x):
+    """The split precisions' pack scale: the power of two 2^(7 - e) with max |x| < 2^e <= 2 max |x|,
+    which puts the largest component in [64, 128)."""
+    return 2.0 ** (7 - np.frexp(np.abs(x).max())[1])
+
+
+def fp16f8_parts(x, ps):
+    """FP16F8's three operand arrays for the components x, in x's units: hi = f16(v), its e4m3
+    rounding, and e4m3((v - hi) 2^11) 2^-11, with v = f32(x ps) as the pack forms it."""
+    v = (x * ps).astype(np.float32)
+    hi = v.astype(np.float16).astype(np.float32)
+    return hi / ps, e4m3(hi) / ps, e4m3((v - hi) * np.float32(2048.0)) / (2048.0 * ps)
+
+
 def sqt_connected(L, X, Y, T0, nlags, q):
     """(1 / (Ns T0)) sum_t0 sum_c F_X(t0 + tau) conj F_Y(t0): the connected part of S(q,tau)
     for the operand arrays X (A side) and Y (B side), with the oracle's phase convention
@@ -300,8 +314,9 @@ def sqt_connected(L, X, Y, T0, nlags, q):
 @pytest.mark.gpu
 @pytest.mark.parametrize("L,T0,nlags", [(20, 30, 4), (36, 25, 3)])
 def test_fp16f8_matches_its_arithmetic_on_the_cpu(gctx, orc, L, T0, nlags):
-    """FP16F8 against a CPU restatement of its own arithmetic: hi = f16(v), the cross parts
-    e4m3(hi) and e4m3((v - hi) 2^11), products hi.hi + 2^-11 (hi8.lo8 + lo8.hi8).  The
+    """FP16F8 against a CPU restatement of its own arithmetic: v = f32(x ps) with the pack scale,
+    hi = f16(v), the cross parts e4m3(hi) and e4m3((v - hi) 2^11), products
+    hi.hi + 2^-11 (hi8.lo8 + lo8.hi8), divided by ps^2.  The
     connected part is linear in the operand products, so the restatement is the oracle's FP64
     S(q,tau) with the connected part of the exact snapshots replaced by those of the three
     products (separable form).  The GPU sums in FP32 where this sums in FP64, so the two agree
@@ -310,10 +325,7 @@ def test_fp16f8_matches_its_arithmetic_on_the_cpu(gctx, orc, L, T0, nlags):
     torch = need_gpu()
     snaps = trajectory(orc, L, T0 + nlags - 1, D=0.1, Bz=0.05)
     q = gxmg(L)
-    v = snaps.astype(np.float32)
-    hi = v.astype(np.float16).astype(np.float32)
-    lo = (v - hi) * np.float32(2048.0)
-    H, H8, R8 = hi.astype(np.float64), e4m3(hi), e4m3(lo) / 2048.0
+    H, H8, R8 = fp16f8_parts(snaps, pack_scale(snaps))
     ref64 = orc.sqt_gemm(L, snaps, T0, nlags, q)
     emul = (ref64 - sqt_connected(L, snaps, snaps, T0, nlags, q) + sqt_connected(L, H, H, T0, nlags, q)
             + sqt_connected(L, H8, R8, T0, nlags, q) + sqt_connected(L, R8, H8, T0, nlags, q))
@@ -360,12 +372,8 @@ def test_fp16f8_replica_form_matches_its_arithmetic(gctx, orc):
     s64 = gctx.sqt_replicas(L, dn, dz, q, "fp64")
     got = gctx.sqt_replicas(L, dn, dz, q, "fp16f8")
 
-    def parts(x):
-        v = x.astype(np.float32)
-        hi = v.astype(np.float16).astype(np.float32)
-        return hi.astype(np.float64), e4m3(hi), e4m3((v - hi) * np.float32(2048.0)) / 2048.0
-
-    (Hn, H8n, R8n), (Hz, H8z, R8z) = parts(now), parts(zero)
+    ps = pack_scale(np.concatenate([now.ravel(), zero.ravel()]))
+    (Hn, H8n, R8n), (Hz, H8z, R8z) = fp16f8_parts(now, ps), fp16f8_parts(zero, ps)
     conn = lambda X, Y: sqt_connected(L, X, Y, nrep, 1, q)[0]
     emul = s64 - conn(now, zero) + conn(Hn, Hz) + conn(H8n, R8z) + conn(R8n, H8z)
     scale = np.abs(s64).max()

@@ -154,11 +154,11 @@ def test_full_grid_replica_form(gctx, orc):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp16x2", 1e-5)])
+@pytest.mark.parametrize("prec,tol", [("fp64", 1e-10), ("fp16x2", 1e-5), ("fp16f8", 2e-4)])
 def test_large_work_list_all_lags_computed(gctx, orc, prec, tol):
     """L = 256 with 70 lags is 18.4M tile work items (4.7e9 work-items if launched one
     workgroup per tile, beyond the 32-bit dispatch size); every lag must match the
-    separable oracle."""
+    separable oracle (FP16F8 at this K = 12 averages few fp8 roundings: the small-sample bound)."""
     torch = need_gpu()
     L, T0, nlags = 256, 4, 70
     snaps = np.stack([orc.init_spins(1000 + t, L * L) for t in range(T0 + nlags - 1)])
